@@ -1,0 +1,580 @@
+// DDM engine: owns the spectrum ring, lag accumulators and q-ring tables of
+// one analysis on one device, and implements the C ABI of ddm_amd.h.
+//
+// Per chunk of <= chunk_frames frames (runDDM's chunk loop,
+// src/DDM.cu:765-843):
+//   1. (host input) ROI rows -> pinned staging buffer -> async H2D on the
+//      copy stream; the compute stream waits on its event;
+//   2. tile cut + FFT of every scale into ring slots frame % ring_slots;
+//   3. lag engine over the chunk in blocks of lag_block frames.
+// The ring holds chunk_frames + max_lag spectra, so every lag of every
+// frame finds its partner (the reference's 3-chunk ring limits
+// lag <= chunk, src/DDM.cu:462).  Results (analyse_accums,
+// src/DDM.cu:72-146) run one azimuthal launch for every
+// (scale, tile, q, lag).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "ddm_internal.h"
+#include "ddm_kernels.h"
+
+using ddm::Params;
+
+namespace {
+
+enum MarkKind { MK_FFT = 0, MK_LAG = 1, MK_AZ = 2, MK_CHUNK = 3 };
+
+struct Mark {
+    hipEvent_t a, b;
+    int kind;
+};
+
+}  // namespace
+
+struct ddm_engine {
+    Params p;
+    ddm::RingTables rt;
+    int device = 0;
+    int lag_block = 32;
+    float *d_ring = nullptr;
+    float *d_acc = nullptr;
+    int *d_lags = nullptr;
+    ddm::AzScale *d_az = nullptr;
+    int *d_qoff = nullptr, *d_qcnt = nullptr, *d_idx = nullptr;
+    double *d_norm = nullptr, *d_out = nullptr;
+    uint8_t *d_stage[2] = {nullptr, nullptr};
+    uint8_t *h_stage[2] = {nullptr, nullptr};
+    bool stage_used[2] = {false, false};
+    int next_stage = 0;
+    hipStream_t st = nullptr, st_copy = nullptr;
+    hipEvent_t ev_up[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+    long long frames = 0, frames_since_reset = 0;
+    std::vector<long long> pairs;
+    size_t dev_bytes = 0, pinned_bytes = 0;
+    std::string err;
+    bool timing = false;
+    std::vector<Mark> pending;
+    std::vector<hipEvent_t> pool;
+    ddm_stage_stats stats{};
+    std::vector<float> chunk_ms;
+};
+
+#define HIP_TRY(eng, call)                                                                  \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            (eng)->err = std::string(#call) + ": " + hipGetErrorString(e_);                 \
+            return (e_ == hipErrorOutOfMemory) ? DDM_ENOMEM : DDM_EDEVICE;                  \
+        }                                                                                   \
+    } while (0)
+
+// Message of the last ddm_create failure on this thread (no engine exists
+// to hold it); returned by ddm_last_error(NULL).
+static thread_local std::string g_create_err;
+
+static int fail(ddm_engine *e, int code, const std::string &msg) {
+    if (e) e->err = msg;
+    return code;
+}
+
+static hipEvent_t take_event(ddm_engine *e) {
+    if (!e->pool.empty()) {
+        hipEvent_t ev = e->pool.back();
+        e->pool.pop_back();
+        return ev;
+    }
+    hipEvent_t ev = nullptr;
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    return ev;
+}
+
+static void mark_begin(ddm_engine *e, Mark *m, int kind) {
+    m->a = m->b = nullptr;
+    m->kind = kind;
+    if (!e->timing) return;
+    m->a = take_event(e);
+    m->b = take_event(e);
+    if (m->a) (void)hipEventRecord(m->a, e->st);
+}
+
+static void mark_end(ddm_engine *e, Mark *m) {
+    if (!e->timing || !m->a || !m->b) return;
+    (void)hipEventRecord(m->b, e->st);
+    e->pending.push_back(*m);
+}
+
+// Called after the compute stream is idle.
+static void harvest_marks(ddm_engine *e) {
+    for (const Mark &m : e->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) {
+            switch (m.kind) {
+            case MK_FFT: e->stats.fft_ms += ms; break;
+            case MK_LAG: e->stats.lag_ms += ms; break;
+            case MK_AZ: e->stats.azimuthal_ms += ms; break;
+            case MK_CHUNK:
+                e->chunk_ms.push_back(ms);
+                e->stats.chunks += 1;
+                break;
+            }
+        }
+        e->pool.push_back(m.a);
+        e->pool.push_back(m.b);
+    }
+    e->pending.clear();
+}
+
+static void release(ddm_engine *e) {
+    if (!e) return;
+    if (e->st) (void)hipStreamSynchronize(e->st);
+    if (e->st_copy) (void)hipStreamSynchronize(e->st_copy);
+    for (const Mark &m : e->pending) {
+        e->pool.push_back(m.a);
+        e->pool.push_back(m.b);
+    }
+    e->pending.clear();
+    for (hipEvent_t ev : e->pool)
+        if (ev) (void)hipEventDestroy(ev);
+    e->pool.clear();
+    void *dev[] = {e->d_ring, e->d_acc, e->d_lags, e->d_az, e->d_qoff, e->d_qcnt,
+                   e->d_idx,  e->d_norm, e->d_out, e->d_stage[0], e->d_stage[1]};
+    for (void *ptr : dev)
+        if (ptr) (void)hipFree(ptr);
+    for (int i = 0; i < 2; ++i) {
+        if (e->h_stage[i]) (void)hipHostFree(e->h_stage[i]);
+        if (e->ev_up[i]) (void)hipEventDestroy(e->ev_up[i]);
+        if (e->ev_free[i]) (void)hipEventDestroy(e->ev_free[i]);
+    }
+    if (e->st) (void)hipStreamDestroy(e->st);
+    if (e->st_copy) (void)hipStreamDestroy(e->st_copy);
+}
+
+template <class T>
+static int dev_alloc(ddm_engine *e, T **ptr, size_t count, const char *what) {
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t rc = hipMalloc((void **)ptr, bytes);
+    if (rc != hipSuccess) {
+        *ptr = nullptr;
+        e->err = std::string("device allocation of ") + what + " (" + std::to_string(bytes) +
+                 " bytes) failed: " + hipGetErrorString(rc);
+        return rc == hipErrorOutOfMemory ? DDM_ENOMEM : DDM_EDEVICE;
+    }
+    e->dev_bytes += bytes;
+    return DDM_OK;
+}
+
+static int choose_lag_block(const Params &p) {
+    if (p.lag_block) return p.lag_block;
+    return p.chunk <= 16 ? 16 : 32;
+}
+
+extern "C" int ddm_create(const ddm_desc *desc, ddm_engine **out) {
+    if (!out) return DDM_EINVAL;
+    *out = nullptr;
+    ddm_engine *e = new (std::nothrow) ddm_engine();
+    if (!e) return DDM_ENOMEM;
+    std::string msg;
+    int rc = ddm::check_desc(desc, &e->p, &msg);  // validate before touching the device
+    if (rc != DDM_OK) {
+        g_create_err = msg;
+        delete e;
+        return rc;
+    }
+    const Params &p = e->p;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        g_create_err = "no HIP device available";
+        delete e;
+        return DDM_EDEVICE;
+    }
+    auto bail = [&](int code) {
+        g_create_err = e->err;
+        release(e);
+        delete e;
+        return code;
+    };
+#define TRY(x)                 \
+    do {                       \
+        int r_ = (x);          \
+        if (r_) return bail(r_); \
+    } while (0)
+#define HTRY(call)                                                              \
+    do {                                                                        \
+        hipError_t e_ = (call);                                                 \
+        if (e_ != hipSuccess) {                                                 \
+            e->err = std::string(#call) + ": " + hipGetErrorString(e_);         \
+            return bail(e_ == hipErrorOutOfMemory ? DDM_ENOMEM : DDM_EDEVICE);  \
+        }                                                                       \
+    } while (0)
+    HTRY(hipGetDevice(&e->device));
+    e->lag_block = choose_lag_block(p);
+    TRY(ddm::build_ring_tables(p, &e->rt));
+    const long long E = p.elems_per_frame;
+    const size_t L = p.lags.size();
+    HTRY(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
+    HTRY(hipStreamCreateWithFlags(&e->st_copy, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i) {
+        HTRY(hipEventCreateWithFlags(&e->ev_up[i], hipEventDisableTiming));
+        HTRY(hipEventCreateWithFlags(&e->ev_free[i], hipEventDisableTiming));
+    }
+    TRY(dev_alloc(e, &e->d_ring, (size_t)p.ring_slots * (size_t)E * 2, "spectrum ring"));
+    TRY(dev_alloc(e, &e->d_acc, L * (size_t)E, "lag accumulators"));
+    TRY(dev_alloc(e, &e->d_lags, L, "lags"));
+    TRY(dev_alloc(e, &e->d_az, p.geom.size(), "scale table"));
+    TRY(dev_alloc(e, &e->d_qoff, e->rt.off.size(), "ring offsets"));
+    TRY(dev_alloc(e, &e->d_qcnt, e->rt.cnt.size(), "ring counts"));
+    TRY(dev_alloc(e, &e->d_idx, e->rt.idx.size(), "ring elements"));
+    TRY(dev_alloc(e, &e->d_norm, L, "normalisation"));
+    TRY(dev_alloc(e, &e->d_out, (size_t)p.result_count, "results"));
+    const size_t roi_frame = (size_t)p.M * p.M * p.bpp;
+    for (int i = 0; i < 2; ++i) {
+        TRY(dev_alloc(e, &e->d_stage[i], roi_frame * p.chunk, "upload staging"));
+        HTRY(hipHostMalloc((void **)&e->h_stage[i], roi_frame * p.chunk, hipHostMallocDefault));
+        e->pinned_bytes += roi_frame * p.chunk;
+    }
+    std::vector<ddm::AzScale> az;
+    long long out_base = 0;
+    for (const ddm::ScaleGeom &g : p.geom) {
+        ddm::AzScale a;
+        a.out_base = out_base;
+        a.elem_off = g.elem_off;
+        a.tile_elems = g.tile_elems;
+        const double s2 = (double)g.s * (double)g.s;
+        a.inv_s4 = 1.0 / (s2 * s2);
+        az.push_back(a);
+        out_base += (long long)g.tiles * (long long)p.lambdas.size() * (long long)L;
+    }
+    HTRY(hipMemcpy(e->d_lags, p.lags.data(), L * sizeof(int), hipMemcpyHostToDevice));
+    HTRY(hipMemcpy(e->d_az, az.data(), az.size() * sizeof(ddm::AzScale), hipMemcpyHostToDevice));
+    HTRY(hipMemcpy(e->d_qoff, e->rt.off.data(), e->rt.off.size() * sizeof(int), hipMemcpyHostToDevice));
+    HTRY(hipMemcpy(e->d_qcnt, e->rt.cnt.data(), e->rt.cnt.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!e->rt.idx.empty())
+        HTRY(hipMemcpy(e->d_idx, e->rt.idx.data(), e->rt.idx.size() * sizeof(int), hipMemcpyHostToDevice));
+    HTRY(hipMemsetAsync(e->d_acc, 0, L * (size_t)E * sizeof(float), e->st));
+    const float2 *tw = nullptr;
+    HTRY(ddm::twiddle_table(&tw));
+    HTRY(hipStreamSynchronize(e->st));
+    e->pairs.assign(L, 0);
+#undef TRY
+#undef HTRY
+    *out = e;
+    return DDM_OK;
+}
+
+extern "C" void ddm_destroy(ddm_engine *e) {
+    if (!e) return;
+    release(e);
+    delete e;
+}
+
+// Tile + FFT + lag engine for n <= chunk frames at device address src.
+static int process_chunk(ddm_engine *e, const uint8_t *src, int n, long long frame_bytes, int width,
+                         int x_off, int y_off, hipEvent_t consumed) {
+    const Params &p = e->p;
+    const long long t0 = e->frames;
+    const int R = p.ring_slots;
+    const int first_slot = (int)(t0 % R);
+    Mark chunk, fft, lag;
+    mark_begin(e, &chunk, MK_CHUNK);
+    mark_begin(e, &fft, MK_FFT);
+    for (const ddm::ScaleGeom &g : p.geom) {
+        ddm::TileFftArgs a;
+        a.frames = src;
+        a.n_frames = n;
+        a.frame_bytes = frame_bytes;
+        a.width = width;
+        a.bpp = p.bpp;
+        a.fmt = p.fmt;
+        a.x_off = x_off;
+        a.y_off = y_off;
+        a.largest_scale = p.M;
+        a.scale = g.s;
+        a.ring = e->d_ring;
+        a.ring_slots = R;
+        a.first_slot = first_slot;
+        a.elems_per_frame = p.elems_per_frame;
+        a.elem_off = g.elem_off;
+        HIP_TRY(e, ddm::launch_tile_fft(a, e->st));
+        e->stats.fft_launches += (g.s >= 256) ? 2 : 1;
+    }
+    mark_end(e, &fft);
+    if (consumed) HIP_TRY(e, hipEventRecord(consumed, e->st));
+    mark_begin(e, &lag, MK_LAG);
+    for (int b = 0; b < n; b += e->lag_block) {
+        ddm::LagArgs a;
+        a.ring = e->d_ring;
+        a.acc = e->d_acc;
+        a.elems_per_frame = p.elems_per_frame;
+        a.ring_slots = R;
+        a.t0 = t0 + b;
+        a.slot0 = (int)(a.t0 % R);
+        a.n_new = std::min(e->lag_block, n - b);
+        a.lags = e->d_lags;
+        a.lag_count = (int)p.lags.size();
+        a.lag_block = e->lag_block;
+        HIP_TRY(e, ddm::launch_lag(a, e->st));
+        e->stats.lag_launches += 1;
+    }
+    mark_end(e, &lag);
+    mark_end(e, &chunk);
+    // pairs (t - tau, t) formed for t in [t0, t0 + n)
+    for (size_t l = 0; l < p.lags.size(); ++l) {
+        const long long lo = std::max<long long>(t0, p.lags[l]);
+        const long long hi = t0 + n;
+        if (hi > lo) e->pairs[l] += hi - lo;
+    }
+    e->frames += n;
+    e->frames_since_reset += n;
+    return DDM_OK;
+}
+
+extern "C" int ddm_submit_device(ddm_engine *e, const void *frames, int n_frames) {
+    if (!e) return DDM_EINVAL;
+    if (n_frames < 0 || (n_frames > 0 && !frames)) return fail(e, DDM_EINVAL, "bad frame buffer");
+    const Params &p = e->p;
+    const long long fb = (long long)p.width * p.height * p.bpp;
+    const uint8_t *src = (const uint8_t *)frames;
+    for (int i = 0; i < n_frames; i += p.chunk) {
+        const int n = std::min(p.chunk, n_frames - i);
+        int rc = process_chunk(e, src + (long long)i * fb, n, fb, p.width, p.x_off, p.y_off, nullptr);
+        if (rc) return rc;
+    }
+    return DDM_OK;
+}
+
+extern "C" int ddm_submit_host(ddm_engine *e, const void *frames, int n_frames) {
+    if (!e) return DDM_EINVAL;
+    if (n_frames < 0 || (n_frames > 0 && !frames)) return fail(e, DDM_EINVAL, "bad frame buffer");
+    const Params &p = e->p;
+    const size_t row = (size_t)p.M * p.bpp;
+    const size_t roi_frame = row * p.M;
+    const uint8_t *src = (const uint8_t *)frames;
+    const size_t fb = (size_t)p.width * p.height * p.bpp;
+    for (int i = 0; i < n_frames; i += p.chunk) {
+        const int n = std::min(p.chunk, n_frames - i);
+        const int b = e->next_stage;
+        if (e->stage_used[b]) HIP_TRY(e, hipEventSynchronize(e->ev_free[b]));
+        // ROI-only copy into pinned memory (the reference uploads whole frames, src/DDM.cu:506)
+        uint8_t *dst = e->h_stage[b];
+        for (int f = 0; f < n; ++f) {
+            const uint8_t *fr = src + (size_t)(i + f) * fb;
+            for (int y = 0; y < p.M; ++y)
+                std::memcpy(dst + (size_t)f * roi_frame + (size_t)y * row,
+                            fr + ((size_t)(p.y_off + y) * p.width + p.x_off) * p.bpp, row);
+        }
+        HIP_TRY(e, hipMemcpyAsync(e->d_stage[b], dst, roi_frame * n, hipMemcpyHostToDevice, e->st_copy));
+        HIP_TRY(e, hipEventRecord(e->ev_up[b], e->st_copy));
+        HIP_TRY(e, hipStreamWaitEvent(e->st, e->ev_up[b], 0));
+        int rc = process_chunk(e, e->d_stage[b], n, (long long)roi_frame, p.M, 0, 0, e->ev_free[b]);
+        if (rc) return rc;
+        e->stage_used[b] = true;
+        e->next_stage ^= 1;
+    }
+    return DDM_OK;
+}
+
+extern "C" int ddm_sync(ddm_engine *e) {
+    if (!e) return DDM_EINVAL;
+    HIP_TRY(e, hipStreamSynchronize(e->st));
+    HIP_TRY(e, hipStreamSynchronize(e->st_copy));
+    harvest_marks(e);
+    return DDM_OK;
+}
+
+extern "C" int ddm_results(ddm_engine *e, double *results) {
+    if (!e || !results) return DDM_EINVAL;
+    const Params &p = e->p;
+    const size_t L = p.lags.size();
+    std::vector<double> norm(L, 0.0);
+    for (size_t l = 0; l < L; ++l) {
+        if (p.norm_mode == DDM_NORM_PAIRS)
+            norm[l] = e->pairs[l] > 0 ? 1.0 / (double)e->pairs[l] : 0.0;
+        else
+            norm[l] = e->frames_since_reset > 0 ? 1.0 / (double)e->frames_since_reset : 0.0;
+    }
+    HIP_TRY(e, hipMemcpyAsync(e->d_norm, norm.data(), L * sizeof(double), hipMemcpyHostToDevice, e->st));
+    ddm::AzArgs a;
+    a.acc = e->d_acc;
+    a.elems_per_frame = p.elems_per_frame;
+    a.scales = e->d_az;
+    a.scale_count = (int)p.geom.size();
+    a.q_off = e->d_qoff;
+    a.q_cnt = e->d_qcnt;
+    a.idx = e->d_idx;
+    a.norm = e->d_norm;
+    a.lambda_count = (int)p.lambdas.size();
+    a.lag_count = (int)L;
+    a.out = e->d_out;
+    a.result_count = p.result_count;
+    Mark az;
+    mark_begin(e, &az, MK_AZ);
+    HIP_TRY(e, ddm::launch_azimuthal(a, e->st));
+    mark_end(e, &az);
+    e->stats.azimuthal_launches += 1;
+    HIP_TRY(e, hipMemcpyAsync(results, e->d_out, (size_t)p.result_count * sizeof(double), hipMemcpyDeviceToHost,
+                              e->st));
+    return ddm_sync(e);
+}
+
+extern "C" int ddm_reset(ddm_engine *e) {
+    if (!e) return DDM_EINVAL;
+    const Params &p = e->p;
+    HIP_TRY(e, hipMemsetAsync(e->d_acc, 0, p.lags.size() * (size_t)p.elems_per_frame * sizeof(float), e->st));
+    e->frames_since_reset = 0;
+    std::fill(e->pairs.begin(), e->pairs.end(), 0);
+    return DDM_OK;
+}
+
+extern "C" int ddm_get_info(const ddm_engine *e, ddm_info *info) {
+    if (!e || !info) return DDM_EINVAL;
+    const Params &p = e->p;
+    std::memset(info, 0, sizeof(*info));
+    info->largest_scale = p.M;
+    info->scale_count = (int)p.scales.size();
+    info->lag_count = (int)p.lags.size();
+    info->lambda_count = (int)p.lambdas.size();
+    info->ring_slots = p.ring_slots;
+    info->elems_per_frame = p.elems_per_frame;
+    info->result_count = p.result_count;
+    info->frames_submitted = e->frames;
+    info->frames_since_reset = e->frames_since_reset;
+    info->device_bytes = e->dev_bytes;
+    info->pinned_bytes = e->pinned_bytes;
+    return DDM_OK;
+}
+
+extern "C" int ddm_get_params(const ddm_engine *e, int *scales, int *lags, float *lambdas) {
+    if (!e) return DDM_EINVAL;
+    const Params &p = e->p;
+    if (scales) std::copy(p.scales.begin(), p.scales.end(), scales);
+    if (lags) std::copy(p.lags.begin(), p.lags.end(), lags);
+    if (lambdas) std::copy(p.lambdas.begin(), p.lambdas.end(), lambdas);
+    return DDM_OK;
+}
+
+extern "C" int ddm_get_accumulators(ddm_engine *e, int scale_idx, float *out) {
+    if (!e || !out) return DDM_EINVAL;
+    const Params &p = e->p;
+    if (scale_idx < 0 || scale_idx >= (int)p.geom.size()) return fail(e, DDM_ERANGE, "scale index out of range");
+    int rc = ddm_sync(e);
+    if (rc) return rc;
+    const ddm::ScaleGeom &g = p.geom[scale_idx];
+    const size_t n = (size_t)g.tiles * g.tile_elems;
+    for (size_t l = 0; l < p.lags.size(); ++l)
+        HIP_TRY(e, hipMemcpy(out + l * n, e->d_acc + l * (size_t)p.elems_per_frame + g.elem_off, n * sizeof(float),
+                             hipMemcpyDeviceToHost));
+    // the reference accumulates (|dF| / s^2)^2; 1/s^4 is a power of two, so the
+    // scaling is exact
+    const float inv = (float)(1.0 / ((double)g.s * g.s * (double)g.s * g.s));
+    for (size_t i = 0; i < n * p.lags.size(); ++i) out[i] *= inv;
+    return DDM_OK;
+}
+
+extern "C" int ddm_get_spectra(ddm_engine *e, long long frame, int scale_idx, float *out) {
+    if (!e || !out) return DDM_EINVAL;
+    const Params &p = e->p;
+    if (scale_idx < 0 || scale_idx >= (int)p.geom.size()) return fail(e, DDM_ERANGE, "scale index out of range");
+    if (frame < 0 || frame >= e->frames || frame < e->frames - p.ring_slots)
+        return fail(e, DDM_ERANGE, "frame is not held in the spectrum ring");
+    int rc = ddm_sync(e);
+    if (rc) return rc;
+    const ddm::ScaleGeom &g = p.geom[scale_idx];
+    const size_t slot = (size_t)(frame % p.ring_slots);
+    const size_t n = (size_t)g.tiles * g.tile_elems;
+    HIP_TRY(e, hipMemcpy(out, e->d_ring + (slot * (size_t)p.elems_per_frame + g.elem_off) * 2, n * 2 * sizeof(float),
+                         hipMemcpyDeviceToHost));
+    return DDM_OK;
+}
+
+extern "C" int ddm_set_timing(ddm_engine *e, int on) {
+    if (!e) return DDM_EINVAL;
+    int rc = ddm_sync(e);
+    if (rc) return rc;
+    e->timing = on != 0;
+    return DDM_OK;
+}
+
+extern "C" int ddm_get_stage_stats(ddm_engine *e, ddm_stage_stats *s) {
+    if (!e || !s) return DDM_EINVAL;
+    int rc = ddm_sync(e);
+    if (rc) return rc;
+    *s = e->stats;
+    return DDM_OK;
+}
+
+extern "C" int ddm_get_chunk_times(ddm_engine *e, float *ms, int max) {
+    if (!e || (!ms && max > 0)) return DDM_EINVAL;
+    int rc = ddm_sync(e);
+    if (rc) return rc;
+    const int n = std::min<int>(max, (int)e->chunk_ms.size());
+    std::copy(e->chunk_ms.begin(), e->chunk_ms.begin() + n, ms);
+    return n;
+}
+
+extern "C" const char *ddm_last_error(const ddm_engine *e) { return e ? e->err.c_str() : g_create_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Stage entry points on caller-owned device buffers.
+// ---------------------------------------------------------------------------
+static bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+extern "C" int ddm_k_tile_fft(const void *frames, int n_frames, long long frame_bytes, int width, int height,
+                              int bytes_per_pixel, int pixel_format, int x_off, int y_off, int largest_scale,
+                              int scale, float *ring, int ring_slots, long long elems_per_frame,
+                              long long elem_off, long long first_frame, void *stream) {
+    if (!frames || !ring || n_frames <= 0 || ring_slots <= 0 || n_frames > ring_slots || first_frame < 0)
+        return DDM_EINVAL;
+    if (!pow2(scale) || !pow2(largest_scale) || scale > largest_scale || scale > ddm::kMaxScale) return DDM_ESCALE;
+    if (pixel_format < DDM_PIX_U8 || pixel_format > DDM_PIX_U16BE) return DDM_EINVAL;
+    if (bytes_per_pixel < ddm::pixel_bytes_needed(pixel_format)) return DDM_EINVAL;
+    if (x_off < 0 || y_off < 0 || (long long)x_off + largest_scale > width || (long long)y_off + largest_scale > height)
+        return DDM_EROI;
+    if (frame_bytes < (long long)width * height * bytes_per_pixel) return DDM_EINVAL;
+    const long long tiles = (long long)(largest_scale / scale) * (largest_scale / scale);
+    if (elem_off < 0 || elem_off + tiles * scale * (scale / 2 + 1) > elems_per_frame) return DDM_ERANGE;
+    ddm::TileFftArgs a;
+    a.frames = frames;
+    a.n_frames = n_frames;
+    a.frame_bytes = frame_bytes;
+    a.width = width;
+    a.bpp = bytes_per_pixel;
+    a.fmt = pixel_format;
+    a.x_off = x_off;
+    a.y_off = y_off;
+    a.largest_scale = largest_scale;
+    a.scale = scale;
+    a.ring = ring;
+    a.ring_slots = ring_slots;
+    a.first_slot = (int)(first_frame % ring_slots);
+    a.elems_per_frame = elems_per_frame;
+    a.elem_off = elem_off;
+    return ddm::launch_tile_fft(a, (hipStream_t)stream) == hipSuccess ? DDM_OK : DDM_EDEVICE;
+}
+
+extern "C" int ddm_k_lag_accumulate(const float *ring, int ring_slots, long long elems_per_frame,
+                                    long long first_frame, int n_new, const int *lags, int lag_count, int max_lag,
+                                    float *acc, int lag_block, void *stream) {
+    if (!ring || !acc || !lags || ring_slots <= 0 || elems_per_frame <= 0 || first_frame < 0 || lag_count <= 0)
+        return DDM_EINVAL;
+    if (lag_block != 16 && lag_block != 32) return DDM_EINVAL;
+    if (n_new <= 0 || n_new > lag_block) return DDM_EINVAL;
+    if (max_lag < 0 || (long long)max_lag + n_new > ring_slots) return DDM_ERANGE;
+    ddm::LagArgs a;
+    a.ring = ring;
+    a.acc = acc;
+    a.elems_per_frame = elems_per_frame;
+    a.ring_slots = ring_slots;
+    a.t0 = first_frame;
+    a.slot0 = (int)(first_frame % ring_slots);
+    a.n_new = n_new;
+    a.lags = lags;
+    a.lag_count = lag_count;
+    a.lag_block = lag_block;
+    return ddm::launch_lag(a, (hipStream_t)stream) == hipSuccess ? DDM_OK : DDM_EDEVICE;
+}

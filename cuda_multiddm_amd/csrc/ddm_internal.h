@@ -1,0 +1,72 @@
+// Internal declarations shared by the host-side translation units of the
+// DDM engine (validation, q-ring tables, synthetic video, writers) and the
+// HIP engine/kernels.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ddm_amd.h"
+
+namespace ddm {
+
+// Largest tile size the FFT kernels are instantiated for.
+constexpr int kMaxScale = 2048;
+// Lag values above this are rejected (ring memory would be absurd).
+constexpr int kMaxLag = 1 << 20;
+
+struct ScaleGeom {
+    int s = 0;            // tile size
+    int tps = 0;          // tiles per side, M / s
+    int tiles = 0;        // tps * tps
+    long long tile_elems = 0;  // s * (s/2 + 1)
+    long long elem_off = 0;    // offset of this scale inside one frame's spectrum
+};
+
+// Descriptor after validation and sorting: scales descending, lags and
+// lambdas ascending (src/DDM.cu:319-321).
+struct Params {
+    int width = 0, height = 0, bpp = 1, fmt = DDM_PIX_U8;
+    int x_off = 0, y_off = 0;
+    std::vector<int> scales;
+    std::vector<int> lags;
+    std::vector<float> lambdas;
+    float tol = 1.2f;
+    int chunk = 30;
+    int norm_mode = DDM_NORM_FRAMES;
+    int lag_block = 0;
+    // derived
+    int M = 0;
+    int max_lag = 0;
+    int ring_slots = 0;
+    long long elems_per_frame = 0;
+    long long result_count = 0;
+    std::vector<ScaleGeom> geom;
+    std::vector<std::string> warnings;
+};
+
+// Validates a descriptor without touching any device.  Returns DDM_OK or a
+// negative code with the reference's message text where one exists.
+int check_desc(const ddm_desc *d, Params *out, std::string *msg);
+
+// Element lists of every q ring of every scale: for scale i and q j the
+// member elements (ky * (s/2+1) + kx, ascending) are
+// idx[off[i*Q + j] .. off[i*Q + j] + cnt[i*Q + j]).
+struct RingTables {
+    std::vector<int> idx;
+    std::vector<int> off;
+    std::vector<int> cnt;
+};
+int build_ring_tables(const Params &p, RingTables *out);
+
+// Pixel format helpers.
+inline int pixel_bytes_needed(int fmt) { return fmt == DDM_PIX_U8 ? 1 : 2; }
+
+// Twiddle table for every power-of-two length n <= kMaxScale: entries
+// [n, 2n) hold exp(-2 pi i m / n), m < n, as (re, im) floats rounded from
+// double precision.
+void fill_twiddles(std::vector<float> *tw);
+
+}  // namespace ddm

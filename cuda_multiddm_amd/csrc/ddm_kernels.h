@@ -1,0 +1,60 @@
+// Launch interface of the DDM kernels (ddm_kernels.hip), used by the engine
+// and by the stage entry points of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace ddm {
+
+struct TileFftArgs {
+    const void *frames;        // device pointer, frame f at frames + f * frame_bytes
+    int n_frames;
+    long long frame_bytes;
+    int width, bpp, fmt;       // pixel addressing: row stride = width * bpp
+    int x_off, y_off;          // ROI origin inside the frame
+    int largest_scale, scale;
+    float *ring;               // [slot][elems_per_frame] complex
+    int ring_slots;
+    int first_slot;            // slot of frame 0 of this batch
+    long long elems_per_frame;
+    long long elem_off;        // offset of this scale within a frame's spectrum
+};
+
+struct LagArgs {
+    const float *ring;
+    float *acc;                // [lag][elems_per_frame]
+    long long elems_per_frame;
+    int ring_slots;
+    int slot0;                 // slot of frame t0
+    long long t0;              // global index of the first new frame
+    int n_new;                 // <= lag_block
+    const int *lags;           // device, ascending
+    int lag_count;
+    int lag_block;             // 16 or 32
+};
+
+struct AzScale {
+    long long out_base;        // first result index of this scale
+    long long elem_off;
+    long long tile_elems;
+    double inv_s4;             // 1 / s^4: the reference's (1/s^2)^2 per-term factor
+};
+
+struct AzArgs {
+    const float *acc;
+    long long elems_per_frame;
+    const AzScale *scales;     // device
+    int scale_count;
+    const int *q_off, *q_cnt, *idx;  // device ring tables
+    const double *norm;        // device [lag]
+    int lambda_count, lag_count;
+    double *out;               // device [result_count]
+    long long result_count;
+};
+
+hipError_t twiddle_table(const float2 **out);
+hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st);
+hipError_t launch_lag(const LagArgs &a, hipStream_t st);
+hipError_t launch_azimuthal(const AzArgs &a, hipStream_t st);
+
+}  // namespace ddm

@@ -1,0 +1,543 @@
+// HIP kernels of the multi-scale DDM engine for gfx950 (CDNA4, wave64).
+//
+//   tile_fft_small<S>  S <= 128: tile cut + 2-D R2C FFT of whole tiles in LDS
+//   tile_fft_rows<S>   S >= 256: tile cut + row R2C FFTs (first half of the 2-D FFT)
+//   tile_fft_cols<S>   S >= 256: column FFTs, in place in the ring slot
+//   lag_kernel<J>      |F_t - F_{t-tau}|^2 accumulation for a block of J frames
+//   azimuthal_kernel   masked q-ring average of the lag sums, FP64
+//
+// Reference counterparts: parseBufferScalePow2 (src/DDM_kernel.cuh:43-78),
+// cufftExecR2C (src/DDM.cu:189-192), processFFT (src/DDM_kernel.cuh:85-101),
+// kernelMaskReduce (src/azimuthal_average_kernel.cuh:68-135).
+//
+// FFT scheme (all sizes): the real s x s tile is read as s/2 complex rows
+// z_r = x_{2r} + i x_{2r+1}; each gets one complex Stockham FFT of length s
+// (radix-4 passes, one radix-2 pass for odd log2 s); the two real rows'
+// spectra are separated as X_a = (Z_k + conj Z_{s-k}) / 2,
+// X_b = -i (Z_k - conj Z_{s-k}) / 2.  Columns kx = 0 and kx = s/2 are real
+// after the row pass, so they travel packed as one complex column
+// (DC + i Nyquist); this leaves exactly s/2 complex columns of length s,
+// and the packed column is separated the same way when the result is
+// written.  Twiddles come from a table rounded once from double precision.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <vector>
+
+#include "ddm_internal.h"
+#include "ddm_kernels.h"
+
+namespace ddm {
+
+constexpr int NT = 256;  // threads per workgroup (4 waves of 64)
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+template <int R>
+struct Bfly;
+template <>
+struct Bfly<2> {
+    __device__ __forceinline__ static void run(float2 *v) {
+        float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+template <>
+struct Bfly<4> {
+    // X0 = (v0+v2)+(v1+v3), X1 = (v0-v2)-i(v1-v3), X2 = (v0+v2)-(v1+v3), X3 = (v0-v2)+i(v1-v3)
+    __device__ __forceinline__ static void run(float2 *v) {
+        float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        float2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+        v[0] = cadd(s02, s13);
+        v[2] = csub(s02, s13);
+        v[1] = make_float2(d02.x + d13.y, d02.y - d13.x);
+        v[3] = make_float2(d02.x - d13.y, d02.y + d13.x);
+    }
+};
+
+// Sequence q (of NSEQ) starts at (q / SPT) * TSTR + (q % SPT) * SSTR; element
+// i of a sequence is ESTR further.  SEQ_FAST: consecutive threads walk
+// sequences first (column layouts), else elements first (row layouts).
+template <int N, int NSEQ, int SPT, int SSTR, int TSTR, int ESTR, bool SEQ_FAST>
+struct Layout {
+    static constexpr int n = N, nseq = NSEQ, spt = SPT, sstr = SSTR, tstr = TSTR, estr = ESTR;
+    static constexpr bool seq_fast = SEQ_FAST;
+    __device__ __forceinline__ static int base(int q) { return (q / SPT) * TSTR + (q % SPT) * SSTR; }
+};
+
+// One in-place Stockham pass of radix R over all sequences of the layout.
+// Every thread loads all of its butterflies' inputs before the barrier and
+// writes after it, so the pass needs no second buffer.
+template <class LY, int R, int NS>
+__device__ __forceinline__ void stockham_pass(float2 *buf, const float2 *tw) {
+    constexpr int N = LY::n;
+    constexpr int BPS = N / R;
+    constexpr int TOTAL = BPS * LY::nseq;
+    constexpr int KB = (TOTAL + NT - 1) / NT;
+    constexpr int TWS = N / (NS * R);
+    const int tid = threadIdx.x;
+    float2 v[KB][R];
+    int dst[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        const int b = tid + k * NT;
+        dst[k] = -1;
+        if ((TOTAL % NT == 0) || b < TOTAL) {
+            const int q = LY::seq_fast ? (b % LY::nseq) : (b / BPS);
+            const int j = LY::seq_fast ? (b / LY::nseq) : (b % BPS);
+            const int kk = j % NS;
+            const int base = LY::base(q);
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[k][r] = buf[base + (j + r * BPS) * LY::estr];
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[k][r] = cmul(v[k][r], tw[kk * r * TWS]);
+            }
+            Bfly<R>::run(v[k]);
+            dst[k] = base + ((j / NS) * NS * R + kk) * LY::estr;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        if ((TOTAL % NT == 0) || dst[k] >= 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[dst[k] + r * NS * LY::estr] = v[k][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <class LY, int NS>
+__device__ __forceinline__ void fft_passes(float2 *buf, const float2 *tw) {
+    if constexpr (NS < LY::n) {
+        constexpr int REM = LY::n / NS;
+        constexpr int R = (REM % 4 == 0) ? 4 : 2;
+        stockham_pass<LY, R, NS>(buf, tw);
+        fft_passes<LY, NS * R>(buf, tw);
+    }
+}
+
+struct PixSrc {
+    const uint8_t *frames;
+    long long frame_bytes;
+    long long row_bytes;
+    int bpp, fmt, x_off, y_off;
+};
+
+__device__ __forceinline__ float load_px(const uint8_t *fr, const PixSrc &ps, int x, int y) {
+    const uint8_t *p = fr + (long long)y * ps.row_bytes + (long long)x * ps.bpp;
+    if (ps.fmt == DDM_PIX_U8) return (float)p[0];
+    const unsigned b0 = p[0], b1 = p[1];
+    return (float)(ps.fmt == DDM_PIX_U16LE ? (b0 | (b1 << 8)) : ((b0 << 8) | b1));
+}
+
+struct RingDst {
+    float2 *ring;
+    long long elems_per_frame;
+    long long elem_off;  // of this scale
+    int ring_slots;
+    int first_slot;      // slot of frame 0 of this launch
+};
+
+__device__ __forceinline__ float2 *tile_out(const RingDst &rd, int f, int tile, long long tile_elems) {
+    int slot = rd.first_slot + f;
+    slot -= (slot >= rd.ring_slots) ? rd.ring_slots : 0;  // f < ring_slots
+    return rd.ring + (long long)slot * rd.elems_per_frame + rd.elem_off + (long long)tile * tile_elems;
+}
+
+// ---------------------------------------------------------------------------
+// S <= 128: whole tiles in LDS.  TPW tiles per workgroup so that each
+// workgroup holds at least 1024 complex values.
+// ---------------------------------------------------------------------------
+template <int S>
+struct SmallCfg {
+    static constexpr int H = S / 2;
+    static constexpr int TPW = (2048 / (S * S)) > 0 ? (2048 / (S * S)) : 1;
+    static constexpr int TILE = S * H;  // complex values per tile in LDS
+};
+
+template <int S>
+__global__ __launch_bounds__(NT) void tile_fft_small(PixSrc ps, RingDst rd, int tps, int tiles,
+                                                    int groups, const float2 *__restrict__ twg) {
+    using C = SmallCfg<S>;
+    constexpr int H = C::H, TPW = C::TPW, TILE = C::TILE;
+    __shared__ float2 buf[TPW * TILE];
+    __shared__ float2 tw[S];
+    const int tid = threadIdx.x;
+    const int f = blockIdx.x / groups;
+    const int tile0 = (blockIdx.x % groups) * TPW;
+    for (int i = tid; i < S; i += NT) tw[i] = twg[S + i];
+
+    // load: z[t][r][x] = x_{2r}[x] + i x_{2r+1}[x]
+    const uint8_t *fr = ps.frames + (long long)f * ps.frame_bytes;
+    for (int i = tid; i < TPW * TILE; i += NT) {
+        const int t = i / TILE, rem = i % TILE;
+        const int r = rem / S, x = rem % S;
+        const int tile = tile0 + t;
+        float a = 0.f, b = 0.f;
+        if (tile < tiles) {
+            const int px = ps.x_off + (tile % tps) * S + x;
+            const int py = ps.y_off + (tile / tps) * S + 2 * r;
+            a = load_px(fr, ps, px, py);
+            b = load_px(fr, ps, px, py + 1);
+        }
+        buf[i] = make_float2(a, b);
+    }
+    __syncthreads();
+
+    // row FFTs: TPW * H sequences of length S, contiguous
+    using Rows = Layout<S, TPW * H, TPW * H, S, 0, 1, false>;
+    fft_passes<Rows, 1>(buf, tw);
+
+    // separate the row pairs: Y[2r][k] at r*S + k, Y[2r+1][k] at r*S + H + k
+    {
+        constexpr int NU = TPW * H * H;
+        constexpr int KU = (NU + NT - 1) / NT;
+        float2 za[KU], zb[KU];
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+            const int i = tid + u * NT;
+            if ((NU % NT == 0) || i < NU) {
+                const int k = i % H, rr = i / H;  // rr = t * H + r
+                const int base = rr * S;
+                za[u] = buf[base + k];
+                zb[u] = buf[base + (k == 0 ? H : S - k)];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+            const int i = tid + u * NT;
+            if ((NU % NT == 0) || i < NU) {
+                const int k = i % H, rr = i / H;
+                const int base = rr * S;
+                float2 ya, yb;
+                if (k == 0) {
+                    ya = make_float2(za[u].x, zb[u].x);  // (DC, Nyquist) of row 2r
+                    yb = make_float2(za[u].y, zb[u].y);  // (DC, Nyquist) of row 2r+1
+                } else {
+                    ya = make_float2(0.5f * (za[u].x + zb[u].x), 0.5f * (za[u].y - zb[u].y));
+                    yb = make_float2(0.5f * (za[u].y + zb[u].y), -0.5f * (za[u].x - zb[u].x));
+                }
+                buf[base + k] = ya;
+                buf[base + H + k] = yb;
+            }
+        }
+        __syncthreads();
+    }
+
+    // column FFTs: per tile H columns of length S, stride H
+    using Cols = Layout<S, TPW * H, H, 1, TILE, H, true>;
+    fft_passes<Cols, 1>(buf, tw);
+
+    // write [tile][ky][kx], separating the packed DC / Nyquist column
+    constexpr int OUT = S * (H + 1);
+    for (int i = tid; i < TPW * OUT; i += NT) {
+        const int t = i / OUT, rem = i % OUT;
+        const int tile = tile0 + t;
+        if (tile >= tiles) continue;
+        const int ky = rem / (H + 1), kx = rem % (H + 1);
+        const float2 *Y = buf + t * TILE;
+        float2 val;
+        if (kx == 0 || kx == H) {
+            const float2 w = Y[ky * H];
+            const float2 v = Y[((S - ky) & (S - 1)) * H];
+            val = (kx == 0) ? make_float2(0.5f * (w.x + v.x), 0.5f * (w.y - v.y))
+                            : make_float2(0.5f * (w.y + v.y), -0.5f * (w.x - v.x));
+        } else {
+            val = Y[ky * H + kx];
+        }
+        tile_out(rd, f, tile, OUT)[rem] = val;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// S >= 256, pass 1: P row pairs per workgroup; row FFT, separation, and the
+// half-spectrum rows written to the ring slot (columns 0..H-1, with the
+// packed DC/Nyquist value in column 0).
+// ---------------------------------------------------------------------------
+template <int S>
+struct LargeCfg {
+    static constexpr int H = S / 2;
+    static constexpr int P = (2048 / S) > 0 ? (2048 / S) : 1;  // row pairs per workgroup
+    static constexpr int CB = 8192 / S;                        // columns per workgroup
+};
+
+template <int S>
+__global__ __launch_bounds__(NT) void tile_fft_rows(PixSrc ps, RingDst rd, int tps, int tiles,
+                                                   const float2 *__restrict__ twg) {
+    using C = LargeCfg<S>;
+    constexpr int H = C::H, P = C::P, GROUPS = H / P;
+    __shared__ float2 buf[P * S];
+    __shared__ float2 tw[S];
+    const int tid = threadIdx.x;
+    const int g = blockIdx.x % GROUPS;
+    const int tile = (blockIdx.x / GROUPS) % tiles;
+    const int f = blockIdx.x / (GROUPS * tiles);
+    for (int i = tid; i < S; i += NT) tw[i] = twg[S + i];
+    const uint8_t *fr = ps.frames + (long long)f * ps.frame_bytes;
+    const int x0 = ps.x_off + (tile % tps) * S;
+    const int y0 = ps.y_off + (tile / tps) * S + 2 * g * P;
+    for (int i = tid; i < P * S; i += NT) {
+        const int r = i / S, x = i % S;
+        buf[i] = make_float2(load_px(fr, ps, x0 + x, y0 + 2 * r), load_px(fr, ps, x0 + x, y0 + 2 * r + 1));
+    }
+    __syncthreads();
+    using Rows = Layout<S, P, P, S, 0, 1, false>;
+    fft_passes<Rows, 1>(buf, tw);
+    float2 *out = tile_out(rd, f, tile, (long long)S * (H + 1));
+    for (int i = tid; i < P * H; i += NT) {
+        const int r = i / H, k = i % H;
+        const float2 za = buf[r * S + k];
+        const float2 zb = buf[r * S + (k == 0 ? H : S - k)];
+        float2 ya, yb;
+        if (k == 0) {
+            ya = make_float2(za.x, zb.x);
+            yb = make_float2(za.y, zb.y);
+        } else {
+            ya = make_float2(0.5f * (za.x + zb.x), 0.5f * (za.y - zb.y));
+            yb = make_float2(0.5f * (za.y + zb.y), -0.5f * (za.x - zb.x));
+        }
+        const int ly = 2 * (g * P + r);
+        out[(long long)ly * (H + 1) + k] = ya;
+        out[(long long)(ly + 1) * (H + 1) + k] = yb;
+    }
+}
+
+// S >= 256, pass 2: CB columns per workgroup, FFT along ky, in place.
+template <int S>
+__global__ __launch_bounds__(NT) void tile_fft_cols(RingDst rd, int tiles, const float2 *__restrict__ twg) {
+    using C = LargeCfg<S>;
+    constexpr int H = C::H, CB = C::CB, NB = H / CB;
+    __shared__ float2 buf[S * CB];
+    __shared__ float2 tw[S];
+    const int tid = threadIdx.x;
+    const int cb = blockIdx.x % NB;
+    const int tile = (blockIdx.x / NB) % tiles;
+    const int f = blockIdx.x / (NB * tiles);
+    for (int i = tid; i < S; i += NT) tw[i] = twg[S + i];
+    float2 *t = tile_out(rd, f, tile, (long long)S * (H + 1));
+    const int c0 = cb * CB;
+    for (int i = tid; i < S * CB; i += NT) {
+        const int ly = i / CB, c = i % CB;
+        buf[i] = t[(long long)ly * (H + 1) + c0 + c];
+    }
+    __syncthreads();
+    using Cols = Layout<S, CB, CB, 1, 0, CB, true>;
+    fft_passes<Cols, 1>(buf, tw);
+    for (int i = tid; i < S * CB; i += NT) {
+        const int ky = i / CB, c = i % CB;
+        if (c0 + c == 0) {
+            const float2 w = buf[ky * CB];
+            const float2 v = buf[((S - ky) & (S - 1)) * CB];
+            t[(long long)ky * (H + 1)] = make_float2(0.5f * (w.x + v.x), 0.5f * (w.y - v.y));
+            t[(long long)ky * (H + 1) + H] = make_float2(0.5f * (w.y + v.y), -0.5f * (w.x - v.x));
+        } else {
+            t[(long long)ky * (H + 1) + c0 + c] = buf[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lag engine.  One thread per complex spectrum element; the J new frames
+// of the block stay in registers (and in the thread's own LDS column, for
+// partners inside the block), partners before the block come from the ring.
+// For every lag the block's terms are summed in FP32 registers and added to
+// the accumulator once: acc[l][e] += sum_j |F_{t0+j} - F_{t0+j-tau_l}|^2.
+// ---------------------------------------------------------------------------
+template <int J>
+__global__ __launch_bounds__(NT) void lag_kernel(const float2 *__restrict__ ring, float *__restrict__ acc,
+                                                long long E, int R, int slot0, long long t0, int n_new,
+                                                const int *__restrict__ lags, int L) {
+    __shared__ float2 hist[J * NT];
+    const int tid = threadIdx.x;
+    const long long e = (long long)blockIdx.x * NT + tid;
+    if (e >= E) return;
+    float2 nv[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        if (j < n_new) {
+            int s = slot0 + j;
+            s -= (s >= R) ? R : 0;
+            nv[j] = ring[(long long)s * E + e];
+            hist[j * NT + tid] = nv[j];
+        } else {
+            nv[j] = make_float2(0.f, 0.f);
+        }
+    }
+    for (int l = 0; l < L; ++l) {
+        const int tau = lags[l];
+        float sum = 0.f;
+        // partners before the block, from the ring
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            if (j < n_new && j < tau && t0 + j - tau >= 0) {
+                int s = slot0 + j - tau;
+                s += (s < 0) ? R : 0;
+                const float2 p = ring[(long long)s * E + e];
+                const float dx = nv[j].x - p.x, dy = nv[j].y - p.y;
+                sum += dx * dx + dy * dy;
+            }
+        }
+        // partners inside the block, from this thread's LDS column
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            if (j < n_new && j >= tau) {
+                const float2 p = hist[(j - tau) * NT + tid];
+                const float dx = nv[j].x - p.x, dy = nv[j].y - p.y;
+                sum += dx * dx + dy * dy;
+            }
+        }
+        acc[(long long)l * E + e] += sum;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Masked azimuthal average.  One workgroup per output value
+// results[scale][tile][q][lag]; the q ring's element list is summed in FP64
+// and normalised: I = 2 * sum * s^-4 / count * norm[lag].
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void azimuthal_kernel(const float *__restrict__ acc, long long E,
+                                                      const AzScale *__restrict__ scales, int S,
+                                                      const int *__restrict__ q_off,
+                                                      const int *__restrict__ q_cnt,
+                                                      const int *__restrict__ idx,
+                                                      const double *__restrict__ norm, int Q, int L,
+                                                      double *__restrict__ out, long long first_item) {
+    __shared__ double part[NT / 64];
+    const int tid = threadIdx.x;
+    const long long item = first_item + blockIdx.x;
+    int si = 0;
+    while (si + 1 < S && item >= scales[si + 1].out_base) ++si;
+    const AzScale sc = scales[si];
+    const long long r = item - sc.out_base;
+    const int l = (int)(r % L);
+    const int q = (int)((r / L) % Q);
+    const long long tile = r / ((long long)L * Q);
+    const int cnt = q_cnt[si * Q + q];
+    if (cnt == 0) {
+        if (tid == 0) out[item] = 0.0;
+        return;
+    }
+    const int *ix = idx + q_off[si * Q + q];
+    const float *a = acc + (long long)l * E + sc.elem_off + tile * sc.tile_elems;
+    double sum = 0.0;
+    for (int i = tid; i < cnt; i += NT) sum += (double)a[ix[i]];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_down(sum, o, 64);
+    if ((tid & 63) == 0) part[tid >> 6] = sum;
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < NT / 64; ++w) s += part[w];
+        out[item] = 2.0 * s * sc.inv_s4 / (double)cnt * norm[l];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static std::mutex g_tw_mu;
+static float2 *g_tw[64] = {nullptr};
+
+hipError_t twiddle_table(const float2 **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    if (!g_tw[dev]) {
+        std::vector<float> h;
+        fill_twiddles(&h);
+        float2 *d = nullptr;
+        e = hipMalloc(&d, h.size() * sizeof(float));
+        if (e != hipSuccess) return e;
+        e = hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(d);
+            return e;
+        }
+        g_tw[dev] = d;
+    }
+    *out = g_tw[dev];
+    return hipSuccess;
+}
+
+template <int S>
+static void launch_small(const PixSrc &ps, const RingDst &rd, int tps, int tiles, int n_frames,
+                         const float2 *tw, hipStream_t st) {
+    constexpr int TPW = SmallCfg<S>::TPW;
+    const int groups = (tiles + TPW - 1) / TPW;
+    hipLaunchKernelGGL(tile_fft_small<S>, dim3((unsigned)(groups * n_frames)), dim3(NT), 0, st, ps, rd, tps,
+                       tiles, groups, tw);
+}
+
+template <int S>
+static void launch_large(const PixSrc &ps, const RingDst &rd, int tps, int tiles, int n_frames,
+                         const float2 *tw, hipStream_t st) {
+    using C = LargeCfg<S>;
+    const unsigned g1 = (unsigned)((C::H / C::P) * tiles * n_frames);
+    hipLaunchKernelGGL(tile_fft_rows<S>, dim3(g1), dim3(NT), 0, st, ps, rd, tps, tiles, tw);
+    const unsigned g2 = (unsigned)((C::H / C::CB) * tiles * n_frames);
+    hipLaunchKernelGGL(tile_fft_cols<S>, dim3(g2), dim3(NT), 0, st, rd, tiles, tw);
+}
+
+hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st) {
+    const float2 *tw = nullptr;
+    hipError_t e = twiddle_table(&tw);
+    if (e != hipSuccess) return e;
+    PixSrc ps{(const uint8_t *)a.frames, a.frame_bytes, (long long)a.width * a.bpp, a.bpp, a.fmt, a.x_off,
+              a.y_off};
+    RingDst rd{(float2 *)a.ring, a.elems_per_frame, a.elem_off, a.ring_slots, a.first_slot};
+    const int tps = a.largest_scale / a.scale, tiles = tps * tps;
+    switch (a.scale) {
+    case 2: launch_small<2>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
+    case 4: launch_small<4>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
+    case 8: launch_small<8>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
+    case 16: launch_small<16>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
+    case 32: launch_small<32>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
+    case 64: launch_small<64>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
+    case 128: launch_small<128>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
+    case 256: launch_large<256>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
+    case 512: launch_large<512>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
+    case 1024: launch_large<1024>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
+    case 2048: launch_large<2048>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_lag(const LagArgs &a, hipStream_t st) {
+    const unsigned grid = (unsigned)((a.elems_per_frame + NT - 1) / NT);
+    if (a.lag_block == 16)
+        hipLaunchKernelGGL(lag_kernel<16>, dim3(grid), dim3(NT), 0, st, (const float2 *)a.ring, a.acc,
+                           a.elems_per_frame, a.ring_slots, a.slot0, a.t0, a.n_new, a.lags, a.lag_count);
+    else if (a.lag_block == 32)
+        hipLaunchKernelGGL(lag_kernel<32>, dim3(grid), dim3(NT), 0, st, (const float2 *)a.ring, a.acc,
+                           a.elems_per_frame, a.ring_slots, a.slot0, a.t0, a.n_new, a.lags, a.lag_count);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_azimuthal(const AzArgs &a, hipStream_t st) {
+    // grid.x is limited to 2^31 - 1 workgroups; split larger result sets
+    const long long maxg = 1LL << 30;
+    for (long long first = 0; first < a.result_count; first += maxg) {
+        const long long n = (a.result_count - first) < maxg ? (a.result_count - first) : maxg;
+        hipLaunchKernelGGL(azimuthal_kernel, dim3((unsigned)n), dim3(NT), 0, st, a.acc, a.elems_per_frame,
+                           a.scales, a.scale_count, a.q_off, a.q_cnt, a.idx, a.norm, a.lambda_count,
+                           a.lag_count, a.out, first);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace ddm

@@ -1,0 +1,228 @@
+/*
+ * ddm_amd.h -- C ABI of the MI355X (gfx950) multi-scale DDM engine.
+ *
+ * The engine replaces the analysis path of zhangyuyi99/cuda_multiDDM:
+ * runDDM(...) (declared src/main.cpp:40-64, defined src/DDM.cu:268-292),
+ * i.e. tile cut -> 2-D R2C FFT -> lag difference -> masked azimuthal average,
+ * behind the reference's ./ddm CLI (src/main.cpp:104-342).  runDDM owns every
+ * buffer and runs a whole video in one call; this ABI splits the same work
+ * into create / submit frames / read results so that the CLI, tests and a
+ * serving loop can drive it.  Plain C types only.
+ *
+ * Conventions
+ *   - Every function returns DDM_OK (0) or a negative DDM_E* code.  The
+ *     message of the last failure is available from ddm_last_error().  No
+ *     function terminates the process (the reference calls exit(), see
+ *     src/debug.cpp:10-15, 33-45).
+ *   - The engine owns all device memory and its pinned staging buffers; the
+ *     caller owns descriptors, parameter arrays and output arrays.
+ *     Parameter arrays are copied and sorted inside ddm_create; the caller's
+ *     arrays are never modified (the reference sorts them in place,
+ *     src/DDM.cu:319-321).
+ *   - One engine is used from one host thread at a time.  Several engines
+ *     may live in one process and share the GPU.
+ *   - Result layout: results[scale][tile][q][lag] with scales descending,
+ *     tiles in the reference order (tile = ty * (M/s) + tx, M = largest
+ *     scale), q in ascending-lambda order, lags ascending.
+ */
+#ifndef DDM_AMD_H
+#define DDM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DDM_ABI_VERSION 1
+
+/* status codes */
+#define DDM_OK 0
+#define DDM_EINVAL (-1)   /* invalid argument */
+#define DDM_ENOMEM (-2)   /* host or device allocation failed */
+#define DDM_EDEVICE (-3)  /* HIP runtime error / no device */
+#define DDM_EROI (-4)     /* ROI (largest scale at x_off, y_off) outside the frame */
+#define DDM_ESCALE (-5)   /* scale not a power of two, or repeated */
+#define DDM_ELAG (-6)     /* lag negative or repeated */
+#define DDM_ELAMBDA (-7)  /* lambda negative or repeated */
+#define DDM_ESTATE (-8)   /* call not valid in the engine's current state */
+#define DDM_ERANGE (-9)   /* frame or index outside what the engine holds */
+
+/* Pixel formats.  A pixel is bytes_per_pixel bytes; only channel 0 is used.
+ * DDM_PIX_U8 reads byte 0 of each pixel (parseBufferScalePow2,
+ * src/DDM_kernel.cuh:74-75).  The 16-bit formats read bytes 0-1 as one
+ * value (the reference truncates 16-bit input to its low byte). */
+enum ddm_pixel_format { DDM_PIX_U8 = 0, DDM_PIX_U16LE = 1, DDM_PIX_U16BE = 2 };
+
+/* Normalisation of I(q, tau).  DDM_NORM_FRAMES multiplies by 1/N with N the
+ * frames submitted since the last reset, as the reference does
+ * (src/DDM.cu:92, 875).  DDM_NORM_PAIRS divides by the number of pairs
+ * (t, t + tau) summed for each lag instead. */
+enum ddm_norm_mode { DDM_NORM_FRAMES = 0, DDM_NORM_PAIRS = 1 };
+
+typedef struct ddm_desc {
+    int width;            /* frame width in pixels */
+    int height;           /* frame height in pixels */
+    int bytes_per_pixel;  /* pixel stride in bytes (info.bpp) */
+    int pixel_format;     /* enum ddm_pixel_format */
+    int x_off, y_off;     /* ROI origin (-x / -y) */
+    const int *scales;    /* tile sizes, powers of two (-S file) */
+    int scale_count;
+    const int *lags;      /* tau values in frames (-T file); may exceed chunk_frames */
+    int lag_count;
+    const float *lambdas; /* wavelengths in pixels (-Q file); q = scale / lambda */
+    int lambda_count;
+    float mask_tolerance; /* ring [q, tol * q] (-t: 1 + percent/100) */
+    int chunk_frames;     /* frames per upload / FFT batch (-C) */
+    int norm_mode;        /* enum ddm_norm_mode */
+    int lag_block;        /* frames per lag-engine pass: 0 = automatic, else 16 or 32 */
+} ddm_desc;
+
+/* Sizes the engine derived from a descriptor (after sorting). */
+typedef struct ddm_info {
+    int largest_scale;         /* M */
+    int scale_count, lag_count, lambda_count;
+    int ring_slots;            /* spectra kept resident: chunk_frames + max lag */
+    long long elems_per_frame; /* complex spectrum elements per frame, all scales */
+    long long result_count;    /* sum over scales of tiles * lambda_count * lag_count */
+    long long frames_submitted;   /* since creation */
+    long long frames_since_reset; /* since creation or the last ddm_reset */
+    size_t device_bytes;       /* device memory held by the engine */
+    size_t pinned_bytes;       /* pinned host memory held by the engine */
+} ddm_info;
+
+/* Per-stage device time, collected with HIP events on the engine's stream
+ * when timing is on (ddm_set_timing). */
+typedef struct ddm_stage_stats {
+    double fft_ms;        /* tiler + FFT kernels */
+    double lag_ms;        /* lag-engine kernels */
+    double azimuthal_ms;  /* masked azimuthal average */
+    long long fft_launches, lag_launches, azimuthal_launches;
+    long long chunks;     /* chunk latencies recorded */
+} ddm_stage_stats;
+
+typedef struct ddm_engine ddm_engine;
+
+/* Validates the descriptor (before touching the device), then allocates the
+ * spectrum ring, accumulators and q-ring tables on the current device.
+ * Replaces the setup half of runDDM (src/DDM.cu:294-762). */
+int ddm_create(const ddm_desc *desc, ddm_engine **out);
+
+/* Validation only, no device access: returns the code ddm_create would
+ * return for this descriptor and fills info (may be NULL). */
+int ddm_validate(const ddm_desc *desc, ddm_info *info, char *msg, size_t msg_len);
+
+void ddm_destroy(ddm_engine *eng);
+
+/* Appends n_frames frames (n_frames * width * height * bytes_per_pixel bytes,
+ * row-major) from host memory.  Copies the ROI rows into the engine's pinned
+ * double buffers and uploads them asynchronously.  Replaces the chunk loop
+ * of runDDM (src/DDM.cu:765-843): H2D, parseChunk, analyseChunk. */
+int ddm_submit_host(ddm_engine *eng, const void *frames, int n_frames);
+
+/* Same as ddm_submit_host for frames already in device memory.  The buffer
+ * must stay valid until ddm_sync returns. */
+int ddm_submit_device(ddm_engine *eng, const void *frames, int n_frames);
+
+/* Waits for all submitted work. */
+int ddm_sync(ddm_engine *eng);
+
+/* Masked azimuthal average of everything accumulated since the last reset:
+ * results[info.result_count] doubles, layout [scale][tile][q][lag].
+ * Replaces analyse_accums / analyseFFTDevice (src/DDM.cu:72-146,
+ * src/azimuthal_average.cu:292-346). */
+int ddm_results(ddm_engine *eng, double *results);
+
+/* Zeroes the lag accumulators and the frame count used for normalisation;
+ * the spectrum ring is kept, so pairs reaching back across the reset are
+ * still formed.  The -G purge of src/DDM.cu:800-826. */
+int ddm_reset(ddm_engine *eng);
+
+int ddm_get_info(const ddm_engine *eng, ddm_info *info);
+
+/* Sorted parameters as the engine uses them. */
+int ddm_get_params(const ddm_engine *eng, int *scales, int *lags, float *lambdas);
+
+/* Lag sums of one scale, [lag][tile][ky][kx] (kx <= s/2), multiplied by
+ * 1/s^4 like the reference accumulator (src/DDM_kernel.cuh:95-98).
+ * Backs the -K dump. */
+int ddm_get_accumulators(ddm_engine *eng, int scale_idx, float *out);
+
+/* Spectrum of one frame still held in the ring, one scale,
+ * [tile][ky][kx][re, im] unnormalised.  Backs the -A dump and FFT tests. */
+int ddm_get_spectra(ddm_engine *eng, long long frame, int scale_idx, float *out);
+
+int ddm_set_timing(ddm_engine *eng, int on);
+int ddm_get_stage_stats(ddm_engine *eng, ddm_stage_stats *stats);
+/* Device time of each submitted chunk (tiler+FFT+lag), in submission order;
+ * returns the number written (<= max). */
+int ddm_get_chunk_times(ddm_engine *eng, float *ms, int max);
+
+/* Message of the last failure of eng; with eng == NULL, of the last
+ * ddm_create on the calling thread. */
+const char *ddm_last_error(const ddm_engine *eng);
+const char *ddm_status_string(int code);
+
+/* -------------------------------------------------------------------------
+ * Stage entry points on caller-owned device buffers (the kernels the
+ * engine launches).  `stream` is a hipStream_t (NULL = default stream).
+ * Ring layout: ring[slot][elem] complex float (interleaved), slot = frame
+ * index mod ring_slots; elem runs over all scales' tiles of a frame.
+ * ------------------------------------------------------------------------- */
+
+/* Tile cut + unnormalised 2-D R2C FFT of n_frames frames for one scale s
+ * (parseBufferScalePow2 + cufftExecR2C, src/DDM.cu:188-192).  Frame f is at
+ * frames + f * frame_bytes; its spectra go to ring slot (first_frame + f) mod
+ * ring_slots at element offset elem_off, layout [tile][ky][kx]. */
+int ddm_k_tile_fft(const void *frames, int n_frames, long long frame_bytes,
+                   int width, int height, int bytes_per_pixel, int pixel_format,
+                   int x_off, int y_off, int largest_scale, int scale,
+                   float *ring, int ring_slots, long long elems_per_frame,
+                   long long elem_off, long long first_frame, void *stream);
+
+/* Lag engine (processFFT / analyseChunk, src/DDM_kernel.cuh:85-101,
+ * src/DDM.cu:222-262): for the n_new frames first_frame .. first_frame+n_new-1
+ * held in the ring and every lag l, acc[l][e] += sum |F_t[e] - F_{t-lag}[e]|^2
+ * over those t with t - lag >= 0.  No 1/s^2 factor is applied.  lags is a
+ * device array of lag_count ints.  n_new <= lag_block (16 or 32). */
+int ddm_k_lag_accumulate(const float *ring, int ring_slots, long long elems_per_frame,
+                         long long first_frame, int n_new, const int *lags, int lag_count,
+                         int max_lag, float *acc, int lag_block, void *stream);
+
+/* -------------------------------------------------------------------------
+ * Host utilities (no device access).
+ * ------------------------------------------------------------------------- */
+
+/* q-ring membership of scale s for one lambda, computed in FP32 as
+ * buildAzimuthMask does (src/azimuthal_average.cu:229-283) with
+ * q = float(s) / lambda (src/DDM.cu:102).  mask[ky][kx], kx <= s/2.
+ * Returns the member count (>= 0) or a negative code. */
+int ddm_ring_mask(int scale, float lambda, float tolerance, uint8_t *mask);
+
+/* Deterministic synthetic video.  Frames are written as n_frames * h * w
+ * pixels of 1 byte (bits <= 8) or 2 bytes little-endian (bits > 8).
+ * Uniform: i.i.d. values in [0, 2^bits - 1] from a counter-based hash of
+ * (seed, frame, pixel). */
+int ddm_synth_uniform(void *out, int n_frames, int width, int height, int bits,
+                      uint64_t seed, long long first_frame);
+
+/* Brownian spots: n_spots Gaussian spots (sigma_px, amplitude) on
+ * `background`, per-axis step N(0, 2*D) per frame, periodic wrap, plus
+ * N(0, noise_sigma) pixel noise, rounded and clipped to [0, 2^bits - 1]. */
+int ddm_synth_brownian(void *out, int n_frames, int width, int height, int bits,
+                       int n_spots, float sigma_px, float amplitude, float diffusion,
+                       float background, float noise_sigma, uint64_t seed, int nthreads);
+
+/* Writes one I(lambda, tau) file in the reference layout (writeIqtToFile,
+ * src/azimuthal_average.cu:112-150): lambdas, then tau/fps, then one line
+ * of lag values per lambda; values space-separated with a trailing space. */
+int ddm_write_iqt_file(const char *path, const double *iqt_q_lag,
+                       const float *lambdas, int lambda_count,
+                       const int *lags, int lag_count, float fps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DDM_AMD_H */

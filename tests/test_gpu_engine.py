@@ -1,0 +1,147 @@
+"""The whole HIP path (tile -> FFT -> lag engine -> azimuthal average)
+through the C ABI, against the FP64 CPU oracle on the same seeded inputs.
+Tolerance: max relative I(q, tau) error vs FP64 <= 1e-5 for the plain FP32
+path (entries that are exactly 0 in FP64 -- empty rings, lags without
+pairs -- must be exactly 0)."""
+import numpy as np
+import pytest
+
+import cuda_multiddm_amd as ddm
+from oracle import oracle as o
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _both(frames, scales, lags, lambdas, tol=1.2, chunk=30, bpp=1, pixfmt=ddm.PIX_U8, x_off=0, y_off=0,
+          norm=ddm.NORM_FRAMES, device_input=False, lag_block=0):
+    n, h, w = frames.shape[0], frames.shape[1], frames.shape[2]
+    raw = np.ascontiguousarray(frames).view(np.uint8)
+    with ddm.Engine(w, h, scales, lags, lambdas, tol, chunk, bpp, pixfmt, x_off, y_off, norm, lag_block) as e:
+        print(f"engine {w}x{h} n={n} scales={e.scales} lags={len(e.lags)} chunk={chunk}", flush=True)
+        if device_input:
+            import torch
+            t = torch.from_numpy(raw.reshape(-1).copy()).cuda()
+            e.submit_device(t.data_ptr(), n)
+            e.sync()
+        else:
+            e.submit_host(raw)
+        got = e.results()
+    cfg = o.Config(w, h, n, scales, lags, lambdas, np.float32(tol).item(), bpp, pixfmt, x_off, y_off, norm)
+    ref = o.run(cfg, raw)
+    return got, ref
+
+
+def test_config1_uniform_128(gpu):
+    """64 frames 128^2 uint8 uniform, scale 128, tau 1..16, 20 lambdas 4..64, tol 20%, chunk 30."""
+    frames = ddm.synth_uniform(64, 128, 128, 8, seed=1)
+    got, ref = _both(frames, [128], list(range(1, 17)), ddm.lin_lambdas(4, 64, 20), 1.2, 30)
+    assert got.shape == ref.shape
+    err = o.max_rel_err(got, ref)
+    print("config1 max rel err", err)
+    assert err <= TOL
+
+
+def test_multiscale_u16_lags_beyond_chunk(gpu):
+    """12-bit values in uint16 (little endian), ROI at an offset, four scales,
+    lags larger than the chunk, lambdas giving empty rings, a ragged last chunk."""
+    rng = np.random.default_rng(2)
+    frames = rng.integers(0, 4096, (70, 75, 70)).astype("<u2")
+    lags = [1, 2, 3, 7, 16, 17, 30, 45, 69, 80]
+    lams = ddm.log_lambdas(2, 90, 12)
+    got, ref = _both(frames, [8, 64, 32, 16], lags, lams, 1.2, 16, bpp=2, pixfmt=ddm.PIX_U16LE, x_off=5,
+                     y_off=7)
+    err = o.max_rel_err(got, ref)
+    print("multiscale max rel err", err)
+    assert err <= TOL
+    # lag 80 > N - 1: no pair, exactly zero
+    assert np.all(got.reshape(-1, len(lags))[:, -1] == 0)
+
+
+def test_three_byte_pixels_channel0_big_endian(gpu):
+    rng = np.random.default_rng(3)
+    rgb = rng.integers(0, 256, (40, 40, 36, 3), dtype=np.uint8)
+    got, ref = _both(rgb.reshape(40, 40, 36 * 3), [32, 16], [1, 3, 9, 33], [2.0, 3.0, 5.0, 8.0], 1.2, 11, bpp=3,
+                     x_off=4, y_off=2)
+    assert o.max_rel_err(got, ref) <= TOL
+    be = rng.integers(0, 65535, (33, 32, 32)).astype(">u2")
+    got, ref = _both(be, [32], [1, 2, 4, 8], [2.0, 4.0, 6.0], 1.2, 10, bpp=2, pixfmt=ddm.PIX_U16BE)
+    assert o.max_rel_err(got, ref) <= TOL
+
+
+def test_pair_mean_normalisation(gpu):
+    frames = ddm.synth_uniform(50, 64, 64, 8, seed=4)
+    got, ref = _both(frames, [64, 32], [1, 5, 20, 49], [3.0, 6.0, 12.0], 1.2, 20, norm=ddm.NORM_PAIRS)
+    assert o.max_rel_err(got, ref) <= TOL
+
+
+def test_host_and_device_submission_identical(gpu):
+    frames = ddm.synth_uniform(45, 64, 64, 8, seed=5)
+    a, _ = _both(frames, [64, 16], [1, 2, 7, 31], [2.0, 5.0], 1.2, 30)
+    b, ref = _both(frames, [64, 16], [1, 2, 7, 31], [2.0, 5.0], 1.2, 30, device_input=True)
+    assert np.array_equal(a, b)
+    c, _ = _both(frames, [64, 16], [1, 2, 7, 31], [2.0, 5.0], 1.2, 30, device_input=True)
+    assert np.array_equal(b, c)  # deterministic
+    d, _ = _both(frames, [64, 16], [1, 2, 7, 31], [2.0, 5.0], 1.2, 7, lag_block=16)
+    assert o.max_rel_err(d, ref) <= TOL  # other chunking: same answer within tolerance
+
+
+def test_spectra_and_accumulators(gpu):
+    frames = ddm.synth_uniform(20, 64, 64, 8, seed=6)
+    lags = [1, 3, 8]
+    with ddm.Engine(64, 64, [64, 32], lags, [4.0, 8.0], 1.2, 8) as e:
+        e.submit_host(frames)
+        spec = e.spectra(19, 1)  # scale 32, last frame
+        for tile in range(4):
+            ty, tx = divmod(tile, 2)
+            ref = np.fft.rfft2(frames[19, ty * 32:(ty + 1) * 32, tx * 32:(tx + 1) * 32].astype(float))
+            assert np.max(np.abs(spec[tile] - ref)) / np.max(np.abs(ref)) < 4e-6
+        with pytest.raises(ddm.DDMError):
+            e.spectra(0, 0)  # no longer held: the ring keeps chunk + max lag frames
+        acc = e.accumulators(0)
+    cfg = o.Config(64, 64, 20, [64, 32], lags, [4.0, 8.0])
+    _, ref_acc = o.run(cfg, frames, acc_scale=0)
+    assert acc.shape == ref_acc.shape
+    assert np.max(np.abs(acc - ref_acc)) / np.max(np.abs(ref_acc)) < 1e-5
+
+
+def test_reset_window(gpu):
+    """After a reset the engine reports pairs whose later frame arrived since
+    the reset, normalised by the frames since the reset:
+    I_window = (b I_b - a I_a) / (b - a) with I_a, I_b the oracle over [0, a), [0, b)."""
+    frames = ddm.synth_uniform(60, 32, 32, 8, seed=7)
+    lags, lams = [1, 2, 5, 12], [2.0, 3.0, 6.0]
+    with ddm.Engine(32, 32, [32, 16], lags, lams, 1.2, 10) as e:
+        e.submit_host(frames[:30])
+        first = e.results()
+        e.reset()
+        e.submit_host(frames[30:])
+        win = e.results()
+        assert e.info().frames_since_reset == 30
+    Ia = o.run(o.Config(32, 32, 30, [32, 16], lags, lams), frames[:30])
+    Ib = o.run(o.Config(32, 32, 60, [32, 16], lags, lams), frames)
+    assert o.max_rel_err(first, Ia) <= TOL
+    want = (60 * Ib - 30 * Ia) / 30
+    assert np.max(np.abs(win - want) / np.maximum(np.abs(want), 1e-30)) <= 1e-4
+
+
+def test_brownian_256_multiscale(gpu):
+    """Reduced single-stream Brownian case: 128 frames 256^2, scales 256..32,
+    log-spaced lags to 100 (beyond the 30-frame chunk), log lambdas 4..128."""
+    frames = ddm.synth_brownian(128, 256, 256, 8, n_spots=500, sigma=2.0, amplitude=80.0, diffusion=0.5,
+                                background=100.0, noise=5.0, seed=8)
+    got, ref = _both(frames, [256, 128, 64, 32], ddm.log_lags(100, 20), ddm.log_lambdas(4, 128, 16), 1.2, 30)
+    err = o.max_rel_err(got, ref)
+    print("brownian 256 max rel err", err)
+    assert err <= TOL
+
+
+def test_create_rejects_invalid_descriptors(gpu):
+    with pytest.raises(ddm.DDMError) as ex:
+        ddm.Engine(64, 64, [48], [1], [4.0])
+    assert "powers of two" in str(ex.value)
+    with pytest.raises(ddm.DDMError):
+        ddm.Engine(64, 64, [64], [1], [4.0], x_off=1)
+    with pytest.raises(ddm.DDMError):
+        ddm.Engine(64, 64, [64], [2, 2], [4.0])

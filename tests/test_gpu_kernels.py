@@ -1,0 +1,145 @@
+"""Stage kernels through the C ABI on caller-owned device buffers:
+tile cut + FFT against numpy (FP64) with the DC bin bit-exact, and the lag
+engine bit-exact on integer-valued spectra injected into the ring."""
+import numpy as np
+import pytest
+
+import cuda_multiddm_amd as ddm
+
+pytestmark = pytest.mark.gpu
+
+# FP32 FFT: error relative to the largest bin, bound ~ log2(s^2) * 2^-24
+FFT_TOL = 4e-6
+
+
+def _run_tile_fft(gpu, frames, bpp, fmt, M, s, x_off=0, y_off=0, ring_slots=None, first_frame=0):
+    import torch
+    n, h, w = frames.shape[0], frames.shape[1], frames.shape[2]
+    raw = np.ascontiguousarray(frames).view(np.uint8).reshape(n, -1)
+    frame_bytes = raw.shape[1]
+    T = (M // s) ** 2
+    E = T * s * (s // 2 + 1)
+    R = ring_slots or n
+    d_frames = torch.from_numpy(raw.copy()).to(gpu)
+    ring = torch.zeros(R * E * 2, dtype=torch.float32, device=gpu)
+    print(f"tile_fft M={M} s={s} n={n} fmt={fmt} bpp={bpp}", flush=True)
+    ddm.k_tile_fft(d_frames, n, frame_bytes, w, h, bpp, fmt, x_off, y_off, M, s, ring, R, E, 0, first_frame)
+    torch.cuda.synchronize()
+    r = ring.cpu().numpy().reshape(R, T, s, s // 2 + 1, 2)
+    return r[..., 0].astype(np.float64) + 1j * r[..., 1]
+
+
+def _tile(img, M, s, tile, x_off, y_off):
+    tps = M // s
+    ty, tx = divmod(tile, tps)
+    return img[y_off + ty * s:y_off + (ty + 1) * s, x_off + tx * s:x_off + (tx + 1) * s].astype(np.float64)
+
+
+@pytest.mark.parametrize("s", [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048])
+def test_tile_fft_single_tile(gpu, s):
+    rng = np.random.default_rng(s)
+    n = 2 if s >= 1024 else 3
+    frames = rng.integers(0, 256, (n, s, s), dtype=np.uint8)
+    spec = _run_tile_fft(gpu, frames, 1, ddm.PIX_U8, s, s)
+    for f in range(n):
+        ref = np.fft.rfft2(frames[f].astype(np.float64))
+        err = np.max(np.abs(spec[f, 0] - ref)) / np.max(np.abs(ref))
+        assert err < FFT_TOL, (s, f, err)
+        # DC is a sum of integers < 2^24: exact in FP32
+        assert spec[f, 0, 0, 0] == frames[f].astype(np.int64).sum()
+
+
+@pytest.mark.parametrize("M,s", [(64, 8), (64, 16), (128, 32), (256, 64), (256, 128), (512, 256), (1024, 512)])
+def test_tile_fft_tiles_offsets_and_ring_wrap(gpu, M, s):
+    rng = np.random.default_rng(M + s)
+    n, x_off, y_off = 3, 5, 3
+    frames = rng.integers(0, 256, (n, M + 9, M + 7), dtype=np.uint8)
+    R, first = 4, 2  # frames land in slots 2, 3, 0
+    spec = _run_tile_fft(gpu, frames, 1, ddm.PIX_U8, M, s, x_off, y_off, R, first)
+    for f in range(n):
+        slot = (first + f) % R
+        for tile in range((M // s) ** 2):
+            t = _tile(frames[f], M, s, tile, x_off, y_off)
+            ref = np.fft.rfft2(t)
+            got = spec[slot, tile]
+            assert got[0, 0] == t.sum(), (f, tile)  # tile membership, bit-exact
+            assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < FFT_TOL
+
+
+@pytest.mark.parametrize("fmt,bpp", [(ddm.PIX_U16LE, 2), (ddm.PIX_U16BE, 2), (ddm.PIX_U8, 3), (ddm.PIX_U16LE, 4)])
+def test_tile_fft_pixel_formats(gpu, fmt, bpp):
+    rng = np.random.default_rng(fmt * 10 + bpp)
+    M, s, n = 64, 32, 2
+    vals = rng.integers(0, 4096 if fmt != ddm.PIX_U8 else 256, (n, M, M + 4)).astype(np.uint32)
+    raw = np.zeros((n, M, M + 4, bpp), dtype=np.uint8)
+    if fmt == ddm.PIX_U8:
+        raw[..., 0] = vals
+        raw[..., 1:] = rng.integers(0, 256, raw[..., 1:].shape)  # other channels ignored
+    elif fmt == ddm.PIX_U16LE:
+        raw[..., 0], raw[..., 1] = vals & 0xFF, vals >> 8
+    else:
+        raw[..., 0], raw[..., 1] = vals >> 8, vals & 0xFF
+    if bpp > 2 and fmt != ddm.PIX_U8:
+        raw[..., 2:] = 77
+    spec = _run_tile_fft(gpu, raw.reshape(n, M, -1), bpp, fmt, M, s, x_off=4)
+    spec = spec.reshape(n, 4, s, s // 2 + 1)
+    for f in range(n):
+        for tile in range(4):
+            t = _tile(vals[f], M, s, tile, 4, 0)
+            assert spec[f, tile, 0, 0] == t.sum()
+            ref = np.fft.rfft2(t)
+            assert np.max(np.abs(spec[f, tile] - ref)) / np.max(np.abs(ref)) < FFT_TOL
+
+
+def _lag_expected(ring, R, t0, n_new, lags):
+    E = ring.shape[1]
+    acc = np.zeros((len(lags), E), dtype=np.int64)
+    for j in range(n_new):
+        t = t0 + j
+        for li, tau in enumerate(lags):
+            if t - tau < 0:
+                continue
+            d = ring[t % R] - ring[(t - tau) % R]
+            acc[li] += (d.real.astype(np.int64) ** 2 + d.imag.astype(np.int64) ** 2)
+    return acc
+
+
+@pytest.mark.parametrize("J", [16, 32])
+@pytest.mark.parametrize("t0,n_new", [(0, None), (3, 7), (100, None), (57, 1)])
+def test_lag_engine_bit_exact(gpu, J, t0, n_new):
+    import torch
+    n_new = J if n_new is None else n_new
+    lags = [0, 1, 2, 5, 15, 16, 17, 31, 33, 40]
+    R = max(lags) + J + 3
+    E = 1000  # not a multiple of the 256-thread workgroup
+    rng = np.random.default_rng(t0 * 7 + J + n_new)
+    re = rng.integers(-8, 9, (R, E))
+    im = rng.integers(-8, 9, (R, E))
+    ring_np = re + 1j * im
+    ring = torch.from_numpy(np.stack([re, im], -1).astype(np.float32).reshape(-1)).to(gpu)
+    acc0 = rng.integers(0, 1000, (len(lags), E))
+    acc = torch.from_numpy(acc0.astype(np.float32).reshape(-1)).to(gpu)
+    d_lags = torch.tensor(lags, dtype=torch.int32, device=gpu)
+    print(f"lag J={J} t0={t0} n_new={n_new}", flush=True)
+    ddm.k_lag_accumulate(ring, R, E, t0, n_new, d_lags, max(lags), acc, J)
+    torch.cuda.synchronize()
+    got = acc.cpu().numpy().reshape(len(lags), E).astype(np.int64)
+    want = acc0 + _lag_expected(ring_np, R, t0, n_new, lags)
+    assert np.array_equal(got, want)
+
+
+def test_stage_entry_points_reject_bad_arguments(gpu):
+    import torch
+    ring = torch.zeros(16, dtype=torch.float32, device=gpu)
+    frames = torch.zeros(64, dtype=torch.uint8, device=gpu)
+    lags = torch.tensor([1], dtype=torch.int32, device=gpu)
+    with pytest.raises(ddm.DDMError):  # scale not a power of two
+        ddm.k_tile_fft(frames, 1, 64, 8, 8, 1, 0, 0, 0, 8, 6, ring, 1, 8, 0, 0)
+    with pytest.raises(ddm.DDMError):  # ROI outside the frame
+        ddm.k_tile_fft(frames, 1, 64, 8, 8, 1, 0, 1, 0, 8, 8, ring, 1, 40, 0, 0)
+    with pytest.raises(ddm.DDMError):  # spectra would overrun the ring element range
+        ddm.k_tile_fft(frames, 1, 64, 8, 8, 1, 0, 0, 0, 8, 8, ring, 1, 8, 0, 0)
+    with pytest.raises(ddm.DDMError):  # n_new above the lag block
+        ddm.k_lag_accumulate(ring, 4, 2, 0, 17, lags, 1, ring, 16)
+    with pytest.raises(ddm.DDMError):  # ring too short for max lag
+        ddm.k_lag_accumulate(ring, 4, 2, 0, 4, lags, 1, ring, 16)
