@@ -273,6 +273,7 @@ def run_ddm(frames: np.ndarray, width: int, height: int, scales: Sequence[int], 
         if device_input:
             import torch
             t = torch.from_numpy(np.ascontiguousarray(frames).view(np.uint8).reshape(-1)).cuda()
+            torch.cuda.synchronize()  # the engine's stream does not wait for torch's
             e.submit_device(t.data_ptr(), t.numel() // (width * height * bpp))
             e.sync()
             del t

@@ -23,6 +23,7 @@ def _both(frames, scales, lags, lambdas, tol=1.2, chunk=30, bpp=1, pixfmt=ddm.PI
         if device_input:
             import torch
             t = torch.from_numpy(raw.reshape(-1).copy()).cuda()
+            torch.cuda.synchronize()  # the engine's stream does not wait for torch's
             e.submit_device(t.data_ptr(), n)
             e.sync()
         else:
