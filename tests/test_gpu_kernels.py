@@ -12,9 +12,10 @@ pytestmark = pytest.mark.gpu
 FFT_TOL = 4e-6
 
 
-def _run_tile_fft(gpu, frames, bpp, fmt, M, s, x_off=0, y_off=0, ring_slots=None, first_frame=0):
+def _run_tile_fft(gpu, frames, bpp, fmt, M, s, x_off=0, y_off=0, ring_slots=None, first_frame=0, width=None):
     import torch
-    n, h, w = frames.shape[0], frames.shape[1], frames.shape[2]
+    n, h = frames.shape[0], frames.shape[1]
+    w = width or frames.shape[2]
     raw = np.ascontiguousarray(frames).view(np.uint8).reshape(n, -1)
     frame_bytes = raw.shape[1]
     T = (M // s) ** 2
@@ -45,8 +46,17 @@ def test_tile_fft_single_tile(gpu, s):
         ref = np.fft.rfft2(frames[f].astype(np.float64))
         err = np.max(np.abs(spec[f, 0] - ref)) / np.max(np.abs(ref))
         assert err < FFT_TOL, (s, f, err)
-        # DC is a sum of integers < 2^24: exact in FP32
-        assert spec[f, 0, 0, 0] == frames[f].astype(np.int64).sum()
+        total = frames[f].astype(np.int64).sum()
+        if total < 2 ** 24:  # DC is a sum of integers: exact in FP32 below 2^24
+            assert spec[f, 0, 0, 0] == total
+
+
+@pytest.mark.parametrize("s", [512, 1024, 2048])
+def test_tile_fft_dc_exact_large(gpu, s):
+    """Values 0..3 keep every partial sum of a tile below 2^24: DC bit-exact."""
+    frames = np.random.default_rng(s + 1).integers(0, 4, (1, s, s), dtype=np.uint8)
+    spec = _run_tile_fft(gpu, frames, 1, ddm.PIX_U8, s, s)
+    assert spec[0, 0, 0, 0] == frames[0].astype(np.int64).sum()
 
 
 @pytest.mark.parametrize("M,s", [(64, 8), (64, 16), (128, 32), (256, 64), (256, 128), (512, 256), (1024, 512)])
@@ -62,7 +72,8 @@ def test_tile_fft_tiles_offsets_and_ring_wrap(gpu, M, s):
             t = _tile(frames[f], M, s, tile, x_off, y_off)
             ref = np.fft.rfft2(t)
             got = spec[slot, tile]
-            assert got[0, 0] == t.sum(), (f, tile)  # tile membership, bit-exact
+            if t.sum() < 2 ** 24:
+                assert got[0, 0] == t.sum(), (f, tile)  # tile membership, bit-exact
             assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < FFT_TOL
 
 
@@ -70,6 +81,7 @@ def test_tile_fft_tiles_offsets_and_ring_wrap(gpu, M, s):
 def test_tile_fft_pixel_formats(gpu, fmt, bpp):
     rng = np.random.default_rng(fmt * 10 + bpp)
     M, s, n = 64, 32, 2
+    # 12-bit values: a 32x32 tile sums below 2^24, so DC is exact
     vals = rng.integers(0, 4096 if fmt != ddm.PIX_U8 else 256, (n, M, M + 4)).astype(np.uint32)
     raw = np.zeros((n, M, M + 4, bpp), dtype=np.uint8)
     if fmt == ddm.PIX_U8:
@@ -81,7 +93,7 @@ def test_tile_fft_pixel_formats(gpu, fmt, bpp):
         raw[..., 0], raw[..., 1] = vals >> 8, vals & 0xFF
     if bpp > 2 and fmt != ddm.PIX_U8:
         raw[..., 2:] = 77
-    spec = _run_tile_fft(gpu, raw.reshape(n, M, -1), bpp, fmt, M, s, x_off=4)
+    spec = _run_tile_fft(gpu, raw.reshape(n, M, -1), bpp, fmt, M, s, x_off=4, width=M + 4)
     spec = spec.reshape(n, 4, s, s // 2 + 1)
     for f in range(n):
         for tile in range(4):
