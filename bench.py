@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 tile checks")
     ap.add_argument("--profile-steps", action="store_true",
                     help="only run warmup + timed steps (for rocprofv3 runs)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="with --profile-steps: finish with a device copy of this many MiB")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
@@ -118,6 +120,14 @@ def main():
     fps = args.steps * N / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     if args.profile_steps:
+        if args.calib_copy_mib:
+            # a plain device copy of known size: its FETCH_SIZE / WRITE_SIZE
+            # calibrate the counters of the engine's kernels
+            a = torch.ones(args.calib_copy_mib << 20, dtype=torch.uint8, device="cuda")
+            b = torch.empty_like(a)
+            torch.cuda.synchronize()
+            b.copy_(a)
+            torch.cuda.synchronize()
         print(json.dumps({"frames_per_s": fps, "ms_per_step": ms_per_step}))
         return
 

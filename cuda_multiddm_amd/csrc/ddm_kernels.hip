@@ -346,34 +346,44 @@ __global__ __launch_bounds__(NT) void tile_fft_cols(RingDst rd, int tiles, const
 }
 
 // ---------------------------------------------------------------------------
-// Lag engine.  One thread per complex spectrum element; the J new frames
-// of the block stay in registers (and in the thread's own LDS column, for
-// partners inside the block), partners before the block come from the ring.
-// For every lag the block's terms are summed in FP32 registers and added to
-// the accumulator once: acc[l][e] += sum_j |F_{t0+j} - F_{t0+j-tau_l}|^2.
+// Lag engine.  A workgroup owns LAG_EPW = 64 consecutive spectrum elements
+// (one per lane) and its four waves split the lags (wave w takes lags
+// w, w+4, ...), so four independent load streams per element are in flight.
+// The J new frames of the block are read from the ring once per workgroup
+// into LDS; every wave keeps them in registers and reads partners inside
+// the block from LDS, partners before the block from the ring.  For every
+// lag the block's terms are summed in FP32 registers and added to the
+// accumulator once: acc[l][e] += sum_j |F_{t0+j} - F_{t0+j-tau_l}|^2.
 // ---------------------------------------------------------------------------
+constexpr int LAG_EPW = 64;            // elements per workgroup
+constexpr int LAG_WAVES = NT / LAG_EPW;  // waves sharing one element group
+
 template <int J>
 __global__ __launch_bounds__(NT) void lag_kernel(const float2 *__restrict__ ring, float *__restrict__ acc,
                                                 long long E, int R, int slot0, long long t0, int n_new,
                                                 const int *__restrict__ lags, int L) {
-    __shared__ float2 hist[J * NT];
+    __shared__ float2 hist[J * LAG_EPW];
     const int tid = threadIdx.x;
-    const long long e = (long long)blockIdx.x * NT + tid;
-    if (e >= E) return;
-    float2 nv[J];
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-        if (j < n_new) {
+    const int lane = tid % LAG_EPW, w = tid / LAG_EPW;
+    const long long e = (long long)blockIdx.x * LAG_EPW + lane;
+    const bool valid = e < E;
+    for (int j = w; j < J; j += LAG_WAVES) {
+        float2 v = make_float2(0.f, 0.f);
+        if (j < n_new && valid) {
             int s = slot0 + j;
             s -= (s >= R) ? R : 0;
-            nv[j] = ring[(long long)s * E + e];
-            hist[j * NT + tid] = nv[j];
-        } else {
-            nv[j] = make_float2(0.f, 0.f);
+            v = ring[(long long)s * E + e];
         }
+        hist[j * LAG_EPW + lane] = v;
     }
-    for (int l = 0; l < L; ++l) {
+    __syncthreads();
+    if (!valid) return;
+    float2 nv[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) nv[j] = hist[j * LAG_EPW + lane];
+    for (int l = w; l < L; l += LAG_WAVES) {
         const int tau = lags[l];
+        const float a0 = acc[(long long)l * E + e];  // in flight with the partner loads
         float sum = 0.f;
         // partners before the block, from the ring
 #pragma unroll
@@ -386,16 +396,16 @@ __global__ __launch_bounds__(NT) void lag_kernel(const float2 *__restrict__ ring
                 sum += dx * dx + dy * dy;
             }
         }
-        // partners inside the block, from this thread's LDS column
+        // partners inside the block, from LDS
 #pragma unroll
         for (int j = 0; j < J; ++j) {
             if (j < n_new && j >= tau) {
-                const float2 p = hist[(j - tau) * NT + tid];
+                const float2 p = hist[(j - tau) * LAG_EPW + lane];
                 const float dx = nv[j].x - p.x, dy = nv[j].y - p.y;
                 sum += dx * dx + dy * dy;
             }
         }
-        acc[(long long)l * E + e] += sum;
+        acc[(long long)l * E + e] = a0 + sum;
     }
 }
 
@@ -516,7 +526,7 @@ hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st) {
 }
 
 hipError_t launch_lag(const LagArgs &a, hipStream_t st) {
-    const unsigned grid = (unsigned)((a.elems_per_frame + NT - 1) / NT);
+    const unsigned grid = (unsigned)((a.elems_per_frame + LAG_EPW - 1) / LAG_EPW);
     if (a.lag_block == 16)
         hipLaunchKernelGGL(lag_kernel<16>, dim3(grid), dim3(NT), 0, st, (const float2 *)a.ring, a.acc,
                            a.elems_per_frame, a.ring_slots, a.slot0, a.t0, a.n_new, a.lags, a.lag_count);
