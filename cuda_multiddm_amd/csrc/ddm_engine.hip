@@ -55,6 +55,7 @@ struct ddm_engine {
     hipStream_t st = nullptr, st_copy = nullptr;
     hipEvent_t ev_up[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
     long long frames = 0, frames_since_reset = 0;
+    long long frames_lagged = 0;  // frames the lag engine has processed (<= frames)
     std::vector<long long> pairs;
     size_t dev_bytes = 0, pinned_bytes = 0;
     std::string err;
@@ -169,11 +170,6 @@ static int dev_alloc(ddm_engine *e, T **ptr, size_t count, const char *what) {
     return DDM_OK;
 }
 
-static int choose_lag_block(const Params &p) {
-    if (p.lag_block) return p.lag_block;
-    return p.chunk <= 16 ? 16 : 32;
-}
-
 extern "C" int ddm_create(const ddm_desc *desc, ddm_engine **out) {
     if (!out) return DDM_EINVAL;
     *out = nullptr;
@@ -213,9 +209,9 @@ extern "C" int ddm_create(const ddm_desc *desc, ddm_engine **out) {
         }                                                                       \
     } while (0)
     HTRY(hipGetDevice(&e->device));
-    e->lag_block = choose_lag_block(p);
+    e->lag_block = p.lag_block;
     TRY(ddm::build_ring_tables(p, &e->rt));
-    const long long E = p.elems_per_frame;
+    const long long E = p.elems_padded;
     const size_t L = p.lags.size();
     HTRY(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
     HTRY(hipStreamCreateWithFlags(&e->st_copy, hipStreamNonBlocking));
@@ -257,6 +253,8 @@ extern "C" int ddm_create(const ddm_desc *desc, ddm_engine **out) {
     if (!e->rt.idx.empty())
         HTRY(hipMemcpy(e->d_idx, e->rt.idx.data(), e->rt.idx.size() * sizeof(int), hipMemcpyHostToDevice));
     HTRY(hipMemsetAsync(e->d_acc, 0, L * (size_t)E * sizeof(float), e->st));
+    // the padding elements of the last ring group are read by the lag engine
+    HTRY(hipMemsetAsync(e->d_ring, 0, (size_t)p.ring_slots * (size_t)E * 2 * sizeof(float), e->st));
     const float2 *tw = nullptr;
     HTRY(ddm::twiddle_table(&tw));
     HTRY(hipStreamSynchronize(e->st));
@@ -273,14 +271,45 @@ extern "C" void ddm_destroy(ddm_engine *e) {
     delete e;
 }
 
-// Tile + FFT + lag engine for n <= chunk frames at device address src.
+// Lag engine over the next n frames waiting in the ring.
+static int lag_block(ddm_engine *e, int n) {
+    const Params &p = e->p;
+    const long long t0 = e->frames_lagged;
+    ddm::LagArgs a;
+    a.ring = e->d_ring;
+    a.acc = e->d_acc;
+    a.elems_per_frame = p.elems_padded;
+    a.ring_slots = p.ring_slots;
+    a.slot0 = (int)(t0 % p.ring_slots);
+    a.n_new = n;
+    a.dmax = (int)std::min<long long>(p.max_lag, t0);
+    a.lags = e->d_lags;
+    a.lag_count = (int)p.lags.size();
+    a.lag_block = e->lag_block;
+    Mark lag;
+    mark_begin(e, &lag, MK_LAG);
+    HIP_TRY(e, ddm::launch_lag(a, e->st));
+    mark_end(e, &lag);
+    e->stats.lag_launches += 1;
+    e->frames_lagged += n;
+    return DDM_OK;
+}
+
+// Runs the lag engine on frames still waiting for a full block.
+static int flush_lags(ddm_engine *e) {
+    const long long pending = e->frames - e->frames_lagged;
+    return pending > 0 ? lag_block(e, (int)pending) : DDM_OK;
+}
+
+// Tile + FFT for n <= chunk frames at device address src, then the lag
+// engine over every full block of lag_block frames now waiting.
 static int process_chunk(ddm_engine *e, const uint8_t *src, int n, long long frame_bytes, int width,
                          int x_off, int y_off, hipEvent_t consumed) {
     const Params &p = e->p;
     const long long t0 = e->frames;
     const int R = p.ring_slots;
     const int first_slot = (int)(t0 % R);
-    Mark chunk, fft, lag;
+    Mark chunk, fft;
     mark_begin(e, &chunk, MK_CHUNK);
     mark_begin(e, &fft, MK_FFT);
     for (const ddm::ScaleGeom &g : p.geom) {
@@ -298,31 +327,13 @@ static int process_chunk(ddm_engine *e, const uint8_t *src, int n, long long fra
         a.ring = e->d_ring;
         a.ring_slots = R;
         a.first_slot = first_slot;
-        a.elems_per_frame = p.elems_per_frame;
+        a.elems_per_frame = p.elems_padded;
         a.elem_off = g.elem_off;
         HIP_TRY(e, ddm::launch_tile_fft(a, e->st));
         e->stats.fft_launches += (g.s >= 256) ? 2 : 1;
     }
     mark_end(e, &fft);
     if (consumed) HIP_TRY(e, hipEventRecord(consumed, e->st));
-    mark_begin(e, &lag, MK_LAG);
-    for (int b = 0; b < n; b += e->lag_block) {
-        ddm::LagArgs a;
-        a.ring = e->d_ring;
-        a.acc = e->d_acc;
-        a.elems_per_frame = p.elems_per_frame;
-        a.ring_slots = R;
-        a.t0 = t0 + b;
-        a.slot0 = (int)(a.t0 % R);
-        a.n_new = std::min(e->lag_block, n - b);
-        a.lags = e->d_lags;
-        a.lag_count = (int)p.lags.size();
-        a.lag_block = e->lag_block;
-        HIP_TRY(e, ddm::launch_lag(a, e->st));
-        e->stats.lag_launches += 1;
-    }
-    mark_end(e, &lag);
-    mark_end(e, &chunk);
     // pairs (t - tau, t) formed for t in [t0, t0 + n)
     for (size_t l = 0; l < p.lags.size(); ++l) {
         const long long lo = std::max<long long>(t0, p.lags[l]);
@@ -331,6 +342,11 @@ static int process_chunk(ddm_engine *e, const uint8_t *src, int n, long long fra
     }
     e->frames += n;
     e->frames_since_reset += n;
+    while (e->frames - e->frames_lagged >= e->lag_block) {
+        int rc = lag_block(e, e->lag_block);
+        if (rc) return rc;
+    }
+    mark_end(e, &chunk);
     return DDM_OK;
 }
 
@@ -391,6 +407,8 @@ extern "C" int ddm_results(ddm_engine *e, double *results) {
     if (!e || !results) return DDM_EINVAL;
     const Params &p = e->p;
     const size_t L = p.lags.size();
+    int rc = flush_lags(e);
+    if (rc) return rc;
     std::vector<double> norm(L, 0.0);
     for (size_t l = 0; l < L; ++l) {
         if (p.norm_mode == DDM_NORM_PAIRS)
@@ -401,7 +419,7 @@ extern "C" int ddm_results(ddm_engine *e, double *results) {
     HIP_TRY(e, hipMemcpyAsync(e->d_norm, norm.data(), L * sizeof(double), hipMemcpyHostToDevice, e->st));
     ddm::AzArgs a;
     a.acc = e->d_acc;
-    a.elems_per_frame = p.elems_per_frame;
+    a.elems_per_frame = p.elems_padded;
     a.scales = e->d_az;
     a.scale_count = (int)p.geom.size();
     a.q_off = e->d_qoff;
@@ -425,7 +443,9 @@ extern "C" int ddm_results(ddm_engine *e, double *results) {
 extern "C" int ddm_reset(ddm_engine *e) {
     if (!e) return DDM_EINVAL;
     const Params &p = e->p;
-    HIP_TRY(e, hipMemsetAsync(e->d_acc, 0, p.lags.size() * (size_t)p.elems_per_frame * sizeof(float), e->st));
+    int rc = flush_lags(e);  // waiting frames belong to the window being closed
+    if (rc) return rc;
+    HIP_TRY(e, hipMemsetAsync(e->d_acc, 0, p.lags.size() * (size_t)p.elems_padded * sizeof(float), e->st));
     e->frames_since_reset = 0;
     std::fill(e->pairs.begin(), e->pairs.end(), 0);
     return DDM_OK;
@@ -440,7 +460,9 @@ extern "C" int ddm_get_info(const ddm_engine *e, ddm_info *info) {
     info->lag_count = (int)p.lags.size();
     info->lambda_count = (int)p.lambdas.size();
     info->ring_slots = p.ring_slots;
+    info->lag_block = p.lag_block;
     info->elems_per_frame = p.elems_per_frame;
+    info->elems_per_frame_padded = p.elems_padded;
     info->result_count = p.result_count;
     info->frames_submitted = e->frames;
     info->frames_since_reset = e->frames_since_reset;
@@ -462,12 +484,14 @@ extern "C" int ddm_get_accumulators(ddm_engine *e, int scale_idx, float *out) {
     if (!e || !out) return DDM_EINVAL;
     const Params &p = e->p;
     if (scale_idx < 0 || scale_idx >= (int)p.geom.size()) return fail(e, DDM_ERANGE, "scale index out of range");
-    int rc = ddm_sync(e);
+    int rc = flush_lags(e);
+    if (rc) return rc;
+    rc = ddm_sync(e);
     if (rc) return rc;
     const ddm::ScaleGeom &g = p.geom[scale_idx];
     const size_t n = (size_t)g.tiles * g.tile_elems;
     for (size_t l = 0; l < p.lags.size(); ++l)
-        HIP_TRY(e, hipMemcpy(out + l * n, e->d_acc + l * (size_t)p.elems_per_frame + g.elem_off, n * sizeof(float),
+        HIP_TRY(e, hipMemcpy(out + l * n, e->d_acc + l * (size_t)p.elems_padded + g.elem_off, n * sizeof(float),
                              hipMemcpyDeviceToHost));
     // the reference accumulates (|dF| / s^2)^2; 1/s^4 is a power of two, so the
     // scaling is exact
@@ -487,8 +511,13 @@ extern "C" int ddm_get_spectra(ddm_engine *e, long long frame, int scale_idx, fl
     const ddm::ScaleGeom &g = p.geom[scale_idx];
     const size_t slot = (size_t)(frame % p.ring_slots);
     const size_t n = (size_t)g.tiles * g.tile_elems;
-    HIP_TRY(e, hipMemcpy(out, e->d_ring + (slot * (size_t)p.elems_per_frame + g.elem_off) * 2, n * 2 * sizeof(float),
-                         hipMemcpyDeviceToHost));
+    // gather the slot's rows of every ring group the scale touches
+    const size_t G = ddm::RING_GROUP, g0 = (size_t)g.elem_off / G, g1 = ((size_t)g.elem_off + n + G - 1) / G;
+    std::vector<float> rows((g1 - g0) * G * 2);
+    HIP_TRY(e, hipMemcpy2D(rows.data(), G * 2 * sizeof(float),
+                           e->d_ring + ((g0 * p.ring_slots + slot) * G) * 2, (size_t)p.ring_slots * G * 2 * sizeof(float),
+                           G * 2 * sizeof(float), g1 - g0, hipMemcpyDeviceToHost));
+    std::memcpy(out, rows.data() + ((size_t)g.elem_off - g0 * G) * 2, n * 2 * sizeof(float));
     return DDM_OK;
 }
 
@@ -536,6 +565,7 @@ extern "C" int ddm_k_tile_fft(const void *frames, int n_frames, long long frame_
     if (x_off < 0 || y_off < 0 || (long long)x_off + largest_scale > width || (long long)y_off + largest_scale > height)
         return DDM_EROI;
     if (frame_bytes < (long long)width * height * bytes_per_pixel) return DDM_EINVAL;
+    if (elems_per_frame % ddm::RING_GROUP != 0) return DDM_EINVAL;
     const long long tiles = (long long)(largest_scale / scale) * (largest_scale / scale);
     if (elem_off < 0 || elem_off + tiles * scale * (scale / 2 + 1) > elems_per_frame) return DDM_ERANGE;
     ddm::TileFftArgs a;
@@ -562,7 +592,8 @@ extern "C" int ddm_k_lag_accumulate(const float *ring, int ring_slots, long long
                                     float *acc, int lag_block, void *stream) {
     if (!ring || !acc || !lags || ring_slots <= 0 || elems_per_frame <= 0 || first_frame < 0 || lag_count <= 0)
         return DDM_EINVAL;
-    if (lag_block != 16 && lag_block != 32) return DDM_EINVAL;
+    if (elems_per_frame % ddm::RING_GROUP != 0) return DDM_EINVAL;
+    if (lag_block != 32 && lag_block != 64 && lag_block != 128) return DDM_EINVAL;
     if (n_new <= 0 || n_new > lag_block) return DDM_EINVAL;
     if (max_lag < 0 || (long long)max_lag + n_new > ring_slots) return DDM_ERANGE;
     ddm::LagArgs a;
@@ -570,9 +601,9 @@ extern "C" int ddm_k_lag_accumulate(const float *ring, int ring_slots, long long
     a.acc = acc;
     a.elems_per_frame = elems_per_frame;
     a.ring_slots = ring_slots;
-    a.t0 = first_frame;
     a.slot0 = (int)(first_frame % ring_slots);
     a.n_new = n_new;
+    a.dmax = (int)std::min<long long>(max_lag, first_frame);
     a.lags = lags;
     a.lag_count = lag_count;
     a.lag_block = lag_block;

@@ -34,8 +34,8 @@ int check_desc(const ddm_desc *d, Params *out, std::string *msg) {
     if (d->chunk_frames <= 0) return fail(DDM_EINVAL, "chunk frame count must be positive");
     if (d->norm_mode != DDM_NORM_FRAMES && d->norm_mode != DDM_NORM_PAIRS)
         return fail(DDM_EINVAL, "unknown normalisation mode");
-    if (d->lag_block != 0 && d->lag_block != 16 && d->lag_block != 32)
-        return fail(DDM_EINVAL, "lag block must be 0 (automatic), 16 or 32");
+    if (d->lag_block != 0 && d->lag_block != 32 && d->lag_block != 64 && d->lag_block != 128)
+        return fail(DDM_EINVAL, "lag block must be 0 (automatic), 32, 64 or 128");
 
     p.width = d->width;
     p.height = d->height;
@@ -49,7 +49,7 @@ int check_desc(const ddm_desc *d, Params *out, std::string *msg) {
     p.tol = d->mask_tolerance;
     p.chunk = d->chunk_frames;
     p.norm_mode = d->norm_mode;
-    p.lag_block = d->lag_block;
+    p.lag_block = d->lag_block ? d->lag_block : kDefaultLagBlock;
 
     // src/DDM.cu:319-321
     std::sort(p.lags.begin(), p.lags.end());
@@ -89,10 +89,11 @@ int check_desc(const ddm_desc *d, Params *out, std::string *msg) {
         return fail(DDM_EROI, "the specified out dimensions must be smaller than actual image size");
 
     p.max_lag = p.lags.back();
-    // The ring keeps every frame a lag can reach back to: the frames of the
-    // chunk being analysed plus max_lag frames before it (the reference keeps
-    // 3 chunks and so requires max_lag <= chunk, src/DDM.cu:462).
-    long long slots = (long long)p.chunk + p.max_lag;
+    // The ring keeps every frame a lag can reach back to: the frames waiting
+    // for the lag engine (less than one lag block plus one chunk) and max_lag
+    // frames before them (the reference keeps 3 chunks and so requires
+    // max_lag <= chunk, src/DDM.cu:462).
+    long long slots = (long long)p.chunk + p.lag_block + p.max_lag;
     if (slots > (1LL << 30)) return fail(DDM_EINVAL, "chunk + largest tau is too large");
     p.ring_slots = (int)slots;
 
@@ -109,6 +110,7 @@ int check_desc(const ddm_desc *d, Params *out, std::string *msg) {
         p.geom.push_back(g);
     }
     p.elems_per_frame = off;
+    p.elems_padded = pad_elems(off);
     p.result_count = results;
     if (out) *out = std::move(p);
     if (msg) msg->clear();
@@ -331,8 +333,9 @@ extern "C" int ddm_validate(const ddm_desc *desc, ddm_info *info, char *msg, siz
         info->ring_slots = p.ring_slots;
         info->elems_per_frame = p.elems_per_frame;
         info->result_count = p.result_count;
-        info->device_bytes = (size_t)p.ring_slots * p.elems_per_frame * 8 +
-                             (size_t)p.lags.size() * p.elems_per_frame * 4;
+        info->elems_per_frame_padded = p.elems_padded;
+        info->lag_block = p.lag_block;
+        info->device_bytes = (size_t)p.ring_slots * p.elems_padded * 8 + (size_t)p.lags.size() * p.elems_padded * 4;
     }
     return rc;
 }

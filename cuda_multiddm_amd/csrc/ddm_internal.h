@@ -14,6 +14,16 @@ namespace ddm {
 
 // Largest tile size the FFT kernels are instantiated for.
 constexpr int kMaxScale = 2048;
+
+// Spectrum ring layout: elements in groups of RING_GROUP, each group holding
+// all slots contiguously (see ring_index in ddm_kernels.hip).  Frames are
+// padded to a whole number of groups.
+constexpr int RING_GROUP_LOG2 = 6;
+constexpr int RING_GROUP = 1 << RING_GROUP_LOG2;
+inline long long pad_elems(long long e) { return (e + RING_GROUP - 1) / RING_GROUP * RING_GROUP; }
+
+// Frames per lag-engine pass when the descriptor leaves it to the engine.
+constexpr int kDefaultLagBlock = 64;
 // Lag values above this are rejected (ring memory would be absurd).
 constexpr int kMaxLag = 1 << 20;
 
@@ -42,6 +52,7 @@ struct Params {
     int max_lag = 0;
     int ring_slots = 0;
     long long elems_per_frame = 0;
+    long long elems_padded = 0;  // ring / accumulator row stride
     long long result_count = 0;
     std::vector<ScaleGeom> geom;
     std::vector<std::string> warnings;

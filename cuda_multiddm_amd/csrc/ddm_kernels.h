@@ -23,14 +23,14 @@ struct TileFftArgs {
 struct LagArgs {
     const float *ring;
     float *acc;                // [lag][elems_per_frame]
-    long long elems_per_frame;
+    long long elems_per_frame; // padded to a multiple of RING_GROUP
     int ring_slots;
-    int slot0;                 // slot of frame t0
-    long long t0;              // global index of the first new frame
-    int n_new;                 // <= lag_block
+    int slot0;                 // slot of the block's first frame t0
+    int n_new;                 // frames in the block, <= lag_block
+    int dmax;                  // history frames paired with: min(max lag, t0)
     const int *lags;           // device, ascending
     int lag_count;
-    int lag_block;             // 16 or 32
+    int lag_block;             // 32, 64 or 128
 };
 
 struct AzScale {

@@ -138,18 +138,36 @@ __device__ __forceinline__ float load_px(const uint8_t *fr, const PixSrc &ps, in
     return (float)(ps.fmt == DDM_PIX_U16LE ? (b0 | (b1 << 8)) : ((b0 << 8) | b1));
 }
 
+// Ring layout: the spectrum elements of a frame are cut into groups of
+// RING_GROUP consecutive elements, and each group keeps all ring slots
+// together: element e of slot s lives at ((e / RING_GROUP) * R + s) *
+// RING_GROUP + e % RING_GROUP.  A lag-engine workgroup (one group) then
+// walks the frame history as one contiguous stream.
+__device__ __forceinline__ long long ring_index(long long e, int slot, int R) {
+    return ((e >> RING_GROUP_LOG2) * R + slot) * RING_GROUP + (e & (RING_GROUP - 1));
+}
+
 struct RingDst {
     float2 *ring;
-    long long elems_per_frame;
     long long elem_off;  // of this scale
     int ring_slots;
     int first_slot;      // slot of frame 0 of this launch
 };
 
-__device__ __forceinline__ float2 *tile_out(const RingDst &rd, int f, int tile, long long tile_elems) {
+// Writes element `local` of tile `tile` of batch frame f.
+struct TileOut {
+    float2 *ring;
+    long long base;  // element index of the tile's first element
+    int slot, R;
+    __device__ __forceinline__ float2 &operator[](long long local) const {
+        return ring[ring_index(base + local, slot, R)];
+    }
+};
+
+__device__ __forceinline__ TileOut tile_out(const RingDst &rd, int f, int tile, long long tile_elems) {
     int slot = rd.first_slot + f;
     slot -= (slot >= rd.ring_slots) ? rd.ring_slots : 0;  // f < ring_slots
-    return rd.ring + (long long)slot * rd.elems_per_frame + rd.elem_off + (long long)tile * tile_elems;
+    return TileOut{rd.ring, rd.elem_off + (long long)tile * tile_elems, slot, rd.ring_slots};
 }
 
 // ---------------------------------------------------------------------------
@@ -292,7 +310,7 @@ __global__ __launch_bounds__(NT) void tile_fft_rows(PixSrc ps, RingDst rd, int t
     __syncthreads();
     using Rows = Layout<S, P, P, S, 0, 1, false>;
     fft_passes<Rows, 1>(buf, tw);
-    float2 *out = tile_out(rd, f, tile, (long long)S * (H + 1));
+    const TileOut out = tile_out(rd, f, tile, (long long)S * (H + 1));
     for (int i = tid; i < P * H; i += NT) {
         const int r = i / H, k = i % H;
         const float2 za = buf[r * S + k];
@@ -323,7 +341,7 @@ __global__ __launch_bounds__(NT) void tile_fft_cols(RingDst rd, int tiles, const
     const int tile = (blockIdx.x / NB) % tiles;
     const int f = blockIdx.x / (NB * tiles);
     for (int i = tid; i < S; i += NT) tw[i] = twg[S + i];
-    float2 *t = tile_out(rd, f, tile, (long long)S * (H + 1));
+    const TileOut t = tile_out(rd, f, tile, (long long)S * (H + 1));
     const int c0 = cb * CB;
     for (int i = tid; i < S * CB; i += NT) {
         const int ly = i / CB, c = i % CB;
@@ -346,66 +364,95 @@ __global__ __launch_bounds__(NT) void tile_fft_cols(RingDst rd, int tiles, const
 }
 
 // ---------------------------------------------------------------------------
-// Lag engine.  A workgroup owns LAG_EPW = 64 consecutive spectrum elements
-// (one per lane) and its four waves split the lags (wave w takes lags
-// w, w+4, ...), so four independent load streams per element are in flight.
-// The J new frames of the block are read from the ring once per workgroup
-// into LDS; every wave keeps them in registers and reads partners inside
-// the block from LDS, partners before the block from the ring.  For every
-// lag the block's terms are summed in FP32 registers and added to the
-// accumulator once: acc[l][e] += sum_j |F_{t0+j} - F_{t0+j-tau_l}|^2.
+// Lag engine.  A workgroup owns one ring group (RING_GROUP = 64 consecutive
+// spectrum elements, one per lane) and a block of n <= J new frames
+// t0 .. t0+n-1 (slots slot0 ..), which it stages in LDS.  It then walks
+// every frame the block pairs with exactly once -- the history
+// t0-dmax .. t0-1 from the ring, the block itself from LDS -- with the four
+// waves taking every fourth frame.  The frame at distance d = t0 - p pairs
+// with new frame j = tau_l - d for every lag with 0 <= j < n.  Each wave
+// keeps partial sums for all lags in registers; they are combined in a fixed
+// order through LDS and added to the accumulator once per block:
+//   acc[l][e] += sum_j |F_{t0+j}[e] - F_{t0+j-tau_l}[e]|^2.
 // ---------------------------------------------------------------------------
-constexpr int LAG_EPW = 64;            // elements per workgroup
-constexpr int LAG_WAVES = NT / LAG_EPW;  // waves sharing one element group
+constexpr int LAG_WAVES = NT / RING_GROUP;  // waves sharing one ring group
+constexpr int LAG_PREFETCH = 4;             // history frames in flight per wave
 
-template <int J>
-__global__ __launch_bounds__(NT) void lag_kernel(const float2 *__restrict__ ring, float *__restrict__ acc,
-                                                long long E, int R, int slot0, long long t0, int n_new,
-                                                const int *__restrict__ lags, int L) {
-    __shared__ float2 hist[J * LAG_EPW];
-    const int tid = threadIdx.x;
-    const int lane = tid % LAG_EPW, w = tid / LAG_EPW;
-    const long long e = (long long)blockIdx.x * LAG_EPW + lane;
-    const bool valid = e < E;
-    for (int j = w; j < J; j += LAG_WAVES) {
-        float2 v = make_float2(0.f, 0.f);
-        if (j < n_new && valid) {
-            int s = slot0 + j;
-            s -= (s >= R) ? R : 0;
-            v = ring[(long long)s * E + e];
+template <int LMAX>
+__device__ __forceinline__ void lag_pairs(float (&a)[LMAX], const int (&tau)[LMAX], const float2 *nb, int lane,
+                                          int n, int d, float2 F) {
+#pragma unroll
+    for (int l = 0; l < LMAX; ++l) {
+        const int j = tau[l] - d;
+        if ((unsigned)j < (unsigned)n) {
+            const float2 v = nb[j * RING_GROUP + lane];
+            const float dx = v.x - F.x, dy = v.y - F.y;
+            a[l] += dx * dx + dy * dy;
         }
-        hist[j * LAG_EPW + lane] = v;
+    }
+}
+
+template <int J, int LMAX>
+__global__ __launch_bounds__(NT) void lag_kernel(const float2 *__restrict__ ring, float *__restrict__ acc,
+                                                long long E, int R, int slot0, int n, int dmax,
+                                                const int *__restrict__ lags, int L) {
+    __shared__ float2 nb[J * RING_GROUP];
+    const int tid = threadIdx.x;
+    const int lane = tid % RING_GROUP;
+    const int w = __builtin_amdgcn_readfirstlane(tid / RING_GROUP);
+    const float2 *g = ring + (long long)blockIdx.x * R * RING_GROUP + lane;  // slot s at g[s * RING_GROUP]
+    for (int j = w; j < n; j += LAG_WAVES) {
+        int s = slot0 + j;
+        s -= (s >= R) ? R : 0;
+        nb[j * RING_GROUP + lane] = g[(long long)s * RING_GROUP];
+    }
+    int tau[LMAX];
+    float a[LMAX];
+#pragma unroll
+    for (int l = 0; l < LMAX; ++l) {
+        tau[l] = (l < L) ? lags[l] : -(1 << 30);  // an absent lag never matches
+        a[l] = 0.f;
     }
     __syncthreads();
-    if (!valid) return;
-    float2 nv[J];
+    // history frames, d = dmax - w, dmax - w - 4, ... > 0
+    for (int d0 = dmax - w; d0 > 0; d0 -= LAG_WAVES * LAG_PREFETCH) {
+        float2 F[LAG_PREFETCH];
 #pragma unroll
-    for (int j = 0; j < J; ++j) nv[j] = hist[j * LAG_EPW + lane];
-    for (int l = w; l < L; l += LAG_WAVES) {
-        const int tau = lags[l];
-        const float a0 = acc[(long long)l * E + e];  // in flight with the partner loads
-        float sum = 0.f;
-        // partners before the block, from the ring
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-            if (j < n_new && j < tau && t0 + j - tau >= 0) {
-                int s = slot0 + j - tau;
+        for (int k = 0; k < LAG_PREFETCH; ++k) {
+            const int d = d0 - k * LAG_WAVES;
+            if (d > 0) {
+                int s = slot0 - d;
                 s += (s < 0) ? R : 0;
-                const float2 p = ring[(long long)s * E + e];
-                const float dx = nv[j].x - p.x, dy = nv[j].y - p.y;
-                sum += dx * dx + dy * dy;
+                F[k] = g[(long long)s * RING_GROUP];
             }
         }
-        // partners inside the block, from LDS
 #pragma unroll
-        for (int j = 0; j < J; ++j) {
-            if (j < n_new && j >= tau) {
-                const float2 p = hist[(j - tau) * LAG_EPW + lane];
-                const float dx = nv[j].x - p.x, dy = nv[j].y - p.y;
-                sum += dx * dx + dy * dy;
-            }
+        for (int k = 0; k < LAG_PREFETCH; ++k) {
+            const int d = d0 - k * LAG_WAVES;
+            if (d > 0) lag_pairs<LMAX>(a, tau, nb, lane, n, d, F[k]);
         }
-        acc[(long long)l * E + e] = a0 + sum;
+    }
+    // frames of the block, i = w, w + 4, ... < n (distance d = -i)
+    for (int i = w; i < n; i += LAG_WAVES) lag_pairs<LMAX>(a, tau, nb, lane, n, -i, nb[i * RING_GROUP + lane]);
+    __syncthreads();
+    // combine the waves' partial sums in a fixed order, LR lags per round
+    constexpr int LR = (J * 2) / LAG_WAVES;
+    float *red = reinterpret_cast<float *>(nb);  // [LAG_WAVES][LR][RING_GROUP]
+    float *accg = acc + (long long)blockIdx.x * RING_GROUP + lane;
+    for (int l0 = 0; l0 < L; l0 += LR) {
+#pragma unroll
+        for (int l = 0; l < LMAX; ++l)
+            if (l >= l0 && l < l0 + LR) red[(w * LR + (l - l0)) * RING_GROUP + lane] = a[l];
+        __syncthreads();
+        const int l1 = (L < l0 + LR) ? L : l0 + LR;
+        for (int l = l0 + w; l < l1; l += LAG_WAVES) {
+            const int r = (l - l0) * RING_GROUP + lane;
+            float s = red[r];
+#pragma unroll
+            for (int v = 1; v < LAG_WAVES; ++v) s += red[v * LR * RING_GROUP + r];
+            accg[(long long)l * E] += s;
+        }
+        __syncthreads();
     }
 }
 
@@ -506,7 +553,7 @@ hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st) {
     if (e != hipSuccess) return e;
     PixSrc ps{(const uint8_t *)a.frames, a.frame_bytes, (long long)a.width * a.bpp, a.bpp, a.fmt, a.x_off,
               a.y_off};
-    RingDst rd{(float2 *)a.ring, a.elems_per_frame, a.elem_off, a.ring_slots, a.first_slot};
+    RingDst rd{(float2 *)a.ring, a.elem_off, a.ring_slots, a.first_slot};
     const int tps = a.largest_scale / a.scale, tiles = tps * tps;
     switch (a.scale) {
     case 2: launch_small<2>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
@@ -525,16 +572,27 @@ hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st) {
     return hipGetLastError();
 }
 
+template <int J, int LMAX>
+static void launch_lag_t(const LagArgs &a, int l0, int L, hipStream_t st) {
+    const unsigned grid = (unsigned)(a.elems_per_frame / RING_GROUP);
+    hipLaunchKernelGGL((lag_kernel<J, LMAX>), dim3(grid), dim3(NT), 0, st, (const float2 *)a.ring,
+                       a.acc + (long long)l0 * a.elems_per_frame, a.elems_per_frame, a.ring_slots, a.slot0, a.n_new,
+                       a.dmax, a.lags + l0, L);
+}
+
 hipError_t launch_lag(const LagArgs &a, hipStream_t st) {
-    const unsigned grid = (unsigned)((a.elems_per_frame + LAG_EPW - 1) / LAG_EPW);
-    if (a.lag_block == 16)
-        hipLaunchKernelGGL(lag_kernel<16>, dim3(grid), dim3(NT), 0, st, (const float2 *)a.ring, a.acc,
-                           a.elems_per_frame, a.ring_slots, a.slot0, a.t0, a.n_new, a.lags, a.lag_count);
-    else if (a.lag_block == 32)
-        hipLaunchKernelGGL(lag_kernel<32>, dim3(grid), dim3(NT), 0, st, (const float2 *)a.ring, a.acc,
-                           a.elems_per_frame, a.ring_slots, a.slot0, a.t0, a.n_new, a.lags, a.lag_count);
-    else
-        return hipErrorInvalidValue;
+    if (a.elems_per_frame % RING_GROUP != 0 || a.n_new < 1 || a.n_new > a.lag_block) return hipErrorInvalidValue;
+    // at most 64 lags per pass (registers); more lags take more passes
+    for (int l0 = 0; l0 < a.lag_count; l0 += 64) {
+        const int L = (a.lag_count - l0) < 64 ? (a.lag_count - l0) : 64;
+        const bool small = L <= 32;
+        switch (a.lag_block) {
+        case 32: small ? launch_lag_t<32, 32>(a, l0, L, st) : launch_lag_t<32, 64>(a, l0, L, st); break;
+        case 64: small ? launch_lag_t<64, 32>(a, l0, L, st) : launch_lag_t<64, 64>(a, l0, L, st); break;
+        case 128: small ? launch_lag_t<128, 32>(a, l0, L, st) : launch_lag_t<128, 64>(a, l0, L, st); break;
+        default: return hipErrorInvalidValue;
+        }
+    }
     return hipGetLastError();
 }
 

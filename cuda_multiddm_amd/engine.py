@@ -57,8 +57,9 @@ class Desc(ctypes.Structure):
 class Info(ctypes.Structure):
     _fields_ = [
         ("largest_scale", ctypes.c_int), ("scale_count", ctypes.c_int), ("lag_count", ctypes.c_int),
-        ("lambda_count", ctypes.c_int), ("ring_slots", ctypes.c_int),
-        ("elems_per_frame", ctypes.c_longlong), ("result_count", ctypes.c_longlong),
+        ("lambda_count", ctypes.c_int), ("ring_slots", ctypes.c_int), ("lag_block", ctypes.c_int),
+        ("elems_per_frame", ctypes.c_longlong), ("elems_per_frame_padded", ctypes.c_longlong),
+        ("result_count", ctypes.c_longlong),
         ("frames_submitted", ctypes.c_longlong), ("frames_since_reset", ctypes.c_longlong),
         ("device_bytes", ctypes.c_size_t), ("pinned_bytes", ctypes.c_size_t),
     ]

@@ -76,15 +76,17 @@ typedef struct ddm_desc {
     float mask_tolerance; /* ring [q, tol * q] (-t: 1 + percent/100) */
     int chunk_frames;     /* frames per upload / FFT batch (-C) */
     int norm_mode;        /* enum ddm_norm_mode */
-    int lag_block;        /* frames per lag-engine pass: 0 = automatic, else 16 or 32 */
+    int lag_block;        /* frames per lag-engine pass: 0 = automatic (64), else 32, 64 or 128 */
 } ddm_desc;
 
 /* Sizes the engine derived from a descriptor (after sorting). */
 typedef struct ddm_info {
     int largest_scale;         /* M */
     int scale_count, lag_count, lambda_count;
-    int ring_slots;            /* spectra kept resident: chunk_frames + max lag */
+    int ring_slots;            /* spectra kept resident: chunk_frames + lag_block + max lag */
+    int lag_block;             /* frames per lag-engine pass */
     long long elems_per_frame; /* complex spectrum elements per frame, all scales */
+    long long elems_per_frame_padded; /* rounded up to a multiple of 64: ring and accumulator row stride */
     long long result_count;    /* sum over scales of tiles * lambda_count * lag_count */
     long long frames_submitted;   /* since creation */
     long long frames_since_reset; /* since creation or the last ddm_reset */
@@ -167,8 +169,12 @@ const char *ddm_status_string(int code);
 /* -------------------------------------------------------------------------
  * Stage entry points on caller-owned device buffers (the kernels the
  * engine launches).  `stream` is a hipStream_t (NULL = default stream).
- * Ring layout: ring[slot][elem] complex float (interleaved), slot = frame
- * index mod ring_slots; elem runs over all scales' tiles of a frame.
+ * Ring layout: complex floats (interleaved re, im).  elem runs over all
+ * scales' tiles of a frame, elems_per_frame is a multiple of 64 (padded),
+ * and slot = frame index mod ring_slots.  Elements are grouped by 64 and a
+ * group keeps all its slots together:
+ *   ring[((elem / 64) * ring_slots + slot) * 64 + elem % 64].
+ * Accumulators: acc[lag][elem], row stride elems_per_frame.
  * ------------------------------------------------------------------------- */
 
 /* Tile cut + unnormalised 2-D R2C FFT of n_frames frames for one scale s
@@ -185,7 +191,9 @@ int ddm_k_tile_fft(const void *frames, int n_frames, long long frame_bytes,
  * src/DDM.cu:222-262): for the n_new frames first_frame .. first_frame+n_new-1
  * held in the ring and every lag l, acc[l][e] += sum |F_t[e] - F_{t-lag}[e]|^2
  * over those t with t - lag >= 0.  No 1/s^2 factor is applied.  lags is a
- * device array of lag_count ints.  n_new <= lag_block (16 or 32). */
+ * device array of lag_count ints, all <= max_lag; the ring must hold frames
+ * first_frame - max_lag .. first_frame + n_new - 1 (max_lag + n_new <=
+ * ring_slots).  n_new <= lag_block (32, 64 or 128). */
 int ddm_k_lag_accumulate(const float *ring, int ring_slots, long long elems_per_frame,
                          long long first_frame, int n_new, const int *lags, int lag_count,
                          int max_lag, float *acc, int lag_block, void *stream);

@@ -83,8 +83,23 @@ def test_host_and_device_submission_identical(gpu):
     assert np.array_equal(a, b)
     c, _ = _both(frames, [64, 16], [1, 2, 7, 31], [2.0, 5.0], 1.2, 30, device_input=True)
     assert np.array_equal(b, c)  # deterministic
-    d, _ = _both(frames, [64, 16], [1, 2, 7, 31], [2.0, 5.0], 1.2, 7, lag_block=16)
-    assert o.max_rel_err(d, ref) <= TOL  # other chunking: same answer within tolerance
+    for chunk, lb in [(7, 32), (30, 128), (45, 64)]:  # other chunk / lag-block splits
+        d, _ = _both(frames, [64, 16], [1, 2, 7, 31], [2.0, 5.0], 1.2, chunk, lag_block=lb)
+        assert o.max_rel_err(d, ref) <= TOL
+
+
+def test_results_mid_stream(gpu):
+    """Reading results flushes a partial lag block; the stream then goes on
+    and the final answer still equals a single pass."""
+    frames = ddm.synth_uniform(100, 64, 64, 8, seed=12)
+    lags, lams = [1, 2, 5, 17, 60, 90], [2.0, 4.0, 9.0]
+    with ddm.Engine(64, 64, [64, 32], lags, lams, 1.2, 30, lag_block=64) as e:
+        e.submit_host(frames[:37])
+        mid = e.results()
+        e.submit_host(frames[37:])
+        end = e.results()
+    assert o.max_rel_err(mid, o.run(o.Config(64, 64, 37, [64, 32], lags, lams), frames[:37])) <= TOL
+    assert o.max_rel_err(end, o.run(o.Config(64, 64, 100, [64, 32], lags, lams), frames)) <= TOL
 
 
 def test_spectra_and_accumulators(gpu):

@@ -12,6 +12,26 @@ pytestmark = pytest.mark.gpu
 FFT_TOL = 4e-6
 
 
+G = 64  # ring group: ring[((e // G) * R + slot) * G + e % G]
+
+
+def pad64(e):
+    return (e + G - 1) // G * G
+
+
+def ring_to_frames(ring_flat, R, E_pad):
+    """Blocked ring layout -> [slot][elem] complex."""
+    r = np.asarray(ring_flat).reshape(E_pad // G, R, G, 2).transpose(1, 0, 2, 3).reshape(R, E_pad, 2)
+    return r[..., 0].astype(np.float64) + 1j * r[..., 1]
+
+
+def frames_to_ring(re, im):
+    """[slot][elem] integer parts (elem count a multiple of 64) -> blocked flat float32 ring."""
+    R, E = re.shape
+    a = np.stack([re, im], -1).astype(np.float32).reshape(R, E // G, G, 2).transpose(1, 0, 2, 3)
+    return np.ascontiguousarray(a).reshape(-1)
+
+
 def _run_tile_fft(gpu, frames, bpp, fmt, M, s, x_off=0, y_off=0, ring_slots=None, first_frame=0, width=None):
     import torch
     n, h = frames.shape[0], frames.shape[1]
@@ -20,14 +40,14 @@ def _run_tile_fft(gpu, frames, bpp, fmt, M, s, x_off=0, y_off=0, ring_slots=None
     frame_bytes = raw.shape[1]
     T = (M // s) ** 2
     E = T * s * (s // 2 + 1)
+    Ep = pad64(E)
     R = ring_slots or n
     d_frames = torch.from_numpy(raw.copy()).to(gpu)
-    ring = torch.zeros(R * E * 2, dtype=torch.float32, device=gpu)
+    ring = torch.zeros(R * Ep * 2, dtype=torch.float32, device=gpu)
     print(f"tile_fft M={M} s={s} n={n} fmt={fmt} bpp={bpp}", flush=True)
-    ddm.k_tile_fft(d_frames, n, frame_bytes, w, h, bpp, fmt, x_off, y_off, M, s, ring, R, E, 0, first_frame)
+    ddm.k_tile_fft(d_frames, n, frame_bytes, w, h, bpp, fmt, x_off, y_off, M, s, ring, R, Ep, 0, first_frame)
     torch.cuda.synchronize()
-    r = ring.cpu().numpy().reshape(R, T, s, s // 2 + 1, 2)
-    return r[..., 0].astype(np.float64) + 1j * r[..., 1]
+    return ring_to_frames(ring.cpu().numpy(), R, Ep)[:, :E].reshape(R, T, s, s // 2 + 1)
 
 
 def _tile(img, M, s, tile, x_off, y_off):
@@ -116,23 +136,28 @@ def _lag_expected(ring, R, t0, n_new, lags):
     return acc
 
 
-@pytest.mark.parametrize("J", [16, 32])
-@pytest.mark.parametrize("t0,n_new", [(0, None), (3, 7), (100, None), (57, 1)])
-def test_lag_engine_bit_exact(gpu, J, t0, n_new):
+@pytest.mark.parametrize("J", [32, 64, 128])
+@pytest.mark.parametrize("t0,n_new", [(0, None), (3, 7), (100, None), (57, 1), (200, 29)])
+@pytest.mark.parametrize("nlags", [10, 40, 70])
+def test_lag_engine_bit_exact(gpu, J, t0, n_new, nlags):
+    """Integer spectra: every |dF|^2 and every partial sum is an integer below
+    2^24, so FP32 gives the exact answer whatever the summation order.
+    40 lags use the 64-register variant, 70 lags two passes."""
     import torch
     n_new = J if n_new is None else n_new
-    lags = [0, 1, 2, 5, 15, 16, 17, 31, 33, 40]
+    base = [0, 1, 2, 5, 15, 16, 17, 31, 33, 40, 63, 64, 65, 127, 128, 129, 150]
+    lags = sorted(set(base + list(range(3, 3 + 2 * nlags, 2))))[:nlags]
     R = max(lags) + J + 3
-    E = 1000  # not a multiple of the 256-thread workgroup
-    rng = np.random.default_rng(t0 * 7 + J + n_new)
+    E = 15 * G
+    rng = np.random.default_rng(t0 * 7 + J + n_new + nlags)
     re = rng.integers(-8, 9, (R, E))
     im = rng.integers(-8, 9, (R, E))
     ring_np = re + 1j * im
-    ring = torch.from_numpy(np.stack([re, im], -1).astype(np.float32).reshape(-1)).to(gpu)
+    ring = torch.from_numpy(frames_to_ring(re, im)).to(gpu)
     acc0 = rng.integers(0, 1000, (len(lags), E))
     acc = torch.from_numpy(acc0.astype(np.float32).reshape(-1)).to(gpu)
     d_lags = torch.tensor(lags, dtype=torch.int32, device=gpu)
-    print(f"lag J={J} t0={t0} n_new={n_new}", flush=True)
+    print(f"lag J={J} t0={t0} n_new={n_new} L={len(lags)}", flush=True)
     ddm.k_lag_accumulate(ring, R, E, t0, n_new, d_lags, max(lags), acc, J)
     torch.cuda.synchronize()
     got = acc.cpu().numpy().reshape(len(lags), E).astype(np.int64)
@@ -151,7 +176,12 @@ def test_stage_entry_points_reject_bad_arguments(gpu):
         ddm.k_tile_fft(frames, 1, 64, 8, 8, 1, 0, 1, 0, 8, 8, ring, 1, 40, 0, 0)
     with pytest.raises(ddm.DDMError):  # spectra would overrun the ring element range
         ddm.k_tile_fft(frames, 1, 64, 8, 8, 1, 0, 0, 0, 8, 8, ring, 1, 8, 0, 0)
+    big = torch.zeros(64 * 40 * 2, dtype=torch.float32, device=gpu)
     with pytest.raises(ddm.DDMError):  # n_new above the lag block
-        ddm.k_lag_accumulate(ring, 4, 2, 0, 17, lags, 1, ring, 16)
+        ddm.k_lag_accumulate(big, 40, 64, 0, 33, lags, 1, big, 32)
     with pytest.raises(ddm.DDMError):  # ring too short for max lag
-        ddm.k_lag_accumulate(ring, 4, 2, 0, 4, lags, 1, ring, 16)
+        ddm.k_lag_accumulate(big, 40, 64, 0, 32, lags, 9, big, 32)
+    with pytest.raises(ddm.DDMError):  # element count not a whole number of ring groups
+        ddm.k_lag_accumulate(big, 40, 60, 0, 4, lags, 1, big, 32)
+    with pytest.raises(ddm.DDMError):  # unsupported lag block
+        ddm.k_lag_accumulate(big, 40, 64, 0, 4, lags, 1, big, 16)
