@@ -43,17 +43,16 @@ def distinct_frames(t0, n, lags):
     return len(s)
 
 
-def lag_engine_bytes(E, lags, n_frames, chunk, block):
-    """Algorithmic HBM bytes of every lag-engine launch of one analysis:
-    each distinct ring frame a launch touches read once (8 B per complex
+def lag_engine_bytes(E, lags, n_frames, block):
+    """Algorithmic HBM bytes of every lag-engine launch of one analysis (the
+    engine runs one launch per block of `block` frames, the last one partial):
+    each distinct ring frame a launch pairs with read once (8 B per complex
     element) plus the FP32 accumulators of every lag read and written once."""
     total, launches = 0, 0
-    for c0 in range(0, n_frames, chunk):
-        n = min(chunk, n_frames - c0)
-        for b in range(0, n, block):
-            m = min(block, n - b)
-            total += 8 * E * distinct_frames(c0 + b, m, lags) + 8 * E * len(lags)
-            launches += 1
+    for t0 in range(0, n_frames, block):
+        m = min(block, n_frames - t0)
+        total += 8 * E * distinct_frames(t0, m, lags) + 8 * E * len(lags)
+        launches += 1
     return total, launches
 
 
@@ -67,6 +66,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=WORKLOAD["frames"])
+    ap.add_argument("--lag-block", type=int, default=0, help="frames per lag-engine pass (0 = engine default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=0, help="frames for the CPU baseline sample (0 = all)")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 tile checks")
@@ -97,7 +97,7 @@ def main():
     torch.cuda.synchronize()
 
     def make():
-        return ddm.Engine(S, S, W["scales"], lags, lams, W["tol"], W["chunk"])
+        return ddm.Engine(S, S, W["scales"], lags, lams, W["tol"], W["chunk"], lag_block=args.lag_block)
 
     def step(e):
         e.submit_device(d_frames.data_ptr(), N)
@@ -138,9 +138,9 @@ def main():
         st = e.stage_stats()
         chunk_ms = e.chunk_times()
         info = e.info()
-        lag_block = 32 if W["chunk"] > 16 else 16
+        lag_block = info.lag_block
     E = info.elems_per_frame
-    lag_bytes, lag_launches = lag_engine_bytes(E, lags, N, W["chunk"], lag_block)
+    lag_bytes, lag_launches = lag_engine_bytes(E, lags, N, lag_block)
     assert lag_launches == st.lag_launches, (lag_launches, st.lag_launches)
     lag_avg_ms = st.lag_ms / st.lag_launches
     achieved = lag_bytes / (st.lag_ms * 1e-3) / 1e9
@@ -174,7 +174,7 @@ def main():
                               "azimuthal": round(st.azimuthal_ms, 3)},
         "roofline": {
             "bound": "hbm",
-            "kernel": "lag_kernel<32>",
+            "kernel": f"lag_kernel<{lag_block}, {32 if len(lags) <= 32 else 64}>",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

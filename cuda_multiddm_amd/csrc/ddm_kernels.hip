@@ -3,8 +3,8 @@
 //   tile_fft_small<S>  S <= 128: tile cut + 2-D R2C FFT of whole tiles in LDS
 //   tile_fft_rows<S>   S >= 256: tile cut + row R2C FFTs (first half of the 2-D FFT)
 //   tile_fft_cols<S>   S >= 256: column FFTs, in place in the ring slot
-//   lag_kernel<J>      |F_t - F_{t-tau}|^2 accumulation for a block of J frames
 //   azimuthal_kernel   masked q-ring average of the lag sums, FP64
+// (the lag engine is in ddm_lag.hip)
 //
 // Reference counterparts: parseBufferScalePow2 (src/DDM_kernel.cuh:43-78),
 // cufftExecR2C (src/DDM.cu:189-192), processFFT (src/DDM_kernel.cuh:85-101),
@@ -364,99 +364,6 @@ __global__ __launch_bounds__(NT) void tile_fft_cols(RingDst rd, int tiles, const
 }
 
 // ---------------------------------------------------------------------------
-// Lag engine.  A workgroup owns one ring group (RING_GROUP = 64 consecutive
-// spectrum elements, one per lane) and a block of n <= J new frames
-// t0 .. t0+n-1 (slots slot0 ..), which it stages in LDS.  It then walks
-// every frame the block pairs with exactly once -- the history
-// t0-dmax .. t0-1 from the ring, the block itself from LDS -- with the four
-// waves taking every fourth frame.  The frame at distance d = t0 - p pairs
-// with new frame j = tau_l - d for every lag with 0 <= j < n.  Each wave
-// keeps partial sums for all lags in registers; they are combined in a fixed
-// order through LDS and added to the accumulator once per block:
-//   acc[l][e] += sum_j |F_{t0+j}[e] - F_{t0+j-tau_l}[e]|^2.
-// ---------------------------------------------------------------------------
-constexpr int LAG_WAVES = NT / RING_GROUP;  // waves sharing one ring group
-constexpr int LAG_PREFETCH = 4;             // history frames in flight per wave
-
-template <int LMAX>
-__device__ __forceinline__ void lag_pairs(float (&a)[LMAX], const int (&tau)[LMAX], const float2 *nb, int lane,
-                                          int n, int d, float2 F) {
-#pragma unroll
-    for (int l = 0; l < LMAX; ++l) {
-        const int j = tau[l] - d;
-        if ((unsigned)j < (unsigned)n) {
-            const float2 v = nb[j * RING_GROUP + lane];
-            const float dx = v.x - F.x, dy = v.y - F.y;
-            a[l] += dx * dx + dy * dy;
-        }
-    }
-}
-
-template <int J, int LMAX>
-__global__ __launch_bounds__(NT) void lag_kernel(const float2 *__restrict__ ring, float *__restrict__ acc,
-                                                long long E, int R, int slot0, int n, int dmax,
-                                                const int *__restrict__ lags, int L) {
-    __shared__ float2 nb[J * RING_GROUP];
-    const int tid = threadIdx.x;
-    const int lane = tid % RING_GROUP;
-    const int w = __builtin_amdgcn_readfirstlane(tid / RING_GROUP);
-    const float2 *g = ring + (long long)blockIdx.x * R * RING_GROUP + lane;  // slot s at g[s * RING_GROUP]
-    for (int j = w; j < n; j += LAG_WAVES) {
-        int s = slot0 + j;
-        s -= (s >= R) ? R : 0;
-        nb[j * RING_GROUP + lane] = g[(long long)s * RING_GROUP];
-    }
-    int tau[LMAX];
-    float a[LMAX];
-#pragma unroll
-    for (int l = 0; l < LMAX; ++l) {
-        tau[l] = (l < L) ? lags[l] : -(1 << 30);  // an absent lag never matches
-        a[l] = 0.f;
-    }
-    __syncthreads();
-    // history frames, d = dmax - w, dmax - w - 4, ... > 0
-    for (int d0 = dmax - w; d0 > 0; d0 -= LAG_WAVES * LAG_PREFETCH) {
-        float2 F[LAG_PREFETCH];
-#pragma unroll
-        for (int k = 0; k < LAG_PREFETCH; ++k) {
-            const int d = d0 - k * LAG_WAVES;
-            if (d > 0) {
-                int s = slot0 - d;
-                s += (s < 0) ? R : 0;
-                F[k] = g[(long long)s * RING_GROUP];
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < LAG_PREFETCH; ++k) {
-            const int d = d0 - k * LAG_WAVES;
-            if (d > 0) lag_pairs<LMAX>(a, tau, nb, lane, n, d, F[k]);
-        }
-    }
-    // frames of the block, i = w, w + 4, ... < n (distance d = -i)
-    for (int i = w; i < n; i += LAG_WAVES) lag_pairs<LMAX>(a, tau, nb, lane, n, -i, nb[i * RING_GROUP + lane]);
-    __syncthreads();
-    // combine the waves' partial sums in a fixed order, LR lags per round
-    constexpr int LR = (J * 2) / LAG_WAVES;
-    float *red = reinterpret_cast<float *>(nb);  // [LAG_WAVES][LR][RING_GROUP]
-    float *accg = acc + (long long)blockIdx.x * RING_GROUP + lane;
-    for (int l0 = 0; l0 < L; l0 += LR) {
-#pragma unroll
-        for (int l = 0; l < LMAX; ++l)
-            if (l >= l0 && l < l0 + LR) red[(w * LR + (l - l0)) * RING_GROUP + lane] = a[l];
-        __syncthreads();
-        const int l1 = (L < l0 + LR) ? L : l0 + LR;
-        for (int l = l0 + w; l < l1; l += LAG_WAVES) {
-            const int r = (l - l0) * RING_GROUP + lane;
-            float s = red[r];
-#pragma unroll
-            for (int v = 1; v < LAG_WAVES; ++v) s += red[v * LR * RING_GROUP + r];
-            accg[(long long)l * E] += s;
-        }
-        __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Masked azimuthal average.  One workgroup per output value
 // results[scale][tile][q][lag]; the q ring's element list is summed in FP64
 // and normalised: I = 2 * sum * s^-4 / count * norm[lag].
@@ -568,30 +475,6 @@ hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st) {
     case 1024: launch_large<1024>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
     case 2048: launch_large<2048>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
     default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-template <int J, int LMAX>
-static void launch_lag_t(const LagArgs &a, int l0, int L, hipStream_t st) {
-    const unsigned grid = (unsigned)(a.elems_per_frame / RING_GROUP);
-    hipLaunchKernelGGL((lag_kernel<J, LMAX>), dim3(grid), dim3(NT), 0, st, (const float2 *)a.ring,
-                       a.acc + (long long)l0 * a.elems_per_frame, a.elems_per_frame, a.ring_slots, a.slot0, a.n_new,
-                       a.dmax, a.lags + l0, L);
-}
-
-hipError_t launch_lag(const LagArgs &a, hipStream_t st) {
-    if (a.elems_per_frame % RING_GROUP != 0 || a.n_new < 1 || a.n_new > a.lag_block) return hipErrorInvalidValue;
-    // at most 64 lags per pass (registers); more lags take more passes
-    for (int l0 = 0; l0 < a.lag_count; l0 += 64) {
-        const int L = (a.lag_count - l0) < 64 ? (a.lag_count - l0) : 64;
-        const bool small = L <= 32;
-        switch (a.lag_block) {
-        case 32: small ? launch_lag_t<32, 32>(a, l0, L, st) : launch_lag_t<32, 64>(a, l0, L, st); break;
-        case 64: small ? launch_lag_t<64, 32>(a, l0, L, st) : launch_lag_t<64, 64>(a, l0, L, st); break;
-        case 128: small ? launch_lag_t<128, 32>(a, l0, L, st) : launch_lag_t<128, 64>(a, l0, L, st); break;
-        default: return hipErrorInvalidValue;
-        }
     }
     return hipGetLastError();
 }

@@ -103,19 +103,23 @@ def test_results_mid_stream(gpu):
 
 
 def test_spectra_and_accumulators(gpu):
-    frames = ddm.synth_uniform(20, 64, 64, 8, seed=6)
+    frames = ddm.synth_uniform(100, 64, 64, 8, seed=6)
     lags = [1, 3, 8]
-    with ddm.Engine(64, 64, [64, 32], lags, [4.0, 8.0], 1.2, 8) as e:
+    with ddm.Engine(64, 64, [64, 32], lags, [4.0, 8.0], 1.2, 8, lag_block=32) as e:
         e.submit_host(frames)
-        spec = e.spectra(19, 1)  # scale 32, last frame
-        for tile in range(4):
-            ty, tx = divmod(tile, 2)
-            ref = np.fft.rfft2(frames[19, ty * 32:(ty + 1) * 32, tx * 32:(tx + 1) * 32].astype(float))
-            assert np.max(np.abs(spec[tile] - ref)) / np.max(np.abs(ref)) < 4e-6
+        assert e.info().ring_slots == 8 + 32 + 8
+        for fr in (99, 60):
+            spec = e.spectra(fr, 1)  # scale 32
+            for tile in range(4):
+                ty, tx = divmod(tile, 2)
+                ref = np.fft.rfft2(frames[fr, ty * 32:(ty + 1) * 32, tx * 32:(tx + 1) * 32].astype(float))
+                assert np.max(np.abs(spec[tile] - ref)) / np.max(np.abs(ref)) < 4e-6
         with pytest.raises(ddm.DDMError):
-            e.spectra(0, 0)  # no longer held: the ring keeps chunk + max lag frames
+            e.spectra(40, 0)  # no longer held: the ring keeps chunk + lag block + max lag frames
+        with pytest.raises(ddm.DDMError):
+            e.spectra(100, 0)  # not submitted yet
         acc = e.accumulators(0)
-    cfg = o.Config(64, 64, 20, [64, 32], lags, [4.0, 8.0])
+    cfg = o.Config(64, 64, 100, [64, 32], lags, [4.0, 8.0])
     _, ref_acc = o.run(cfg, frames, acc_scale=0)
     assert acc.shape == ref_acc.shape
     assert np.max(np.abs(acc - ref_acc)) / np.max(np.abs(ref_acc)) < 1e-5
