@@ -74,8 +74,9 @@ struct Layout {
 // One in-place Stockham pass of radix R over all sequences of the layout.
 // Every thread loads all of its butterflies' inputs before the barrier and
 // writes after it, so the pass needs no second buffer.
-template <class LY, int R, int NS>
+template <class LY, int R, int NS, int NTH>
 __device__ __forceinline__ void stockham_pass(float2 *buf, const float2 *tw) {
+    constexpr int NT = NTH;
     constexpr int N = LY::n;
     constexpr int BPS = N / R;
     constexpr int TOTAL = BPS * LY::nseq;
@@ -114,13 +115,13 @@ __device__ __forceinline__ void stockham_pass(float2 *buf, const float2 *tw) {
     __syncthreads();
 }
 
-template <class LY, int NS>
+template <class LY, int NS, int NTH>
 __device__ __forceinline__ void fft_passes(float2 *buf, const float2 *tw) {
     if constexpr (NS < LY::n) {
         constexpr int REM = LY::n / NS;
         constexpr int R = (REM % 4 == 0) ? 4 : 2;
-        stockham_pass<LY, R, NS>(buf, tw);
-        fft_passes<LY, NS * R>(buf, tw);
+        stockham_pass<LY, R, NS, NTH>(buf, tw);
+        fft_passes<LY, NS * R, NTH>(buf, tw);
     }
 }
 
@@ -179,13 +180,15 @@ struct SmallCfg {
     static constexpr int H = S / 2;
     static constexpr int TPW = (2048 / (S * S)) > 0 ? (2048 / (S * S)) : 1;
     static constexpr int TILE = S * H;  // complex values per tile in LDS
+    // 128x128 tiles: 1024 threads, two butterflies per thread and pass
+    static constexpr int NTH = S >= 128 ? 1024 : 256;
 };
 
 template <int S>
-__global__ __launch_bounds__(NT) void tile_fft_small(PixSrc ps, RingDst rd, int tps, int tiles,
-                                                    int groups, const float2 *__restrict__ twg) {
+__global__ __launch_bounds__(SmallCfg<S>::NTH) void tile_fft_small(PixSrc ps, RingDst rd, int tps, int tiles,
+                                                                  int groups, const float2 *__restrict__ twg) {
     using C = SmallCfg<S>;
-    constexpr int H = C::H, TPW = C::TPW, TILE = C::TILE;
+    constexpr int H = C::H, TPW = C::TPW, TILE = C::TILE, NT = C::NTH;
     __shared__ float2 buf[TPW * TILE];
     __shared__ float2 tw[S];
     const int tid = threadIdx.x;
@@ -212,7 +215,7 @@ __global__ __launch_bounds__(NT) void tile_fft_small(PixSrc ps, RingDst rd, int 
 
     // row FFTs: TPW * H sequences of length S, contiguous
     using Rows = Layout<S, TPW * H, TPW * H, S, 0, 1, false>;
-    fft_passes<Rows, 1>(buf, tw);
+    fft_passes<Rows, 1, NT>(buf, tw);
 
     // separate the row pairs: Y[2r][k] at r*S + k, Y[2r+1][k] at r*S + H + k
     {
@@ -253,7 +256,7 @@ __global__ __launch_bounds__(NT) void tile_fft_small(PixSrc ps, RingDst rd, int 
 
     // column FFTs: per tile H columns of length S, stride H
     using Cols = Layout<S, TPW * H, H, 1, TILE, H, true>;
-    fft_passes<Cols, 1>(buf, tw);
+    fft_passes<Cols, 1, NT>(buf, tw);
 
     // write [tile][ky][kx], separating the packed DC / Nyquist column
     constexpr int OUT = S * (H + 1);
@@ -286,6 +289,7 @@ struct LargeCfg {
     static constexpr int H = S / 2;
     static constexpr int P = (2048 / S) > 0 ? (2048 / S) : 1;  // row pairs per workgroup
     static constexpr int CB = 8192 / S;                        // columns per workgroup
+    static constexpr int NTH_COLS = 1024;                      // two butterflies per thread and pass
 };
 
 template <int S>
@@ -309,7 +313,7 @@ __global__ __launch_bounds__(NT) void tile_fft_rows(PixSrc ps, RingDst rd, int t
     }
     __syncthreads();
     using Rows = Layout<S, P, P, S, 0, 1, false>;
-    fft_passes<Rows, 1>(buf, tw);
+    fft_passes<Rows, 1, NT>(buf, tw);
     const TileOut out = tile_out(rd, f, tile, (long long)S * (H + 1));
     for (int i = tid; i < P * H; i += NT) {
         const int r = i / H, k = i % H;
@@ -331,9 +335,10 @@ __global__ __launch_bounds__(NT) void tile_fft_rows(PixSrc ps, RingDst rd, int t
 
 // S >= 256, pass 2: CB columns per workgroup, FFT along ky, in place.
 template <int S>
-__global__ __launch_bounds__(NT) void tile_fft_cols(RingDst rd, int tiles, const float2 *__restrict__ twg) {
+__global__ __launch_bounds__(LargeCfg<S>::NTH_COLS) void tile_fft_cols(RingDst rd, int tiles,
+                                                                      const float2 *__restrict__ twg) {
     using C = LargeCfg<S>;
-    constexpr int H = C::H, CB = C::CB, NB = H / CB;
+    constexpr int H = C::H, CB = C::CB, NB = H / CB, NT = C::NTH_COLS;
     __shared__ float2 buf[S * CB];
     __shared__ float2 tw[S];
     const int tid = threadIdx.x;
@@ -349,7 +354,7 @@ __global__ __launch_bounds__(NT) void tile_fft_cols(RingDst rd, int tiles, const
     }
     __syncthreads();
     using Cols = Layout<S, CB, CB, 1, 0, CB, true>;
-    fft_passes<Cols, 1>(buf, tw);
+    fft_passes<Cols, 1, NT>(buf, tw);
     for (int i = tid; i < S * CB; i += NT) {
         const int ky = i / CB, c = i % CB;
         if (c0 + c == 0) {
@@ -440,7 +445,8 @@ static void launch_small(const PixSrc &ps, const RingDst &rd, int tps, int tiles
                          const float2 *tw, hipStream_t st) {
     constexpr int TPW = SmallCfg<S>::TPW;
     const int groups = (tiles + TPW - 1) / TPW;
-    hipLaunchKernelGGL(tile_fft_small<S>, dim3((unsigned)(groups * n_frames)), dim3(NT), 0, st, ps, rd, tps,
+    hipLaunchKernelGGL(tile_fft_small<S>, dim3((unsigned)(groups * n_frames)), dim3(SmallCfg<S>::NTH), 0, st, ps, rd,
+                       tps,
                        tiles, groups, tw);
 }
 
@@ -451,7 +457,7 @@ static void launch_large(const PixSrc &ps, const RingDst &rd, int tps, int tiles
     const unsigned g1 = (unsigned)((C::H / C::P) * tiles * n_frames);
     hipLaunchKernelGGL(tile_fft_rows<S>, dim3(g1), dim3(NT), 0, st, ps, rd, tps, tiles, tw);
     const unsigned g2 = (unsigned)((C::H / C::CB) * tiles * n_frames);
-    hipLaunchKernelGGL(tile_fft_cols<S>, dim3(g2), dim3(NT), 0, st, rd, tiles, tw);
+    hipLaunchKernelGGL(tile_fft_cols<S>, dim3(g2), dim3(C::NTH_COLS), 0, st, rd, tiles, tw);
 }
 
 hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st) {

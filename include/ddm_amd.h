@@ -76,7 +76,7 @@ typedef struct ddm_desc {
     float mask_tolerance; /* ring [q, tol * q] (-t: 1 + percent/100) */
     int chunk_frames;     /* frames per upload / FFT batch (-C) */
     int norm_mode;        /* enum ddm_norm_mode */
-    int lag_block;        /* frames per lag-engine pass: 0 = automatic (64), else 32, 64 or 128 */
+    int lag_block;        /* frames per lag-engine pass: 0 = automatic (128), else 32, 64 or 128 */
 } ddm_desc;
 
 /* Sizes the engine derived from a descriptor (after sorting). */
