@@ -174,7 +174,7 @@ def main():
                               "azimuthal": round(st.azimuthal_ms, 3)},
         "roofline": {
             "bound": "hbm",
-            "kernel": f"lag_kernel<{lag_block}, {32 if len(lags) <= 32 else 64}>",
+            "kernel": f"lag_kernel<{lag_block}>",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -196,20 +196,30 @@ __global__ __launch_bounds__(SmallCfg<S>::NTH) void tile_fft_small(PixSrc ps, Ri
     const int tile0 = (blockIdx.x % groups) * TPW;
     for (int i = tid; i < S; i += NT) tw[i] = twg[S + i];
 
-    // load: z[t][r][x] = x_{2r}[x] + i x_{2r+1}[x]
+    // load: z[t][r][x] = x_{2r}[x] + i x_{2r+1}[x]; every pixel load is in
+    // flight before the first LDS store
     const uint8_t *fr = ps.frames + (long long)f * ps.frame_bytes;
-    for (int i = tid; i < TPW * TILE; i += NT) {
-        const int t = i / TILE, rem = i % TILE;
-        const int r = rem / S, x = rem % S;
-        const int tile = tile0 + t;
-        float a = 0.f, b = 0.f;
-        if (tile < tiles) {
-            const int px = ps.x_off + (tile % tps) * S + x;
-            const int py = ps.y_off + (tile / tps) * S + 2 * r;
-            a = load_px(fr, ps, px, py);
-            b = load_px(fr, ps, px, py + 1);
+    {
+        constexpr int NL = TPW * TILE, KL = (NL + NT - 1) / NT;
+        float2 v[KL];
+#pragma unroll
+        for (int k = 0; k < KL; ++k) {
+            const int i = tid + k * NT;
+            const int t = i / TILE, rem = i % TILE;
+            const int r = rem / S, x = rem % S;
+            const int tile = tile0 + t;
+            v[k] = make_float2(0.f, 0.f);
+            if ((NL % NT == 0 || i < NL) && tile < tiles) {
+                const int px = ps.x_off + (tile % tps) * S + x;
+                const int py = ps.y_off + (tile / tps) * S + 2 * r;
+                v[k] = make_float2(load_px(fr, ps, px, py), load_px(fr, ps, px, py + 1));
+            }
         }
-        buf[i] = make_float2(a, b);
+#pragma unroll
+        for (int k = 0; k < KL; ++k) {
+            const int i = tid + k * NT;
+            if (NL % NT == 0 || i < NL) buf[i] = v[k];
+        }
     }
     __syncthreads();
 
@@ -307,9 +317,17 @@ __global__ __launch_bounds__(NT) void tile_fft_rows(PixSrc ps, RingDst rd, int t
     const uint8_t *fr = ps.frames + (long long)f * ps.frame_bytes;
     const int x0 = ps.x_off + (tile % tps) * S;
     const int y0 = ps.y_off + (tile / tps) * S + 2 * g * P;
-    for (int i = tid; i < P * S; i += NT) {
-        const int r = i / S, x = i % S;
-        buf[i] = make_float2(load_px(fr, ps, x0 + x, y0 + 2 * r), load_px(fr, ps, x0 + x, y0 + 2 * r + 1));
+    {
+        constexpr int KL = P * S / NT;  // P * S = 2048 = 8 * NT
+        float2 v[KL];
+#pragma unroll
+        for (int k = 0; k < KL; ++k) {
+            const int i = tid + k * NT;
+            const int r = i / S, x = i % S;
+            v[k] = make_float2(load_px(fr, ps, x0 + x, y0 + 2 * r), load_px(fr, ps, x0 + x, y0 + 2 * r + 1));
+        }
+#pragma unroll
+        for (int k = 0; k < KL; ++k) buf[tid + k * NT] = v[k];
     }
     __syncthreads();
     using Rows = Layout<S, P, P, S, 0, 1, false>;
@@ -348,9 +366,17 @@ __global__ __launch_bounds__(LargeCfg<S>::NTH_COLS) void tile_fft_cols(RingDst r
     for (int i = tid; i < S; i += NT) tw[i] = twg[S + i];
     const TileOut t = tile_out(rd, f, tile, (long long)S * (H + 1));
     const int c0 = cb * CB;
-    for (int i = tid; i < S * CB; i += NT) {
-        const int ly = i / CB, c = i % CB;
-        buf[i] = t[(long long)ly * (H + 1) + c0 + c];
+    {
+        constexpr int KL = S * CB / NT;  // S * CB = 8192 = 8 * NT
+        float2 v[KL];
+#pragma unroll
+        for (int k = 0; k < KL; ++k) {
+            const int i = tid + k * NT;
+            const int ly = i / CB, c = i % CB;
+            v[k] = t[(long long)ly * (H + 1) + c0 + c];
+        }
+#pragma unroll
+        for (int k = 0; k < KL; ++k) buf[tid + k * NT] = v[k];
     }
     __syncthreads();
     using Cols = Layout<S, CB, CB, 1, 0, CB, true>;

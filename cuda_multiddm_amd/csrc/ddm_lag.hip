@@ -65,10 +65,24 @@ __global__ __launch_bounds__(LAG_NT) void lag_kernel(const float2 *__restrict__ 
     const int lane = tid % RING_GROUP;
     const int w = __builtin_amdgcn_readfirstlane(tid / RING_GROUP);
     const float2 *g = ring + (long long)blockIdx.x * R * RING_GROUP + lane;  // slot s at g[s * RING_GROUP]
-    for (int j = w; j < n; j += LAG_WAVES) {
-        int s = slot0 + j;
-        s -= (s >= R) ? R : 0;
-        nb[j * RING_GROUP + lane] = g[(long long)s * RING_GROUP];
+    {
+        // stage the new frames: all loads in flight before the LDS stores
+        constexpr int K = (J + LAG_WAVES - 1) / LAG_WAVES;
+        float2 v[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int j = w + k * LAG_WAVES;
+            if (j < n) {
+                int s = slot0 + j;
+                s -= (s >= R) ? R : 0;
+                v[k] = g[(long long)s * RING_GROUP];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int j = w + k * LAG_WAVES;
+            if (j < n) nb[j * RING_GROUP + lane] = v[k];
+        }
     }
     __syncthreads();
     const float2 *nl = nb + lane;
