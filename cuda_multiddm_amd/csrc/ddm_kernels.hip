@@ -36,6 +36,11 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+
+// LDS index with one padding slot per 32 complex values: Stockham writes
+// whose lanes step by 4 or 16 values then fall on distinct banks.
+__device__ __forceinline__ int lp(int i) { return i + (i >> 5); }
+constexpr int lds_len(int n) { return n + (n >> 5) + 1; }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
 template <int R>
@@ -95,7 +100,7 @@ __device__ __forceinline__ void stockham_pass(float2 *buf, const float2 *tw) {
             const int kk = j % NS;
             const int base = LY::base(q);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[k][r] = buf[base + (j + r * BPS) * LY::estr];
+            for (int r = 0; r < R; ++r) v[k][r] = buf[lp(base + (j + r * BPS) * LY::estr)];
             if (NS > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) v[k][r] = cmul(v[k][r], tw[kk * r * TWS]);
@@ -109,7 +114,7 @@ __device__ __forceinline__ void stockham_pass(float2 *buf, const float2 *tw) {
     for (int k = 0; k < KB; ++k) {
         if ((TOTAL % NT == 0) || dst[k] >= 0) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[dst[k] + r * NS * LY::estr] = v[k][r];
+            for (int r = 0; r < R; ++r) buf[lp(dst[k] + r * NS * LY::estr)] = v[k][r];
         }
     }
     __syncthreads();
@@ -189,7 +194,7 @@ __global__ __launch_bounds__(SmallCfg<S>::NTH) void tile_fft_small(PixSrc ps, Ri
                                                                   int groups, const float2 *__restrict__ twg) {
     using C = SmallCfg<S>;
     constexpr int H = C::H, TPW = C::TPW, TILE = C::TILE, NT = C::NTH;
-    __shared__ float2 buf[TPW * TILE];
+    __shared__ float2 buf[lds_len(TPW * TILE)];
     __shared__ float2 tw[S];
     const int tid = threadIdx.x;
     const int f = blockIdx.x / groups;
@@ -218,7 +223,7 @@ __global__ __launch_bounds__(SmallCfg<S>::NTH) void tile_fft_small(PixSrc ps, Ri
 #pragma unroll
         for (int k = 0; k < KL; ++k) {
             const int i = tid + k * NT;
-            if (NL % NT == 0 || i < NL) buf[i] = v[k];
+            if (NL % NT == 0 || i < NL) buf[lp(i)] = v[k];
         }
     }
     __syncthreads();
@@ -238,8 +243,8 @@ __global__ __launch_bounds__(SmallCfg<S>::NTH) void tile_fft_small(PixSrc ps, Ri
             if ((NU % NT == 0) || i < NU) {
                 const int k = i % H, rr = i / H;  // rr = t * H + r
                 const int base = rr * S;
-                za[u] = buf[base + k];
-                zb[u] = buf[base + (k == 0 ? H : S - k)];
+                za[u] = buf[lp(base + k)];
+                zb[u] = buf[lp(base + (k == 0 ? H : S - k))];
             }
         }
         __syncthreads();
@@ -257,8 +262,8 @@ __global__ __launch_bounds__(SmallCfg<S>::NTH) void tile_fft_small(PixSrc ps, Ri
                     ya = make_float2(0.5f * (za[u].x + zb[u].x), 0.5f * (za[u].y - zb[u].y));
                     yb = make_float2(0.5f * (za[u].y + zb[u].y), -0.5f * (za[u].x - zb[u].x));
                 }
-                buf[base + k] = ya;
-                buf[base + H + k] = yb;
+                buf[lp(base + k)] = ya;
+                buf[lp(base + H + k)] = yb;
             }
         }
         __syncthreads();
@@ -275,15 +280,15 @@ __global__ __launch_bounds__(SmallCfg<S>::NTH) void tile_fft_small(PixSrc ps, Ri
         const int tile = tile0 + t;
         if (tile >= tiles) continue;
         const int ky = rem / (H + 1), kx = rem % (H + 1);
-        const float2 *Y = buf + t * TILE;
+        const int y0 = t * TILE;
         float2 val;
         if (kx == 0 || kx == H) {
-            const float2 w = Y[ky * H];
-            const float2 v = Y[((S - ky) & (S - 1)) * H];
+            const float2 w = buf[lp(y0 + ky * H)];
+            const float2 v = buf[lp(y0 + ((S - ky) & (S - 1)) * H)];
             val = (kx == 0) ? make_float2(0.5f * (w.x + v.x), 0.5f * (w.y - v.y))
                             : make_float2(0.5f * (w.y + v.y), -0.5f * (w.x - v.x));
         } else {
-            val = Y[ky * H + kx];
+            val = buf[lp(y0 + ky * H + kx)];
         }
         tile_out(rd, f, tile, OUT)[rem] = val;
     }
@@ -307,7 +312,7 @@ __global__ __launch_bounds__(NT) void tile_fft_rows(PixSrc ps, RingDst rd, int t
                                                    const float2 *__restrict__ twg) {
     using C = LargeCfg<S>;
     constexpr int H = C::H, P = C::P, GROUPS = H / P;
-    __shared__ float2 buf[P * S];
+    __shared__ float2 buf[lds_len(P * S)];
     __shared__ float2 tw[S];
     const int tid = threadIdx.x;
     const int g = blockIdx.x % GROUPS;
@@ -327,7 +332,7 @@ __global__ __launch_bounds__(NT) void tile_fft_rows(PixSrc ps, RingDst rd, int t
             v[k] = make_float2(load_px(fr, ps, x0 + x, y0 + 2 * r), load_px(fr, ps, x0 + x, y0 + 2 * r + 1));
         }
 #pragma unroll
-        for (int k = 0; k < KL; ++k) buf[tid + k * NT] = v[k];
+        for (int k = 0; k < KL; ++k) buf[lp(tid + k * NT)] = v[k];
     }
     __syncthreads();
     using Rows = Layout<S, P, P, S, 0, 1, false>;
@@ -335,8 +340,8 @@ __global__ __launch_bounds__(NT) void tile_fft_rows(PixSrc ps, RingDst rd, int t
     const TileOut out = tile_out(rd, f, tile, (long long)S * (H + 1));
     for (int i = tid; i < P * H; i += NT) {
         const int r = i / H, k = i % H;
-        const float2 za = buf[r * S + k];
-        const float2 zb = buf[r * S + (k == 0 ? H : S - k)];
+        const float2 za = buf[lp(r * S + k)];
+        const float2 zb = buf[lp(r * S + (k == 0 ? H : S - k))];
         float2 ya, yb;
         if (k == 0) {
             ya = make_float2(za.x, zb.x);
@@ -357,7 +362,7 @@ __global__ __launch_bounds__(LargeCfg<S>::NTH_COLS) void tile_fft_cols(RingDst r
                                                                       const float2 *__restrict__ twg) {
     using C = LargeCfg<S>;
     constexpr int H = C::H, CB = C::CB, NB = H / CB, NT = C::NTH_COLS;
-    __shared__ float2 buf[S * CB];
+    __shared__ float2 buf[lds_len(S * CB)];
     __shared__ float2 tw[S];
     const int tid = threadIdx.x;
     const int cb = blockIdx.x % NB;
@@ -376,7 +381,7 @@ __global__ __launch_bounds__(LargeCfg<S>::NTH_COLS) void tile_fft_cols(RingDst r
             v[k] = t[(long long)ly * (H + 1) + c0 + c];
         }
 #pragma unroll
-        for (int k = 0; k < KL; ++k) buf[tid + k * NT] = v[k];
+        for (int k = 0; k < KL; ++k) buf[lp(tid + k * NT)] = v[k];
     }
     __syncthreads();
     using Cols = Layout<S, CB, CB, 1, 0, CB, true>;
@@ -384,12 +389,12 @@ __global__ __launch_bounds__(LargeCfg<S>::NTH_COLS) void tile_fft_cols(RingDst r
     for (int i = tid; i < S * CB; i += NT) {
         const int ky = i / CB, c = i % CB;
         if (c0 + c == 0) {
-            const float2 w = buf[ky * CB];
-            const float2 v = buf[((S - ky) & (S - 1)) * CB];
+            const float2 w = buf[lp(ky * CB)];
+            const float2 v = buf[lp(((S - ky) & (S - 1)) * CB)];
             t[(long long)ky * (H + 1)] = make_float2(0.5f * (w.x + v.x), 0.5f * (w.y - v.y));
             t[(long long)ky * (H + 1) + H] = make_float2(0.5f * (w.y + v.y), -0.5f * (w.x - v.x));
         } else {
-            t[(long long)ky * (H + 1) + c0 + c] = buf[i];
+            t[(long long)ky * (H + 1) + c0 + c] = buf[lp(i)];
         }
     }
 }
