@@ -60,6 +60,100 @@ def percentile(a, p):
     return float(np.percentile(np.asarray(a, dtype=np.float64), p)) if len(a) else None
 
 
+def serving(args):
+    """Serving workload: 64 streams of 256^2 / 512^2 / 1024^2 uint8 frames
+    (size drawn uniformly), scales from the frame size down to 32, 24 log
+    lags up to 300, 30 log lambdas 4..256, chunks of 30 frames arriving as a
+    Poisson process at --rate chunks/s in aggregate for --seconds.  Streams
+    are admitted by device footprint; a stream leaves after --chunks-per-stream
+    chunks and a waiting one takes its place.  Reports frames/s and
+    chunk-to-result latency percentiles."""
+    import cuda_multiddm_amd as ddm
+
+    rng = np.random.default_rng(7)
+    sizes = [int(s) for s in rng.choice([256, 512, 1024], size=args.streams)]
+    lags = ddm.log_lags(300, 24)
+    lams = ddm.log_lambdas(4, 256, 30)
+    pool = {s: ddm.synth_brownian(30, s, s, 8, max(1, 2000 * s * s // (512 * 512)), seed=s,
+                                  nthreads=min(os.cpu_count() or 1, 16)) for s in (256, 512, 1024)}
+
+    def desc(s):
+        sc = [1 << k for k in range(int(np.log2(s)), 4, -1)]
+        return dict(width=s, height=s, scales=sc, lags=lags, lambdas=lams, chunk=30, lag_block=32)
+
+    srv = ddm.Server(max_streams=args.streams)
+    waiting = list(range(args.streams))
+    active, sent = {}, {}
+    admitted, refused = 0, 0
+
+    def admit():
+        nonlocal admitted, refused
+        while waiting:
+            k = waiting[0]
+            try:
+                sid = srv.open(**desc(sizes[k]))
+            except ddm.DDMError as e:
+                if e.code != ddm.DDM_EBUSY:
+                    raise
+                refused += 1
+                return
+            waiting.pop(0)
+            active[k] = sid
+            sent[k] = 0
+            admitted += 1
+
+    admit()
+    t_start = time.perf_counter()
+    t_next = t_start
+    frames = 0
+    completed = 0
+    while time.perf_counter() - t_start < args.seconds:
+        now = time.perf_counter()
+        if now < t_next:
+            time.sleep(min(t_next - now, 0.002))
+            continue
+        t_next += rng.exponential(1.0 / args.rate)
+        if not active:
+            admit()
+            continue
+        k = list(active)[rng.integers(len(active))]
+        srv.submit(active[k], pool[sizes[k]])
+        frames += 30
+        sent[k] += 1
+        if sent[k] >= args.chunks_per_stream:
+            srv.results(active[k])
+            srv.close_stream(active.pop(k))
+            completed += 1
+            waiting.append(k)  # the stream comes back later as a new request
+            admit()
+    srv.drain()
+    elapsed = time.perf_counter() - t_start
+    lat = srv.latencies()
+    st = srv.stats()
+    out = {
+        "metric": "serving: analysed frames/s + chunk-to-result latency",
+        "value": round(frames / elapsed, 1),
+        "unit": "frames/s",
+        "offered_chunks_per_s": args.rate,
+        "seconds": round(elapsed, 2),
+        "p50_chunk_ms": percentile(lat, 50),
+        "p99_chunk_ms": percentile(lat, 99),
+        "max_chunk_ms": float(np.max(lat)) if len(lat) else None,
+        "chunks": int(st.chunks),
+        "streams_admitted": admitted,
+        "streams_completed": completed,
+        "admission_refusals": refused,
+        "active_streams_at_end": st.streams,
+        "device_budget_GB": round(st.budget_bytes / 1e9, 1),
+        "device_used_GB_at_end": round(st.used_bytes / 1e9, 1),
+        "config": {"workload": f"{args.streams} streams, frame sizes {{256,512,1024}}^2 u8, scales to 32, "
+                               f"24 log lags <= 300, 30 log lambdas 4-256, chunk 30, lag block 32, "
+                               f"Poisson arrivals {args.rate} chunks/s"},
+    }
+    srv.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,7 +168,15 @@ def main():
                     help="only run warmup + timed steps (for rocprofv3 runs)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="with --profile-steps: finish with a device copy of this many MiB")
+    ap.add_argument("--serving", action="store_true", help="run the multi-stream serving workload instead")
+    ap.add_argument("--streams", type=int, default=64)
+    ap.add_argument("--rate", type=float, default=200.0, help="serving: offered chunks per second")
+    ap.add_argument("--seconds", type=float, default=20.0, help="serving: arrival window")
+    ap.add_argument("--chunks-per-stream", type=int, default=12)
     args = ap.parse_args()
+    if args.serving and args.gpus == 1:
+        serving(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))

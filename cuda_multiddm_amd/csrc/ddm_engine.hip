@@ -395,6 +395,15 @@ extern "C" int ddm_submit_host(ddm_engine *e, const void *frames, int n_frames) 
     return DDM_OK;
 }
 
+extern "C" int ddm_flush(ddm_engine *e) {
+    if (!e) return DDM_EINVAL;
+    return flush_lags(e);
+}
+
+namespace ddm {
+hipStream_t engine_stream(ddm_engine *e) { return e ? e->st : nullptr; }
+}  // namespace ddm
+
 extern "C" int ddm_sync(ddm_engine *e) {
     if (!e) return DDM_EINVAL;
     HIP_TRY(e, hipStreamSynchronize(e->st));

@@ -117,6 +117,16 @@ int check_desc(const ddm_desc *d, Params *out, std::string *msg) {
     return DDM_OK;
 }
 
+size_t device_footprint(const Params &p) {
+    const size_t L = p.lags.size(), Q = p.lambdas.size();
+    size_t bytes = (size_t)p.ring_slots * (size_t)p.elems_padded * 8;  // spectrum ring
+    bytes += L * (size_t)p.elems_padded * 4;                           // accumulators
+    bytes += 2 * (size_t)p.chunk * p.M * p.M * p.bpp;                  // upload staging
+    bytes += (size_t)p.result_count * 8 + L * 12;                      // results, lags, norms
+    for (const ScaleGeom &g : p.geom) bytes += Q * ((size_t)g.tile_elems * 4 + 8);  // ring tables (bound)
+    return bytes;
+}
+
 int build_ring_tables(const Params &p, RingTables *out) {
     const int Q = (int)p.lambdas.size();
     out->idx.clear();
@@ -312,6 +322,7 @@ extern "C" const char *ddm_status_string(int code) {
     case DDM_ELAMBDA: return "invalid lambda";
     case DDM_ESTATE: return "invalid engine state";
     case DDM_ERANGE: return "out of range";
+    case DDM_EBUSY: return "busy: no slot or device budget left";
     default: return "unknown status";
     }
 }
@@ -335,7 +346,7 @@ extern "C" int ddm_validate(const ddm_desc *desc, ddm_info *info, char *msg, siz
         info->result_count = p.result_count;
         info->elems_per_frame_padded = p.elems_padded;
         info->lag_block = p.lag_block;
-        info->device_bytes = (size_t)p.ring_slots * p.elems_padded * 8 + (size_t)p.lags.size() * p.elems_padded * 4;
+        info->device_bytes = device_footprint(p);
     }
     return rc;
 }

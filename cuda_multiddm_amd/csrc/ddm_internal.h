@@ -62,6 +62,10 @@ struct Params {
 // negative code with the reference's message text where one exists.
 int check_desc(const ddm_desc *d, Params *out, std::string *msg);
 
+// Device memory an engine for these parameters allocates (an upper bound:
+// the q-ring tables are counted at their largest possible size).
+size_t device_footprint(const Params &p);
+
 // Element lists of every q ring of every scale: for scale i and q j the
 // member elements (ky * (s/2+1) + kx, ascending) are
 // idx[off[i*Q + j] .. off[i*Q + j] + cnt[i*Q + j]).

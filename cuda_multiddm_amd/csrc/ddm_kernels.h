@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "ddm_amd.h"
+
 namespace ddm {
 
 struct TileFftArgs {
@@ -53,6 +55,9 @@ struct AzArgs {
 };
 
 hipError_t twiddle_table(const float2 **out);
+
+// The HIP stream an engine launches its kernels on (ddm_engine.hip).
+hipStream_t engine_stream(ddm_engine *e);
 hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st);
 hipError_t launch_lag(const LagArgs &a, hipStream_t st);
 hipError_t launch_azimuthal(const AzArgs &a, hipStream_t st);

@@ -22,7 +22,9 @@ CLI_PATH = os.path.join(_HERE, "_build", "ddm")
 
 DDM_OK = 0
 STATUS = {0: "DDM_OK", -1: "DDM_EINVAL", -2: "DDM_ENOMEM", -3: "DDM_EDEVICE", -4: "DDM_EROI",
-          -5: "DDM_ESCALE", -6: "DDM_ELAG", -7: "DDM_ELAMBDA", -8: "DDM_ESTATE", -9: "DDM_ERANGE"}
+          -5: "DDM_ESCALE", -6: "DDM_ELAG", -7: "DDM_ELAMBDA", -8: "DDM_ESTATE", -9: "DDM_ERANGE",
+          -10: "DDM_EBUSY"}
+DDM_EBUSY = -10
 PIX_U8, PIX_U16LE, PIX_U16BE = 0, 1, 2
 NORM_FRAMES, NORM_PAIRS = 0, 1
 
@@ -32,7 +34,10 @@ EXPORTED = [
     "ddm_results", "ddm_reset", "ddm_get_info", "ddm_get_params", "ddm_get_accumulators",
     "ddm_get_spectra", "ddm_set_timing", "ddm_get_stage_stats", "ddm_get_chunk_times",
     "ddm_last_error", "ddm_status_string", "ddm_k_tile_fft", "ddm_k_lag_accumulate", "ddm_ring_mask",
-    "ddm_synth_uniform", "ddm_synth_brownian", "ddm_write_iqt_file",
+    "ddm_synth_uniform", "ddm_synth_brownian", "ddm_write_iqt_file", "ddm_flush",
+    "ddm_server_create", "ddm_server_destroy", "ddm_server_open", "ddm_server_submit", "ddm_server_results",
+    "ddm_server_engine", "ddm_server_drain", "ddm_server_close", "ddm_server_latencies", "ddm_server_stats",
+    "ddm_server_last_error",
 ]
 
 
@@ -70,6 +75,14 @@ class StageStats(ctypes.Structure):
         ("fft_ms", ctypes.c_double), ("lag_ms", ctypes.c_double), ("azimuthal_ms", ctypes.c_double),
         ("fft_launches", ctypes.c_longlong), ("lag_launches", ctypes.c_longlong),
         ("azimuthal_launches", ctypes.c_longlong), ("chunks", ctypes.c_longlong),
+    ]
+
+
+class ServerInfo(ctypes.Structure):
+    _fields_ = [
+        ("budget_bytes", ctypes.c_size_t), ("used_bytes", ctypes.c_size_t), ("streams", ctypes.c_int),
+        ("max_streams", ctypes.c_int), ("frames", ctypes.c_longlong), ("chunks", ctypes.c_longlong),
+        ("chunks_in_flight", ctypes.c_longlong),
     ]
 
 
@@ -120,6 +133,20 @@ def lib():
                                      ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                      ctypes.c_float, ctypes.c_uint64, ctypes.c_int]
     L.ddm_write_iqt_file.argtypes = [ctypes.c_char_p, dp, fp, ctypes.c_int, ip, ctypes.c_int, ctypes.c_float]
+    L.ddm_flush.argtypes = [vp]
+    L.ddm_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(vp)]
+    L.ddm_server_destroy.argtypes = [vp]
+    L.ddm_server_destroy.restype = None
+    L.ddm_server_open.argtypes = [vp, ctypes.POINTER(Desc), ctypes.POINTER(ctypes.c_int)]
+    L.ddm_server_submit.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int]
+    L.ddm_server_results.argtypes = [vp, ctypes.c_int, dp]
+    L.ddm_server_engine.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+    L.ddm_server_drain.argtypes = [vp]
+    L.ddm_server_close.argtypes = [vp, ctypes.c_int]
+    L.ddm_server_latencies.argtypes = [vp, ctypes.c_int, fp, ctypes.c_int]
+    L.ddm_server_stats.argtypes = [vp, ctypes.POINTER(ServerInfo)]
+    L.ddm_server_last_error.argtypes = [vp]
+    L.ddm_server_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -263,6 +290,92 @@ class Engine:
         n = self._check(lib().ddm_get_chunk_times(self._h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                                                   max_n))
         return buf[:n].copy()
+
+
+class Server:
+    """Many DDM streams on one device (serving): admission by device
+    footprint, chunks of different streams overlapping on the GPU, and
+    chunk-to-result latencies."""
+
+    def __init__(self, device_budget: int = 0, max_streams: int = 256):
+        h = ctypes.c_void_p()
+        rc = lib().ddm_server_create(device_budget, max_streams, ctypes.byref(h))
+        if rc != DDM_OK:
+            raise DDMError(rc, lib().ddm_server_last_error(None).decode())
+        self._h = h
+        self._keep = {}
+        self._shape = {}
+
+    def _check(self, rc):
+        if rc < 0:
+            raise DDMError(rc, lib().ddm_server_last_error(self._h).decode())
+        return rc
+
+    def open(self, width, height, scales, lags, lambdas, tolerance=1.2, chunk=30, bpp=1, pixfmt=PIX_U8,
+             x_off=0, y_off=0, norm=NORM_FRAMES, lag_block=0) -> int:
+        """Returns the stream id; raises DDMError with code DDM_EBUSY when the
+        stream does not fit yet."""
+        d, keep = make_desc(width, height, scales, lags, lambdas, tolerance, chunk, bpp, pixfmt, x_off, y_off,
+                            norm, lag_block)
+        sid = ctypes.c_int()
+        self._check(lib().ddm_server_open(self._h, ctypes.byref(d), ctypes.byref(sid)))
+        self._keep[sid.value] = keep
+        self._shape[sid.value] = (int(width), int(height), int(bpp))
+        return sid.value
+
+    def submit(self, sid: int, frames: np.ndarray):
+        w, h, b = self._shape[sid]
+        frames = np.ascontiguousarray(frames)
+        n = frames.nbytes // (w * h * b)
+        self._check(lib().ddm_server_submit(self._h, sid, frames.ctypes.data_as(ctypes.c_void_p), n))
+
+    def engine_info(self, sid: int) -> Info:
+        e = ctypes.c_void_p()
+        self._check(lib().ddm_server_engine(self._h, sid, ctypes.byref(e)))
+        i = Info()
+        self._check(lib().ddm_get_info(e, ctypes.byref(i)))
+        return i
+
+    def results(self, sid: int) -> np.ndarray:
+        out = np.zeros(self.engine_info(sid).result_count, dtype=np.float64)
+        self._check(lib().ddm_server_results(self._h, sid, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def drain(self):
+        self._check(lib().ddm_server_drain(self._h))
+
+    def close_stream(self, sid: int):
+        self._check(lib().ddm_server_close(self._h, sid))
+        self._keep.pop(sid, None)
+        self._shape.pop(sid, None)
+
+    def latencies(self, sid: int = -1, max_n: int = 1 << 22) -> np.ndarray:
+        buf = np.zeros(max_n, dtype=np.float32)
+        n = self._check(lib().ddm_server_latencies(self._h, sid, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                   max_n))
+        return buf[:n].copy()
+
+    def stats(self) -> ServerInfo:
+        s = ServerInfo()
+        self._check(lib().ddm_server_stats(self._h, ctypes.byref(s)))
+        return s
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().ddm_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def run_ddm(frames: np.ndarray, width: int, height: int, scales: Sequence[int], lags: Sequence[int],

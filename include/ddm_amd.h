@@ -48,6 +48,7 @@ extern "C" {
 #define DDM_ELAMBDA (-7)  /* lambda negative or repeated */
 #define DDM_ESTATE (-8)   /* call not valid in the engine's current state */
 #define DDM_ERANGE (-9)   /* frame or index outside what the engine holds */
+#define DDM_EBUSY (-10)   /* server: no stream slot or not enough device budget left */
 
 /* Pixel formats.  A pixel is bytes_per_pixel bytes; only channel 0 is used.
  * DDM_PIX_U8 reads byte 0 of each pixel (parseBufferScalePow2,
@@ -130,6 +131,11 @@ int ddm_submit_device(ddm_engine *eng, const void *frames, int n_frames);
 /* Waits for all submitted work. */
 int ddm_sync(ddm_engine *eng);
 
+/* Runs the lag engine on frames still waiting for a full lag block
+ * (asynchronous).  ddm_results, ddm_reset and ddm_get_accumulators flush by
+ * themselves; latency-sensitive callers flush after every chunk. */
+int ddm_flush(ddm_engine *eng);
+
 /* Masked azimuthal average of everything accumulated since the last reset:
  * results[info.result_count] doubles, layout [scale][tile][q][lag].
  * Replaces analyse_accums / analyseFFTDevice (src/DDM.cu:72-146,
@@ -165,6 +171,45 @@ int ddm_get_chunk_times(ddm_engine *eng, float *ms, int max);
  * ddm_create on the calling thread. */
 const char *ddm_last_error(const ddm_engine *eng);
 const char *ddm_status_string(int code);
+
+/* -------------------------------------------------------------------------
+ * Serving: many streams on one device.  No counterpart in the reference
+ * (it analyses one video per process, src/DDM.cu:268-896).  A stream is
+ * admitted when its device footprint (ddm_info.device_bytes) fits in the
+ * budget left; each stream runs on its own HIP stream, so chunks of
+ * different streams overlap on the device.  Every submitted chunk is
+ * flushed through the lag engine and its chunk-to-result latency (host
+ * submission -> chunk accumulated on the device) is recorded.  Results of a
+ * stream depend only on its own frames.  The server functions may be called
+ * from one host thread at a time.
+ * ------------------------------------------------------------------------- */
+typedef struct ddm_server ddm_server;
+
+typedef struct ddm_server_info {
+    size_t budget_bytes, used_bytes;
+    int streams, max_streams;
+    long long frames, chunks, chunks_in_flight;
+} ddm_server_info;
+
+/* device_budget 0 = 90 % of the device memory free at creation. */
+int ddm_server_create(size_t device_budget, int max_streams, ddm_server **out);
+void ddm_server_destroy(ddm_server *srv);
+/* DDM_EBUSY when no slot is free or the footprint does not fit yet. */
+int ddm_server_open(ddm_server *srv, const ddm_desc *desc, int *stream_id);
+/* Host frames of one chunk (any count); returns once the chunk is queued. */
+int ddm_server_submit(ddm_server *srv, int stream_id, const void *frames, int n_frames);
+int ddm_server_results(ddm_server *srv, int stream_id, double *results);
+/* The stream's engine, for ddm_get_info / ddm_get_params / ddm_reset. */
+int ddm_server_engine(ddm_server *srv, int stream_id, ddm_engine **eng);
+/* Waits until every submitted chunk of every stream is done. */
+int ddm_server_drain(ddm_server *srv);
+int ddm_server_close(ddm_server *srv, int stream_id);
+/* Chunk-to-result latencies (ms) of finished chunks of one stream, or of
+ * all streams with stream_id < 0; returns the number written. */
+int ddm_server_latencies(ddm_server *srv, int stream_id, float *ms, int max);
+int ddm_server_stats(ddm_server *srv, ddm_server_info *info);
+/* With srv == NULL: the last ddm_server_create failure on this thread. */
+const char *ddm_server_last_error(const ddm_server *srv);
 
 /* -------------------------------------------------------------------------
  * Stage entry points on caller-owned device buffers (the kernels the
