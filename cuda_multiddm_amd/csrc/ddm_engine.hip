@@ -252,9 +252,10 @@ extern "C" int ddm_create(const ddm_desc *desc, ddm_engine **out) {
     HTRY(hipMemcpy(e->d_qcnt, e->rt.cnt.data(), e->rt.cnt.size() * sizeof(int), hipMemcpyHostToDevice));
     if (!e->rt.idx.empty())
         HTRY(hipMemcpy(e->d_idx, e->rt.idx.data(), e->rt.idx.size() * sizeof(int), hipMemcpyHostToDevice));
+    // The ring needs no clearing: the lag engine only pairs frames that were
+    // written, and the padding elements of the last ring group feed only
+    // accumulator entries that no result reads.
     HTRY(hipMemsetAsync(e->d_acc, 0, L * (size_t)E * sizeof(float), e->st));
-    // the padding elements of the last ring group are read by the lag engine
-    HTRY(hipMemsetAsync(e->d_ring, 0, (size_t)p.ring_slots * (size_t)E * 2 * sizeof(float), e->st));
     const float2 *tw = nullptr;
     HTRY(ddm::twiddle_table(&tw));
     HTRY(hipStreamSynchronize(e->st));
@@ -372,20 +373,35 @@ extern "C" int ddm_submit_host(ddm_engine *e, const void *frames, int n_frames) 
     const size_t roi_frame = row * p.M;
     const uint8_t *src = (const uint8_t *)frames;
     const size_t fb = (size_t)p.width * p.height * p.bpp;
+    // Frames that already sit in pinned memory and are exactly the ROI go
+    // to the device without the staging copy.
+    bool direct = false;
+    if (p.x_off == 0 && p.y_off == 0 && p.width == p.M && p.height == p.M && n_frames > 0) {
+        hipPointerAttribute_t attr;
+        direct = hipPointerGetAttributes(&attr, frames) == hipSuccess && attr.type == hipMemoryTypeHost;
+        (void)hipGetLastError();  // pageable memory reports an error here; do not leave it behind
+    }
     for (int i = 0; i < n_frames; i += p.chunk) {
         const int n = std::min(p.chunk, n_frames - i);
         const int b = e->next_stage;
         if (e->stage_used[b]) HIP_TRY(e, hipEventSynchronize(e->ev_free[b]));
-        // ROI-only copy into pinned memory (the reference uploads whole frames, src/DDM.cu:506)
-        uint8_t *dst = e->h_stage[b];
-        for (int f = 0; f < n; ++f) {
-            const uint8_t *fr = src + (size_t)(i + f) * fb;
-            for (int y = 0; y < p.M; ++y)
-                std::memcpy(dst + (size_t)f * roi_frame + (size_t)y * row,
-                            fr + ((size_t)(p.y_off + y) * p.width + p.x_off) * p.bpp, row);
+        if (direct) {
+            HIP_TRY(e, hipMemcpyAsync(e->d_stage[b], src + (size_t)i * fb, roi_frame * n, hipMemcpyHostToDevice,
+                                      e->st_copy));
+        } else {
+            // ROI-only copy into pinned memory (the reference uploads whole frames, src/DDM.cu:506)
+            uint8_t *dst = e->h_stage[b];
+            for (int f = 0; f < n; ++f) {
+                const uint8_t *fr = src + (size_t)(i + f) * fb;
+                for (int y = 0; y < p.M; ++y)
+                    std::memcpy(dst + (size_t)f * roi_frame + (size_t)y * row,
+                                fr + ((size_t)(p.y_off + y) * p.width + p.x_off) * p.bpp, row);
+            }
+            HIP_TRY(e, hipMemcpyAsync(e->d_stage[b], dst, roi_frame * n, hipMemcpyHostToDevice, e->st_copy));
         }
-        HIP_TRY(e, hipMemcpyAsync(e->d_stage[b], dst, roi_frame * n, hipMemcpyHostToDevice, e->st_copy));
         HIP_TRY(e, hipEventRecord(e->ev_up[b], e->st_copy));
+        // the caller may reuse its buffer as soon as this call returns
+        if (direct) HIP_TRY(e, hipEventSynchronize(e->ev_up[b]));
         HIP_TRY(e, hipStreamWaitEvent(e->st, e->ev_up[b], 0));
         int rc = process_chunk(e, e->d_stage[b], n, (long long)roi_frame, p.M, 0, 0, e->ev_free[b]);
         if (rc) return rc;

@@ -7,7 +7,11 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <thread>
+#include <tuple>
 
 #include "ddm_internal.h"
 
@@ -127,6 +131,31 @@ size_t device_footprint(const Params &p) {
     return bytes;
 }
 
+// Member lists of one (scale, lambda, tolerance) ring, cached for the life of
+// the process: a serving process opens many streams with the same rings.
+static std::mutex g_ring_mu;
+static std::map<std::tuple<int, uint32_t, uint32_t>, std::shared_ptr<const std::vector<int>>> g_rings;
+
+static std::shared_ptr<const std::vector<int>> ring_members(int s, float lambda, float tol) {
+    uint32_t lb, tb;
+    std::memcpy(&lb, &lambda, 4);
+    std::memcpy(&tb, &tol, 4);
+    const auto key = std::make_tuple(s, lb, tb);
+    {
+        std::lock_guard<std::mutex> lk(g_ring_mu);
+        auto it = g_rings.find(key);
+        if (it != g_rings.end()) return it->second;
+    }
+    std::vector<uint8_t> m((size_t)s * (s / 2 + 1));
+    if (ddm_ring_mask(s, lambda, tol, m.data()) < 0) return nullptr;
+    auto v = std::make_shared<std::vector<int>>();
+    for (size_t e = 0; e < m.size(); ++e)
+        if (m[e]) v->push_back((int)e);
+    std::lock_guard<std::mutex> lk(g_ring_mu);
+    g_rings.emplace(key, v);
+    return v;
+}
+
 int build_ring_tables(const Params &p, RingTables *out) {
     const int Q = (int)p.lambdas.size();
     out->idx.clear();
@@ -134,14 +163,12 @@ int build_ring_tables(const Params &p, RingTables *out) {
     out->cnt.assign(p.geom.size() * Q, 0);
     for (size_t si = 0; si < p.geom.size(); ++si) {
         const int s = p.geom[si].s;
-        std::vector<uint8_t> m((size_t)p.geom[si].tile_elems);
         for (int q = 0; q < Q; ++q) {
-            int c = ddm_ring_mask(s, p.lambdas[q], p.tol, m.data());
-            if (c < 0) return c;
+            auto members = ring_members(s, p.lambdas[q], p.tol);
+            if (!members) return DDM_EINVAL;
             out->off[si * Q + q] = (int)out->idx.size();
-            out->cnt[si * Q + q] = c;
-            for (size_t e = 0; e < m.size(); ++e)
-                if (m[e]) out->idx.push_back((int)e);
+            out->cnt[si * Q + q] = (int)members->size();
+            out->idx.insert(out->idx.end(), members->begin(), members->end());
         }
     }
     return DDM_OK;
