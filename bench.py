@@ -91,13 +91,21 @@ def serving(args):
     waiting = list(range(args.streams))
     active, sent = {}, {}
     admitted, refused = 0, 0
+    host_s = {"open": 0.0, "submit": 0.0, "results": 0.0, "close": 0.0}
+
+    def timed(name, fn, *a):
+        t = time.perf_counter()
+        try:
+            return fn(*a)
+        finally:
+            host_s[name] += time.perf_counter() - t
 
     def admit():
         nonlocal admitted, refused
         while waiting:
             k = waiting[0]
             try:
-                sid = srv.open(**desc(sizes[k]))
+                sid = timed("open", lambda: srv.open(**desc(sizes[k])))
             except ddm.DDMError as e:
                 if e.code != ddm.DDM_EBUSY:
                     raise
@@ -123,12 +131,12 @@ def serving(args):
             admit()
             continue
         k = list(active)[rng.integers(len(active))]
-        srv.submit(active[k], pool[sizes[k]])
+        timed("submit", srv.submit, active[k], pool[sizes[k]])
         frames += 30
         sent[k] += 1
         if sent[k] >= args.chunks_per_stream:
-            srv.results(active[k])
-            srv.close_stream(active.pop(k))
+            timed("results", srv.results, active[k])
+            timed("close", srv.close_stream, active.pop(k))
             completed += 1
             waiting.append(k)  # the stream comes back later as a new request
             admit()
@@ -152,6 +160,7 @@ def serving(args):
         "active_streams_at_end": st.streams,
         "device_budget_GB": round(st.budget_bytes / 1e9, 1),
         "device_used_GB_at_end": round(st.used_bytes / 1e9, 1),
+        "host_seconds": {k: round(v, 2) for k, v in host_s.items()},
         "config": {"workload": f"{args.streams} streams, frame sizes {{256,512,1024}}^2 u8, scales to 32, "
                                f"24 log lags <= 300, 30 log lambdas 4-256, chunk 30, lag block 32, "
                                f"Poisson arrivals {args.rate} chunks/s"},

@@ -143,10 +143,13 @@ static void release(ddm_engine *e) {
     for (hipEvent_t ev : e->pool)
         if (ev) (void)hipEventDestroy(ev);
     e->pool.clear();
+    // device buffers come from the stream-ordered pool: freeing them does not
+    // synchronise the device (other engines keep running)
     void *dev[] = {e->d_ring, e->d_acc, e->d_lags, e->d_az, e->d_qoff, e->d_qcnt,
                    e->d_idx,  e->d_norm, e->d_out, e->d_stage[0], e->d_stage[1]};
     for (void *ptr : dev)
-        if (ptr) (void)hipFree(ptr);
+        if (ptr && e->st) (void)hipFreeAsync(ptr, e->st);
+    if (e->st) (void)hipStreamSynchronize(e->st);
     for (int i = 0; i < 2; ++i) {
         if (e->h_stage[i]) (void)hipHostFree(e->h_stage[i]);
         if (e->ev_up[i]) (void)hipEventDestroy(e->ev_up[i]);
@@ -159,7 +162,7 @@ static void release(ddm_engine *e) {
 template <class T>
 static int dev_alloc(ddm_engine *e, T **ptr, size_t count, const char *what) {
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    hipError_t rc = hipMalloc((void **)ptr, bytes);
+    hipError_t rc = hipMallocAsync((void **)ptr, bytes, e->st);
     if (rc != hipSuccess) {
         *ptr = nullptr;
         e->err = std::string("device allocation of ") + what + " (" + std::to_string(bytes) +
@@ -229,11 +232,8 @@ extern "C" int ddm_create(const ddm_desc *desc, ddm_engine **out) {
     TRY(dev_alloc(e, &e->d_norm, L, "normalisation"));
     TRY(dev_alloc(e, &e->d_out, (size_t)p.result_count, "results"));
     const size_t roi_frame = (size_t)p.M * p.M * p.bpp;
-    for (int i = 0; i < 2; ++i) {
-        TRY(dev_alloc(e, &e->d_stage[i], roi_frame * p.chunk, "upload staging"));
-        HTRY(hipHostMalloc((void **)&e->h_stage[i], roi_frame * p.chunk, hipHostMallocDefault));
-        e->pinned_bytes += roi_frame * p.chunk;
-    }
+    // pinned host staging is allocated on the first host submission that needs it
+    for (int i = 0; i < 2; ++i) TRY(dev_alloc(e, &e->d_stage[i], roi_frame * p.chunk, "upload staging"));
     std::vector<ddm::AzScale> az;
     long long out_base = 0;
     for (const ddm::ScaleGeom &g : p.geom) {
@@ -246,12 +246,15 @@ extern "C" int ddm_create(const ddm_desc *desc, ddm_engine **out) {
         az.push_back(a);
         out_base += (long long)g.tiles * (long long)p.lambdas.size() * (long long)L;
     }
-    HTRY(hipMemcpy(e->d_lags, p.lags.data(), L * sizeof(int), hipMemcpyHostToDevice));
-    HTRY(hipMemcpy(e->d_az, az.data(), az.size() * sizeof(ddm::AzScale), hipMemcpyHostToDevice));
-    HTRY(hipMemcpy(e->d_qoff, e->rt.off.data(), e->rt.off.size() * sizeof(int), hipMemcpyHostToDevice));
-    HTRY(hipMemcpy(e->d_qcnt, e->rt.cnt.data(), e->rt.cnt.size() * sizeof(int), hipMemcpyHostToDevice));
+    // stream-ordered after the allocations; the host arrays live until the
+    // synchronisation below
+    HTRY(hipMemcpyAsync(e->d_lags, p.lags.data(), L * sizeof(int), hipMemcpyHostToDevice, e->st));
+    HTRY(hipMemcpyAsync(e->d_az, az.data(), az.size() * sizeof(ddm::AzScale), hipMemcpyHostToDevice, e->st));
+    HTRY(hipMemcpyAsync(e->d_qoff, e->rt.off.data(), e->rt.off.size() * sizeof(int), hipMemcpyHostToDevice, e->st));
+    HTRY(hipMemcpyAsync(e->d_qcnt, e->rt.cnt.data(), e->rt.cnt.size() * sizeof(int), hipMemcpyHostToDevice, e->st));
     if (!e->rt.idx.empty())
-        HTRY(hipMemcpy(e->d_idx, e->rt.idx.data(), e->rt.idx.size() * sizeof(int), hipMemcpyHostToDevice));
+        HTRY(hipMemcpyAsync(e->d_idx, e->rt.idx.data(), e->rt.idx.size() * sizeof(int), hipMemcpyHostToDevice,
+                            e->st));
     // The ring needs no clearing: the lag engine only pairs frames that were
     // written, and the padding elements of the last ring group feed only
     // accumulator entries that no result reads.
@@ -389,6 +392,10 @@ extern "C" int ddm_submit_host(ddm_engine *e, const void *frames, int n_frames) 
             HIP_TRY(e, hipMemcpyAsync(e->d_stage[b], src + (size_t)i * fb, roi_frame * n, hipMemcpyHostToDevice,
                                       e->st_copy));
         } else {
+            if (!e->h_stage[b]) {
+                HIP_TRY(e, hipHostMalloc((void **)&e->h_stage[b], roi_frame * p.chunk, hipHostMallocDefault));
+                e->pinned_bytes += roi_frame * p.chunk;
+            }
             // ROI-only copy into pinned memory (the reference uploads whole frames, src/DDM.cu:506)
             uint8_t *dst = e->h_stage[b];
             for (int f = 0; f < n; ++f) {

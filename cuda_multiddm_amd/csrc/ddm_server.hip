@@ -96,6 +96,14 @@ extern "C" int ddm_server_create(size_t device_budget, int max_streams, ddm_serv
         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return DDM_EDEVICE;
         device_budget = free_b / 10 * 9;
     }
+    // Engines allocate from the device's stream-ordered pool; keeping freed
+    // memory in the pool makes admitting a stream after another left cheap.
+    int dev = 0;
+    hipMemPool_t pool = nullptr;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     ddm_server *s = new ddm_server();
     s->budget = device_budget;
     s->max_streams = max_streams;
