@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -48,6 +49,7 @@ struct ddm_engine {
     ddm::AzScale *d_az = nullptr;
     int *d_qoff = nullptr, *d_qcnt = nullptr, *d_idx = nullptr;
     double *d_norm = nullptr, *d_out = nullptr;
+    double *d_tau = nullptr, *d_fit = nullptr;  // allocated by the first ddm_fit_results
     uint8_t *d_stage[2] = {nullptr, nullptr};
     uint8_t *h_stage[2] = {nullptr, nullptr};
     bool stage_used[2] = {false, false};
@@ -145,8 +147,8 @@ static void release(ddm_engine *e) {
     e->pool.clear();
     // device buffers come from the stream-ordered pool: freeing them does not
     // synchronise the device (other engines keep running)
-    void *dev[] = {e->d_ring, e->d_acc, e->d_lags, e->d_az, e->d_qoff, e->d_qcnt,
-                   e->d_idx,  e->d_norm, e->d_out, e->d_stage[0], e->d_stage[1]};
+    void *dev[] = {e->d_ring, e->d_acc, e->d_lags,     e->d_az,       e->d_qoff,  e->d_qcnt, e->d_idx,
+                   e->d_norm, e->d_out, e->d_stage[0], e->d_stage[1], e->d_tau,   e->d_fit};
     for (void *ptr : dev)
         if (ptr && e->st) (void)hipFreeAsync(ptr, e->st);
     if (e->st) (void)hipStreamSynchronize(e->st);
@@ -435,8 +437,8 @@ extern "C" int ddm_sync(ddm_engine *e) {
     return DDM_OK;
 }
 
-extern "C" int ddm_results(ddm_engine *e, double *results) {
-    if (!e || !results) return DDM_EINVAL;
+// Masked averages of the current window into d_out (asynchronous).
+static int compute_results(ddm_engine *e) {
     const Params &p = e->p;
     const size_t L = p.lags.size();
     int rc = flush_lags(e);
@@ -467,9 +469,63 @@ extern "C" int ddm_results(ddm_engine *e, double *results) {
     HIP_TRY(e, ddm::launch_azimuthal(a, e->st));
     mark_end(e, &az);
     e->stats.azimuthal_launches += 1;
-    HIP_TRY(e, hipMemcpyAsync(results, e->d_out, (size_t)p.result_count * sizeof(double), hipMemcpyDeviceToHost,
+    return DDM_OK;
+}
+
+extern "C" int ddm_results(ddm_engine *e, double *results) {
+    if (!e || !results) return DDM_EINVAL;
+    int rc = compute_results(e);
+    if (rc) return rc;
+    HIP_TRY(e, hipMemcpyAsync(results, e->d_out, (size_t)e->p.result_count * sizeof(double), hipMemcpyDeviceToHost,
                               e->st));
     return ddm_sync(e);
+}
+
+extern "C" int ddm_fit_results(ddm_engine *e, float fps, double *fit, double *D) {
+    if (!e || !(fps > 0.0f)) return DDM_EINVAL;
+    const Params &p = e->p;
+    const int L = (int)p.lags.size(), Q = (int)p.lambdas.size();
+    const long long curves = p.result_count / L;
+    if (L < 3) return fail(e, DDM_EINVAL, "fitting needs at least 3 tau values");
+    int rc = compute_results(e);
+    if (rc) return rc;
+    if (!e->d_fit) {
+        rc = dev_alloc(e, &e->d_tau, (size_t)L, "fit lags");
+        if (rc) return rc;
+        rc = dev_alloc(e, &e->d_fit, (size_t)curves * 4, "fit parameters");
+        if (rc) return rc;
+    }
+    std::vector<double> tau(L);
+    for (int l = 0; l < L; ++l) tau[l] = (double)p.lags[l] / (double)fps;
+    HIP_TRY(e, hipMemcpyAsync(e->d_tau, tau.data(), L * sizeof(double), hipMemcpyHostToDevice, e->st));
+    HIP_TRY(e, ddm::launch_fit(e->d_out, e->d_tau, L, curves, 200, e->d_fit, e->st));
+    std::vector<double> prm((size_t)curves * 4);
+    HIP_TRY(e, hipMemcpyAsync(prm.data(), e->d_fit, prm.size() * sizeof(double), hipMemcpyDeviceToHost, e->st));
+    rc = ddm_sync(e);
+    if (rc) return rc;
+    if (fit) std::copy(prm.begin(), prm.end(), fit);
+    if (D) {
+        // ln F = ln D + 2 ln q with q = 2 pi / lambda, over the curves whose
+        // decay time 1/F lies inside the lag range
+        const double tmin = tau.front(), tmax = tau.back();
+        long long c = 0, t_out = 0;
+        for (const ddm::ScaleGeom &g : p.geom) {
+            for (int t = 0; t < g.tiles; ++t, ++t_out) {
+                double sum = 0.0;
+                int n = 0;
+                for (int q = 0; q < Q; ++q, ++c) {
+                    const double F = prm[(size_t)c * 4], A = prm[(size_t)c * 4 + 1];
+                    if (!(F > 0.0) || !(A > 0.0) || 1.0 / F < tmin || 1.0 / F > tmax || !(p.lambdas[q] > 0.0f))
+                        continue;
+                    const double qv = 2.0 * M_PI / (double)p.lambdas[q];
+                    sum += std::log(F) - 2.0 * std::log(qv);
+                    ++n;
+                }
+                D[t_out] = n ? std::exp(sum / n) : 0.0;
+            }
+        }
+    }
+    return DDM_OK;
 }
 
 extern "C" int ddm_reset(ddm_engine *e) {
@@ -640,4 +696,11 @@ extern "C" int ddm_k_lag_accumulate(const float *ring, int ring_slots, long long
     a.lag_count = lag_count;
     a.lag_block = lag_block;
     return ddm::launch_lag(a, (hipStream_t)stream) == hipSuccess ? DDM_OK : DDM_EDEVICE;
+}
+
+extern "C" int ddm_k_fit(const double *curves, const double *tau, int L, long long n, int iters, double *out,
+                         void *stream) {
+    if (!curves || !tau || !out || L < 3 || n < 0 || iters < 1 || iters > 10000) return DDM_EINVAL;
+    return ddm::launch_fit(curves, tau, L, n, iters, out, (hipStream_t)stream) == hipSuccess ? DDM_OK
+                                                                                           : DDM_EDEVICE;
 }

@@ -61,5 +61,8 @@ hipStream_t engine_stream(ddm_engine *e);
 hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st);
 hipError_t launch_lag(const LagArgs &a, hipStream_t st);
 hipError_t launch_azimuthal(const AzArgs &a, hipStream_t st);
+// Fits A (1 - exp(-F tau)) + B to n curves of L points: out[c][4] = F, A, B, rms.
+hipError_t launch_fit(const double *curves, const double *tau, int L, long long n, int iters, double *out,
+                      hipStream_t st);
 
 }  // namespace ddm

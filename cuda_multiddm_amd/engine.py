@@ -37,7 +37,7 @@ EXPORTED = [
     "ddm_synth_uniform", "ddm_synth_brownian", "ddm_write_iqt_file", "ddm_flush",
     "ddm_server_create", "ddm_server_destroy", "ddm_server_open", "ddm_server_submit", "ddm_server_results",
     "ddm_server_engine", "ddm_server_drain", "ddm_server_close", "ddm_server_latencies", "ddm_server_stats",
-    "ddm_server_last_error",
+    "ddm_server_last_error", "ddm_fit_results", "ddm_k_fit",
 ]
 
 
@@ -109,6 +109,8 @@ def lib():
     L.ddm_submit_device.argtypes = [vp, vp, ctypes.c_int]
     L.ddm_sync.argtypes = [vp]
     L.ddm_results.argtypes = [vp, dp]
+    L.ddm_fit_results.argtypes = [vp, ctypes.c_float, dp, dp]
+    L.ddm_k_fit.argtypes = [vp, vp, ctypes.c_int, ll, ctypes.c_int, vp, vp]
     L.ddm_reset.argtypes = [vp]
     L.ddm_get_info.argtypes = [vp, ctypes.POINTER(Info)]
     L.ddm_get_params.argtypes = [vp, ip, ip, fp]
@@ -257,6 +259,23 @@ class Engine:
             out[s] = flat[off:off + T * Q * L].reshape(T, Q, L)
             off += T * Q * L
         return out
+
+    def fit(self, fps: float = 1.0):
+        """On-device fit of A (1 - exp(-F tau)) + B to every result curve,
+        tau = lag / fps.  Returns (params[scale][tile, q, 4] = F, A, B, rms
+        as a dict keyed by scale, D[scale][tile] as a dict of arrays)."""
+        Q = len(self.lambdas)
+        tiles = [(self.largest_scale // s) ** 2 for s in self.scales]
+        prm = np.zeros((sum(tiles) * Q, 4), dtype=np.float64)
+        dco = np.zeros(sum(tiles), dtype=np.float64)
+        dp = ctypes.POINTER(ctypes.c_double)
+        self._check(lib().ddm_fit_results(self._h, float(fps), prm.ctypes.data_as(dp), dco.ctypes.data_as(dp)))
+        params, D, off = {}, {}, 0
+        for s, T in zip(self.scales, tiles):
+            params[s] = prm[off * Q:(off + T) * Q].reshape(T, Q, 4)
+            D[s] = dco[off:off + T]
+            off += T
+        return params, D
 
     def reset(self):
         self._check(lib().ddm_reset(self._h))
@@ -417,6 +436,15 @@ def k_lag_accumulate(ring, ring_slots, elems_per_frame, first_frame, n_new, lags
                                     ctypes.c_void_p(stream) if stream else None)
     if rc != DDM_OK:
         raise DDMError(rc, "ddm_k_lag_accumulate")
+
+
+def k_fit(curves, tau, out, iters=200, stream=None):
+    """curves: device float64 [n, L]; tau: device float64 [L]; out: device float64 [n, 4]."""
+    n, L = curves.shape
+    rc = lib().ddm_k_fit(ctypes.c_void_p(curves.data_ptr()), ctypes.c_void_p(tau.data_ptr()), L, n, iters,
+                         ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream) if stream else None)
+    if rc != DDM_OK:
+        raise DDMError(rc, "ddm_k_fit")
 
 
 # ---------------------------------------------------------------------------

@@ -142,6 +142,16 @@ int ddm_flush(ddm_engine *eng);
  * src/azimuthal_average.cu:292-346). */
 int ddm_results(ddm_engine *eng, double *results);
 
+/* Fits the Brownian model I(q, tau) = A (1 - exp(-F tau)) + B to every
+ * curve of ddm_results on the device (FP64 Levenberg-Marquardt, one thread
+ * per curve), tau = lag / fps in seconds.  fit[curve][4] = F, A, B, rms,
+ * curve = (scale, tile, q) in result order; D[scale][tile] =
+ * exp(mean(ln F - 2 ln q)) with q = 2 pi / lambda, over the curves with
+ * F > 0, A > 0 and 1/F inside the lag range (0 if none).  Either output
+ * may be NULL.  Replaces fit_ISF and plot_diff_coeff of
+ * resources/python_tools/plot_averaged.py:61-136. */
+int ddm_fit_results(ddm_engine *eng, float fps, double *fit, double *D);
+
 /* Zeroes the lag accumulators and the frame count used for normalisation;
  * the spectrum ring is kept, so pairs reaching back across the reset are
  * still formed.  The -G purge of src/DDM.cu:800-826. */
@@ -242,6 +252,15 @@ int ddm_k_tile_fft(const void *frames, int n_frames, long long frame_bytes,
 int ddm_k_lag_accumulate(const float *ring, int ring_slots, long long elems_per_frame,
                          long long first_frame, int n_new, const int *lags, int lag_count,
                          int max_lag, float *acc, int lag_block, void *stream);
+
+/* ISF fit (the fitting step of ddm_fit_results) on device arrays:
+ * curves[n][L] doubles, tau[L] > 0 ascending, L >= 3, out[n][4] =
+ * F, A, B, rms of A (1 - exp(-F tau)) + B.  iters bounds the
+ * Levenberg-Marquardt iterations (1 .. 10000).  A curve that is all zero
+ * gives all-zero parameters.  fit_func / fit_ISF of
+ * resources/python_tools/plot_averaged.py:61-89. */
+int ddm_k_fit(const double *curves, const double *tau, int L, long long n, int iters, double *out,
+              void *stream);
 
 /* -------------------------------------------------------------------------
  * Host utilities (no device access).
