@@ -681,7 +681,7 @@ extern "C" int ddm_k_lag_accumulate(const float *ring, int ring_slots, long long
     if (!ring || !acc || !lags || ring_slots <= 0 || elems_per_frame <= 0 || first_frame < 0 || lag_count <= 0)
         return DDM_EINVAL;
     if (elems_per_frame % ddm::RING_GROUP != 0) return DDM_EINVAL;
-    if (lag_block != 32 && lag_block != 64 && lag_block != 128) return DDM_EINVAL;
+    if (lag_block != 32 && lag_block != 64 && lag_block != 128 && lag_block != 256) return DDM_EINVAL;
     if (n_new <= 0 || n_new > lag_block) return DDM_EINVAL;
     if (max_lag < 0 || (long long)max_lag + n_new > ring_slots) return DDM_ERANGE;
     ddm::LagArgs a;

@@ -38,8 +38,9 @@ int check_desc(const ddm_desc *d, Params *out, std::string *msg) {
     if (d->chunk_frames <= 0) return fail(DDM_EINVAL, "chunk frame count must be positive");
     if (d->norm_mode != DDM_NORM_FRAMES && d->norm_mode != DDM_NORM_PAIRS)
         return fail(DDM_EINVAL, "unknown normalisation mode");
-    if (d->lag_block != 0 && d->lag_block != 32 && d->lag_block != 64 && d->lag_block != 128)
-        return fail(DDM_EINVAL, "lag block must be 0 (automatic), 32, 64 or 128");
+    if (d->lag_block != 0 && d->lag_block != 32 && d->lag_block != 64 && d->lag_block != 128 &&
+        d->lag_block != 256)
+        return fail(DDM_EINVAL, "lag block must be 0 (automatic), 32, 64, 128 or 256");
 
     p.width = d->width;
     p.height = d->height;

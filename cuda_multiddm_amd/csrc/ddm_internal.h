@@ -23,7 +23,7 @@ constexpr int RING_GROUP = 1 << RING_GROUP_LOG2;
 inline long long pad_elems(long long e) { return (e + RING_GROUP - 1) / RING_GROUP * RING_GROUP; }
 
 // Frames per lag-engine pass when the descriptor leaves it to the engine.
-constexpr int kDefaultLagBlock = 128;
+constexpr int kDefaultLagBlock = 256;
 // Lag values above this are rejected (ring memory would be absurd).
 constexpr int kMaxLag = 1 << 20;
 

@@ -32,7 +32,7 @@ struct LagArgs {
     int dmax;                  // history frames paired with: min(max lag, t0)
     const int *lags;           // device, ascending
     int lag_count;
-    int lag_block;             // 32, 64 or 128
+    int lag_block;             // 32, 64, 128 or 256
 };
 
 struct AzScale {

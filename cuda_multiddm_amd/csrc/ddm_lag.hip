@@ -5,15 +5,14 @@
 // (no 1/s^2 factor: it is a power of two, applied exactly in the average).
 //
 // Mapping: one workgroup per ring group (RING_GROUP = 64 consecutive
-// spectrum elements, one per lane); its 16 waves take the lags in turn
-// (wave w: lags w, w+16, ...).  The block's new frames are staged once in
-// LDS.  For a lag tau, new frame j pairs with frame j - tau: for j < tau that
-// frame is in the history, and the slots of j = jh0 .. min(tau, n)-1 are
-// consecutive in the ring (one wrap at most), so each lag reads its
-// partners as at most two contiguous runs; for j >= tau both frames are in
-// LDS.  Runs are summed eight pairs at a time with two partial sums, which
-// keeps the scalar unit out of the inner loop.  The sum per lag is added to
-// the accumulator once per block, so results do not depend on timing.
+// spectrum elements, one per lane), 16 waves.  The block's new frames are
+// staged once in LDS.  For a lag tau, new frame j pairs with frame j - tau:
+// for j < tau that frame is in the ring history and the partner slots of
+// consecutive j are consecutive (one wrap at most), so a range of j reads
+// its partners as at most two contiguous runs; for j >= tau both frames are
+// in LDS.  The leading run of consecutive lags (1, 2, 3, ... in the usual
+// lag lists) is handled with a register window instead: one new frame and
+// one entering partner per step serve DENSE pairs.
 #include <hip/hip_runtime.h>
 
 #include "ddm_internal.h"
@@ -23,43 +22,150 @@ namespace ddm {
 
 constexpr int LAG_NT = 1024;                    // threads per workgroup: 16 waves
 constexpr int LAG_WAVES = LAG_NT / RING_GROUP;  // waves sharing one ring group
-constexpr int LAG_UNROLL = 8;                   // pairs per step of a run
+constexpr int DENSE_SEG = 32;                   // frames per segment of a lag's units
+
+// Per block size: J = 256 stages 128 KB of new frames, so one workgroup (4
+// waves per SIMD) runs per CU and each wave has room for a wider window,
+// more partial sums and longer load batches; smaller blocks run two
+// workgroups per CU within 64 registers.
+template <int J>
+struct LagCfg {
+    static constexpr bool WIDE = J >= 256;
+    static constexpr int DENSE = WIDE ? 16 : 8;      // lags per register window
+    static constexpr int SLOTS_MAX = WIDE ? 16 : 8;  // partial sums per wave
+    static constexpr int UNROLL = WIDE ? 16 : 8;     // pairs per batch of loads
+    static constexpr int WAVES_PER_EU = WIDE ? 4 : 8;
+};
 
 // sum_{k < count} |a[k*G] - b[k*G]|^2, G = RING_GROUP (consecutive ring slots
-// or consecutive LDS rows are RING_GROUP complex values apart)
+// or consecutive LDS rows are RING_GROUP complex values apart).  Loads come
+// in batches of U pairs; each pair costs two subtractions and two fused
+// multiply-adds into one of four independent partial sums.
+template <int U>
 __device__ __forceinline__ float run_sum(const float2 *a, const float2 *b, int count) {
-    float s0 = 0.f, s1 = 0.f;
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
     int k = 0;
-    for (; k + LAG_UNROLL <= count; k += LAG_UNROLL) {
-        float2 va[LAG_UNROLL], vb[LAG_UNROLL];
+    for (; k + U <= count; k += U) {
+        float2 va[U], vb[U];
 #pragma unroll
-        for (int u = 0; u < LAG_UNROLL; ++u) {
+        for (int u = 0; u < U; ++u) {
             va[u] = a[(k + u) * RING_GROUP];
             vb[u] = b[(k + u) * RING_GROUP];
         }
 #pragma unroll
-        for (int u = 0; u < LAG_UNROLL; ++u) {
+        for (int u = 0; u < U; ++u) {
             const float dx = va[u].x - vb[u].x, dy = va[u].y - vb[u].y;
-            if (u & 1)
-                s1 += dx * dx + dy * dy;
-            else
-                s0 += dx * dx + dy * dy;
+            s[u & 3] = fmaf(dx, dx, s[u & 3]);
+            s[u & 3] = fmaf(dy, dy, s[u & 3]);
         }
     }
     for (; k < count; ++k) {
         const float2 va = a[k * RING_GROUP], vb = b[k * RING_GROUP];
         const float dx = va.x - vb.x, dy = va.y - vb.y;
-        s0 += dx * dx + dy * dy;
+        s[0] = fmaf(dx, dx, s[0]);
+        s[0] = fmaf(dy, dy, s[0]);
     }
-    return s0 + s1;
+    return (s[0] + s[1]) + (s[2] + s[3]);
+}
+
+// Sum over j in [ja, jb) of the valid pairs (t0 + j, t0 + j - tau): the
+// partner is valid when j - tau >= -dmax; it is in the ring history for
+// j < tau and in LDS for j >= tau.
+template <int U>
+__device__ __forceinline__ float lag_range_sum(const float2 *g, const float2 *nl, int R, int slot0, int dmax,
+                                               int tau, int ja, int jb) {
+    float sum = 0.f;
+    const int jh0 = max(tau - dmax, ja), jh1 = min(tau, jb);
+    if (jh0 < jh1) {
+        int jw = tau - slot0;  // j < jw: the partner slot wrapped
+        jw = jw < jh0 ? jh0 : (jw > jh1 ? jh1 : jw);
+        if (jh0 < jw)
+            sum += run_sum<U>(g + (long long)(slot0 + jh0 - tau + R) * RING_GROUP, nl + jh0 * RING_GROUP, jw - jh0);
+        if (jw < jh1)
+            sum += run_sum<U>(g + (long long)(slot0 + jw - tau) * RING_GROUP, nl + jw * RING_GROUP, jh1 - jw);
+    }
+    const int ji = max(tau, ja);
+    if (ji < jb) sum += run_sum<U>(nl + (ji - tau) * RING_GROUP, nl + ji * RING_GROUP, jb - ji);
+    return sum;
+}
+
+// Frame t0 + r of the block's window: LDS for r >= 0, ring history for r < 0.
+// Frames of valid pairs are at most max lag <= R - n back, one wrap away;
+// older ones (read only for discarded lags) are clamped to slot 0 so the
+// load stays inside the ring.
+__device__ __forceinline__ float2 window_frame(const float2 *g, const float2 *nl, int R, int slot0, int r) {
+    float2 v;
+    if (r >= 0) {
+        v = nl[r * RING_GROUP];
+    } else {
+        int s = slot0 + r;
+        s += (s < 0) ? R : 0;
+        s = s < 0 ? 0 : s;
+        v = g[(long long)s * RING_GROUP];
+    }
+    return v;
+}
+
+// D consecutive lags tau_a .. tau_a + D - 1 over new frames j in [j0, j1).
+// The partners of frame j are the D frames j - tau_a - i, a sliding window
+// kept in registers: each step loads one new frame and one entering partner
+// and forms D pairs.  Window slot (u - i) & (D - 1) holds the partner of
+// step u for lag tau_a + i, so every register index is a compile-time
+// constant.  Sums of lags beyond the caller's count come from in-bounds
+// (possibly stale) slots and are discarded by the caller.
+template <int D, int S>
+__device__ __forceinline__ void dense_sums(const float2 *g, const float2 *nl, int R, int slot0, int tau_a, int j0,
+                                           int j1, float (&s)[S]) {
+    static_assert(D <= S && (D & (D - 1)) == 0, "window must be a power of two within the partial sums");
+    float2 win[D];
+#pragma unroll
+    for (int k = 1; k < D; ++k) win[k] = window_frame(g, nl, R, slot0, j0 - tau_a - (D - k));
+#pragma unroll
+    for (int i = 0; i < D; ++i) s[i] = 0.f;
+    for (int jb = j0; jb < j1; jb += D) {
+        const int cnt = j1 - jb;
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+            if (u < cnt) {
+                win[u] = window_frame(g, nl, R, slot0, jb + u - tau_a);
+                const float2 a = nl[(jb + u) * RING_GROUP];
+#pragma unroll
+                for (int i = 0; i < D; ++i) {
+                    const float2 b = win[(u - i) & (D - 1)];
+                    const float dx = a.x - b.x, dy = a.y - b.y;
+                    s[i] = fmaf(dx, dx, s[i]);
+                    s[i] = fmaf(dy, dy, s[i]);
+                }
+            }
+        }
+    }
 }
 
 // E: ring / accumulator row stride (padded), R: ring slots, slot0: slot of
 // t0, dmax = min(max lag, t0): oldest history frame the block may pair with.
+//
+// Work split over the 16 waves (identical in every workgroup):
+//  * dense part (J >= 64): the leading run of consecutive lags
+//    (lags[i] = lags[0] + i, DENSE/2 to 8 * DENSE of them, at most 64) in
+//    chunks of DENSE lags; each chunk is cut into sd segments of new frames,
+//    one wave per (chunk, segment);
+//  * the other ls lags on the remaining ws waves: each lag is cut into ss
+//    segments ("units"); unit u goes to wave u % ws as its partial slot
+//    u / ws.  With more lags than partial slots each lag is instead owned by
+//    one wave that adds its sum to the accumulator directly.
+// Dense waves are weighted by their cheaper pairs (one load per DENSE pairs).
+// Partial sums meet in LDS after the pair work and are added per lag in a
+// fixed order, so results do not depend on timing.
 template <int J>
-__global__ __launch_bounds__(LAG_NT) void lag_kernel(const float2 *__restrict__ ring, float *__restrict__ acc,
-                                                    long long E, int R, int slot0, int n, int dmax,
-                                                    const int *__restrict__ lags, int L) {
+__global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J>::WAVES_PER_EU))) void lag_kernel(
+    const float2 *__restrict__ ring, float *__restrict__ acc, long long E, int R, int slot0, int n, int dmax,
+    const int *__restrict__ lags, int L) {
+    using C = LagCfg<J>;
+    constexpr int DENSE = C::DENSE, U = C::UNROLL;
+    constexpr int NSEG = J / DENSE_SEG;  // most segments per lag
+    constexpr int SLOTS = (J / 8) < C::SLOTS_MAX ? (J / 8) : C::SLOTS_MAX;  // partials per wave that fit in LDS
+    constexpr int MAX_DENSE = (8 * DENSE) < RING_GROUP ? (8 * DENSE) : RING_GROUP;
+    static_assert(LAG_WAVES * SLOTS * RING_GROUP <= J * RING_GROUP * 2, "partials exceed the staging buffer");
     __shared__ float2 nb[J * RING_GROUP];
     const int tid = threadIdx.x;
     const int lane = tid % RING_GROUP;
@@ -84,26 +190,98 @@ __global__ __launch_bounds__(LAG_NT) void lag_kernel(const float2 *__restrict__ 
             if (j < n) nb[j * RING_GROUP + lane] = v[k];
         }
     }
+    // ---- plan (wave-uniform) ----
+    int run;  // length of the leading run of consecutive lags (<= 64)
+    {
+        const int first = lags[0];
+        const bool in_run = lane < L && lags[lane < L ? lane : 0] == first + lane;
+        const unsigned long long miss = ~__ballot(in_run);
+        run = miss ? __builtin_ctzll(miss) : RING_GROUP;
+        run = __builtin_amdgcn_readfirstlane(run);
+    }
+    const int dense_lags = (SLOTS >= DENSE && run >= DENSE / 2) ? min(run, MAX_DENSE) : 0;
+    const int nchunks = (dense_lags + DENSE - 1) / DENSE;
+    const int ls = L - dense_lags;
+    int wd = 0;  // dense waves
+    if (nchunks) {
+        // a dense pair costs about 3/5 of a pair of the other lags
+        const int wdense = 3 * dense_lags, wother = 5 * ls;
+        wd = (LAG_WAVES * wdense + (wdense + wother) / 2) / (wdense + wother);
+        wd = max(wd, nchunks);
+        wd = min(wd, ls ? LAG_WAVES - 1 : LAG_WAVES);
+        wd -= wd % nchunks;
+    }
+    const int sd = nchunks ? wd / nchunks : 1;
+    const int ws = LAG_WAVES - wd;
+    int ss = 0;  // segments per other lag; 0: one wave per lag, direct accumulation
+    for (int cand = NSEG; cand >= 1; cand /= 2)
+        if (ls * cand <= SLOTS * ws) {
+            ss = cand;
+            break;
+        }
     __syncthreads();
     const float2 *nl = nb + lane;
     float *accg = acc + (long long)blockIdx.x * RING_GROUP + lane;
-    for (int l = w; l < L; l += LAG_WAVES) {
-        const int tau = lags[l];
-        float sum = 0.f;
-        // history partners: j in [jh0, jh1), partner slot slot0 + j - tau (+R if negative)
-        const int jh0 = (tau - dmax) > 0 ? (tau - dmax) : 0;
-        const int jh1 = tau < n ? tau : n;
-        if (jh0 < jh1) {
-            int jw = tau - slot0;  // j < jw: the partner slot wrapped
-            jw = jw < jh0 ? jh0 : (jw > jh1 ? jh1 : jw);
-            if (jh0 < jw)
-                sum += run_sum(g + (long long)(slot0 + jh0 - tau + R) * RING_GROUP, nl + jh0 * RING_GROUP, jw - jh0);
-            if (jw < jh1)
-                sum += run_sum(g + (long long)(slot0 + jw - tau) * RING_GROUP, nl + jw * RING_GROUP, jh1 - jw);
+    float s[C::SLOTS_MAX];
+#pragma unroll
+    for (int i = 0; i < C::SLOTS_MAX; ++i) s[i] = 0.f;
+    if (w < wd) {
+        const int c = w / sd, seg = w % sd;
+        const int la = c * DENSE, kc = min(DENSE, dense_lags - la);
+        const int tau_a = lags[la];
+        const int len = (n + sd - 1) / sd;
+        const int j0 = seg * len, j1 = min(j0 + len, n);
+        if (j0 < j1) {
+            if (j0 - (tau_a + kc - 1) >= -dmax) {
+                dense_sums<DENSE>(g, nl, R, slot0, tau_a, j0, j1, s);
+            } else {
+                // near the start of the stream some partners do not exist yet
+#pragma unroll
+                for (int i = 0; i < DENSE; ++i)
+                    if (i < kc) s[i] = lag_range_sum<U>(g, nl, R, slot0, dmax, tau_a + i, j0, j1);
+            }
         }
-        // partners inside the block: j in [tau, n) pairs with j - tau
-        if (tau < n) sum += run_sum(nl, nl + tau * RING_GROUP, n - tau);
-        accg[(long long)l * E] += sum;
+    } else if (ss) {
+        const int k = w - wd, units = ls * ss, len = (n + ss - 1) / ss;
+#pragma unroll 1
+        for (int m = 0; m < SLOTS; ++m) {
+            const int u = k + m * ws;
+            if (u < units) {
+                const int l = dense_lags + u / ss, seg = u % ss;
+                const int j0 = seg * len, j1 = min(j0 + len, n);
+                if (j0 < j1) s[m] = lag_range_sum<U>(g, nl, R, slot0, dmax, lags[l], j0, j1);
+            }
+        }
+    } else {
+        for (int l = dense_lags + (w - wd); l < L; l += ws)
+            accg[(long long)l * E] += lag_range_sum<U>(g, nl, R, slot0, dmax, lags[l], 0, n);
+    }
+    if (wd == 0 && ss == 0) return;
+    // ---- combine the partials: [wave][slot][lane] in the staging buffer ----
+    __syncthreads();
+    float *red = reinterpret_cast<float *>(nb);
+#pragma unroll
+    for (int i = 0; i < SLOTS; ++i) red[(w * SLOTS + i) * RING_GROUP + lane] = s[i];
+    __syncthreads();
+    // one lag per wave and step, so every index below is wave-uniform
+    const int nred = ss ? L : dense_lags;
+    for (int l = w; l < nred; l += LAG_WAVES) {
+        float t = 0.f;
+        if (l < dense_lags) {
+            const int c = l / DENSE, i = l % DENSE;
+            for (int seg = 0; seg < sd; ++seg) t += red[((c * sd + seg) * SLOTS + i) * RING_GROUP + lane];
+        } else {
+            const int u0 = (l - dense_lags) * ss;
+            int q = u0 / ws, r = u0 - q * ws;  // unit u -> wave wd + u % ws, slot u / ws
+            for (int seg = 0; seg < ss; ++seg) {
+                t += red[((wd + r) * SLOTS + q) * RING_GROUP + lane];
+                if (++r == ws) {
+                    r = 0;
+                    ++q;
+                }
+            }
+        }
+        accg[(long long)l * E] += t;
     }
 }
 
@@ -122,6 +300,7 @@ hipError_t launch_lag(const LagArgs &a, hipStream_t st) {
     case 32: launch_lag_t<32>(a, st); break;
     case 64: launch_lag_t<64>(a, st); break;
     case 128: launch_lag_t<128>(a, st); break;
+    case 256: launch_lag_t<256>(a, st); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -77,7 +77,7 @@ typedef struct ddm_desc {
     float mask_tolerance; /* ring [q, tol * q] (-t: 1 + percent/100) */
     int chunk_frames;     /* frames per upload / FFT batch (-C) */
     int norm_mode;        /* enum ddm_norm_mode */
-    int lag_block;        /* frames per lag-engine pass: 0 = automatic (128), else 32, 64 or 128 */
+    int lag_block;        /* frames per lag-engine pass: 0 = automatic (256), else 32, 64, 128 or 256 */
 } ddm_desc;
 
 /* Sizes the engine derived from a descriptor (after sorting). */
@@ -248,7 +248,7 @@ int ddm_k_tile_fft(const void *frames, int n_frames, long long frame_bytes,
  * over those t with t - lag >= 0.  No 1/s^2 factor is applied.  lags is a
  * device array of lag_count ints, all <= max_lag; the ring must hold frames
  * first_frame - max_lag .. first_frame + n_new - 1 (max_lag + n_new <=
- * ring_slots).  n_new <= lag_block (32, 64 or 128). */
+ * ring_slots).  n_new <= lag_block (32, 64, 128 or 256). */
 int ddm_k_lag_accumulate(const float *ring, int ring_slots, long long elems_per_frame,
                          long long first_frame, int n_new, const int *lags, int lag_count,
                          int max_lag, float *acc, int lag_block, void *stream);

@@ -83,7 +83,7 @@ def test_host_and_device_submission_identical(gpu):
     assert np.array_equal(a, b)
     c, _ = _both(frames, [64, 16], [1, 2, 7, 31], [2.0, 5.0], 1.2, 30, device_input=True)
     assert np.array_equal(b, c)  # deterministic
-    for chunk, lb in [(7, 32), (30, 128), (45, 64)]:  # other chunk / lag-block splits
+    for chunk, lb in [(7, 32), (30, 128), (45, 64), (9, 256)]:  # other chunk / lag-block splits
         d, _ = _both(frames, [64, 16], [1, 2, 7, 31], [2.0, 5.0], 1.2, chunk, lag_block=lb)
         assert o.max_rel_err(d, ref) <= TOL
 

@@ -136,7 +136,7 @@ def _lag_expected(ring, R, t0, n_new, lags):
     return acc
 
 
-@pytest.mark.parametrize("J", [32, 64, 128])
+@pytest.mark.parametrize("J", [32, 64, 128, 256])
 @pytest.mark.parametrize("t0,n_new", [(0, None), (3, 7), (100, None), (57, 1), (200, 29)])
 @pytest.mark.parametrize("nlags", [10, 40, 70])
 def test_lag_engine_bit_exact(gpu, J, t0, n_new, nlags):
@@ -158,6 +158,45 @@ def test_lag_engine_bit_exact(gpu, J, t0, n_new, nlags):
     acc = torch.from_numpy(acc0.astype(np.float32).reshape(-1)).to(gpu)
     d_lags = torch.tensor(lags, dtype=torch.int32, device=gpu)
     print(f"lag J={J} t0={t0} n_new={n_new} L={len(lags)}", flush=True)
+    ddm.k_lag_accumulate(ring, R, E, t0, n_new, d_lags, max(lags), acc, J)
+    torch.cuda.synchronize()
+    got = acc.cpu().numpy().reshape(len(lags), E).astype(np.int64)
+    want = acc0 + _lag_expected(ring_np, R, t0, n_new, lags)
+    assert np.array_equal(got, want)
+
+
+DENSE_LAG_SETS = {
+    # leading runs of consecutive lags take the register-window path
+    "log32": list(range(1, 15)) + [17, 20, 25, 31, 37, 46, 56, 68, 84, 102, 125, 153, 187, 229],
+    "run70": list(range(1, 71)) + [90, 300],  # run capped at 64 (32 for J = 256), rest single
+    "from5": list(range(5, 30)) + [40, 41],   # run not starting at 1
+    "zero4": [0, 1, 2, 3, 9],                 # shortest run that is dense, lag 0 included
+    "short3": [1, 2, 3, 50],                  # run too short: all single
+    "run16": list(range(2, 18)),               # exactly one full chunk
+}
+
+
+@pytest.mark.parametrize("J", [32, 64, 128, 256])
+@pytest.mark.parametrize("t0,n_new", [(0, None), (3, 7), (20, None), (57, 1), (300, 29), (500, None)])
+@pytest.mark.parametrize("name", sorted(DENSE_LAG_SETS))
+def test_lag_engine_dense_runs_bit_exact(gpu, J, t0, n_new, name):
+    """Integer spectra, lag sets with runs of consecutive lags; the ring is
+    only max lag + n_new slots when that is legal, so stale slots sit right
+    behind the valid history."""
+    import torch
+    lags = DENSE_LAG_SETS[name]
+    n_new = J if n_new is None else n_new
+    R = max(lags) + n_new + (0 if t0 % 2 else 5)
+    E = 3 * G
+    rng = np.random.default_rng(t0 * 11 + J + n_new + len(lags))
+    re = rng.integers(-8, 9, (R, E))
+    im = rng.integers(-8, 9, (R, E))
+    ring_np = re + 1j * im
+    ring = torch.from_numpy(frames_to_ring(re, im)).to(gpu)
+    acc0 = rng.integers(0, 1000, (len(lags), E))
+    acc = torch.from_numpy(acc0.astype(np.float32).reshape(-1)).to(gpu)
+    d_lags = torch.tensor(lags, dtype=torch.int32, device=gpu)
+    print(f"dense lag J={J} t0={t0} n_new={n_new} set={name} R={R}", flush=True)
     ddm.k_lag_accumulate(ring, R, E, t0, n_new, d_lags, max(lags), acc, J)
     torch.cuda.synchronize()
     got = acc.cpu().numpy().reshape(len(lags), E).astype(np.int64)

@@ -39,15 +39,15 @@ def _v(**kw):
 def test_validate_accepts_config1_and_sorts():
     rc, msg, info = _v()
     assert rc == 0 and msg == ""
-    # ring: one chunk + one lag block (128 by default) + the largest lag
-    assert info.largest_scale == 128 and info.lag_block == 128 and info.ring_slots == 30 + 128 + 16
+    # ring: one chunk + one lag block (256 by default) + the largest lag
+    assert info.largest_scale == 128 and info.lag_block == 256 and info.ring_slots == 30 + 256 + 16
     assert info.elems_per_frame == 128 * 65 and info.elems_per_frame_padded == 8320
     assert info.result_count == 20 * 16
     rc, _, info = _v(width=1100, height=1030, scales=[256, 1024, 512], lags=[30, 1, 2],
                      lambdas=[16.0, 1.0, 3.0])
     assert rc == 0 and info.largest_scale == 1024
     assert info.result_count == (1 + 4 + 16) * 3 * 3
-    assert info.ring_slots == 30 + 128 + 30
+    assert info.ring_slots == 30 + 256 + 30
     rc, _, info = _v(lag_block=128, chunk=7)
     assert rc == 0 and info.lag_block == 128 and info.ring_slots == 7 + 128 + 16
 
@@ -62,7 +62,7 @@ def test_validate_accepts_config1_and_sorts():
     (dict(x_off=1), -4, "the specified out dimensions must be smaller than actual image size"),
     (dict(width=127), -4, "the specified out dimensions must be smaller than actual image size"),
     (dict(chunk=0), -1, "chunk frame count must be positive"),
-    (dict(lag_block=16), -1, "lag block must be 0 (automatic), 32, 64 or 128"),
+    (dict(lag_block=16), -1, "lag block must be 0 (automatic), 32, 64, 128 or 256"),
     (dict(scales=[4096], width=4096, height=4096), -5, "scales above 2048 are not supported"),
 ])
 def test_validate_rejects(kw, code, text):
@@ -73,7 +73,7 @@ def test_validate_rejects(kw, code, text):
 def test_validate_lifts_reference_hard_limits_with_warnings():
     # tau > chunk (src/DDM.cu:462) is allowed: the ring grows to chunk + max tau
     rc, msg, info = _v(lags=[1, 100])
-    assert rc == 0 and info.ring_slots == 30 + 128 + 100
+    assert rc == 0 and info.ring_slots == 30 + 256 + 100
     # lambda > smallest scale (src/DDM.cu:349) is a warning: the ring is empty
     rc, msg, _ = _v(lambdas=[4.0, 200.0])
     assert rc == 0 and msg == "The largest q-vector should be smaller than the smallest scale."
