@@ -1,0 +1,71 @@
+// Device helpers shared by the tile / FFT kernels: complex arithmetic, pixel
+// addressing, and the blocked spectrum-ring layout.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ddm_amd.h"
+#include "ddm_internal.h"
+
+namespace ddm {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+struct PixSrc {
+    const uint8_t *frames;
+    long long frame_bytes;
+    long long row_bytes;
+    int bpp, fmt, x_off, y_off;
+};
+
+__device__ __forceinline__ float load_px(const uint8_t *fr, const PixSrc &ps, int x, int y) {
+    const uint8_t *p = fr + (long long)y * ps.row_bytes + (long long)x * ps.bpp;
+    if (ps.fmt == DDM_PIX_U8) return (float)p[0];
+    const unsigned b0 = p[0], b1 = p[1];
+    return (float)(ps.fmt == DDM_PIX_U16LE ? (b0 | (b1 << 8)) : ((b0 << 8) | b1));
+}
+
+// Ring layout: the spectrum elements of a frame are cut into groups of
+// RING_GROUP consecutive elements, and each group keeps all ring slots
+// together: element e of slot s lives at ((e / RING_GROUP) * R + s) *
+// RING_GROUP + e % RING_GROUP.  A lag-engine workgroup (one group) then
+// walks the frame history as one contiguous stream.
+__device__ __forceinline__ long long ring_index(long long e, int slot, int R) {
+    return ((e >> RING_GROUP_LOG2) * R + slot) * RING_GROUP + (e & (RING_GROUP - 1));
+}
+
+struct RingDst {
+    float2 *ring;
+    long long elem_off;  // of this scale
+    int ring_slots;
+    int first_slot;      // slot of frame 0 of this launch
+};
+
+// Writes element `local` of tile `tile` of batch frame f.
+struct TileOut {
+    float2 *ring;
+    long long base;  // element index of the tile's first element
+    int slot, R;
+    __device__ __forceinline__ float2 &operator[](long long local) const {
+        return ring[ring_index(base + local, slot, R)];
+    }
+};
+
+__device__ __forceinline__ TileOut tile_out(const RingDst &rd, int f, int tile, long long tile_elems) {
+    int slot = rd.first_slot + f;
+    slot -= (slot >= rd.ring_slots) ? rd.ring_slots : 0;  // f < ring_slots
+    return TileOut{rd.ring, rd.elem_off + (long long)tile * tile_elems, slot, rd.ring_slots};
+}
+
+// Register-resident tile FFT (ddm_fft.hip) for the scales it covers;
+// returns false for other scales.
+bool launch_tile_fft_reg(const PixSrc &ps, const RingDst &rd, int scale, int tps, int tiles, int n_frames,
+                         const float2 *tw, hipStream_t st);
+
+}  // namespace ddm

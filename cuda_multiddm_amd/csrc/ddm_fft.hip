@@ -1,0 +1,363 @@
+// Register-resident 2-D R2C FFT of 64 x 64 and 128 x 128 tiles (tile cut +
+// FFT, the parseBufferScalePow2 + cufftExecR2C step of the reference,
+// src/DDM_kernel.cuh:43-78 and src/DDM.cu:188-192).
+//
+// Same packing as the LDS kernels of ddm_kernels.hip: the real S x S tile is
+// read as H = S/2 complex rows z_r = x_{2r} + i x_{2r+1}; after the row FFTs
+// the two real rows are separated, X_a = (Z_k + conj Z_{S-k}) / 2 and
+// X_b = -i (Z_k - conj Z_{S-k}) / 2, and columns kx = 0 and kx = H travel
+// packed as one complex column (DC + i Nyquist), leaving H complex columns of
+// length S that are separated again when the result is written.
+//
+// Every 1-D FFT of length S = 16 L runs on L lanes in two register phases:
+//   phase 1: lane q holds x[16 elements at stride L, offset q], a radix-16
+//            DFT in registers, then the twiddles W_S^(q k1);
+//   exchange: the 16 x L values of the sequence go through LDS (a region
+//            private to the wave, so no workgroup barrier);
+//   phase 2: each lane takes the 16 / L values of k1 closed under
+//            k1 -> 16 - k1 and runs length-L DFTs, so the spectrum values at
+//            k and S - k, which the real-pair separation combines, are in the
+//            same lane.
+// The row pass writes the separated half spectra into the tile buffer in
+// LDS (one transpose), the column pass reads them back and writes the
+// spectrum straight to the ring.  LDS addresses are swizzled so that every
+// group of 16 lanes touches 16 distinct 8-byte bank slots in each access.
+#include <hip/hip_runtime.h>
+
+#include "ddm_device.h"
+#include "ddm_kernels.h"
+
+namespace ddm {
+
+namespace {
+
+constexpr int REG_NTH = 512;  // 8 waves
+constexpr int REG_WAVE_SLOTS = 1024;  // complex values of LDS per wave
+
+constexpr float C16 = 0.92387953251128674f;  // cos(pi/8)
+constexpr float S16 = 0.38268343236508977f;  // sin(pi/8)
+constexpr float R2 = 0.70710678118654752f;   // sqrt(2)/2
+
+__device__ __forceinline__ float2 mulw(float2 v, float c, float s) {  // v * (c + i s)
+    return make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
+}
+__device__ __forceinline__ float2 mul_mi(float2 v) { return make_float2(v.y, -v.x); }  // v * (-i)
+
+// In-place DFT of four values, natural order.
+__device__ __forceinline__ void dft4(float2 &a, float2 &b, float2 &c, float2 &d) {
+    const float2 s02 = cadd(a, c), d02 = csub(a, c), s13 = cadd(b, d), d13 = csub(b, d);
+    a = cadd(s02, s13);
+    c = csub(s02, s13);
+    b = make_float2(d02.x + d13.y, d02.y - d13.x);  // d02 - i d13
+    d = make_float2(d02.x - d13.y, d02.y + d13.x);  // d02 + i d13
+}
+
+// X[k] = sum_n v[n] exp(-2 pi i n k / 16), natural order in and out.
+__device__ __forceinline__ void dft16(float2 (&v)[16]) {
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) dft4(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
+    // v[n2 + 4 k1] *= W16^(n2 k1)
+    v[5] = mulw(v[5], C16, -S16);
+    v[9] = mulw(v[9], R2, -R2);
+    v[13] = mulw(v[13], S16, -C16);
+    v[6] = mulw(v[6], R2, -R2);
+    v[10] = mul_mi(v[10]);
+    v[14] = mulw(v[14], -R2, -R2);
+    v[7] = mulw(v[7], S16, -C16);
+    v[11] = mulw(v[11], -R2, -R2);
+    v[15] = mulw(v[15], -C16, S16);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) dft4(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
+    // v[4 k1 + k2] = X[k1 + 4 k2]
+    float2 t[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) t[k] = v[4 * (k % 4) + k / 4];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = t[k];
+}
+
+__device__ __forceinline__ void dft8(float2 (&u)[8]) {
+    float2 e0 = u[0], e1 = u[2], e2 = u[4], e3 = u[6];
+    float2 o0 = u[1], o1 = u[3], o2 = u[5], o3 = u[7];
+    dft4(e0, e1, e2, e3);
+    dft4(o0, o1, o2, o3);
+    o1 = mulw(o1, R2, -R2);
+    o2 = mul_mi(o2);
+    o3 = mulw(o3, -R2, -R2);
+    u[0] = cadd(e0, o0);
+    u[4] = csub(e0, o0);
+    u[1] = cadd(e1, o1);
+    u[5] = csub(e1, o1);
+    u[2] = cadd(e2, o2);
+    u[6] = csub(e2, o2);
+    u[3] = cadd(e3, o3);
+    u[7] = csub(e3, o3);
+}
+
+template <int L>
+__device__ __forceinline__ void dftL(float2 (&u)[L]);
+template <>
+__device__ __forceinline__ void dftL<8>(float2 (&u)[8]) {
+    dft8(u);
+}
+template <>
+__device__ __forceinline__ void dftL<4>(float2 (&u)[4]) {
+    dft4(u[0], u[1], u[2], u[3]);
+}
+
+template <int S>
+struct RegCfg {
+    static constexpr int H = S / 2;
+    static constexpr int L = S / 16;          // lanes per sequence
+    static constexpr int P = 16 / L;          // k1 values per lane in phase 2
+    static constexpr int Q = 64 / L;          // sequences per wave
+    static constexpr int WPT = H / Q;         // waves per tile
+    static constexpr int TPW = (REG_NTH / 64) / WPT;  // tiles per workgroup
+    static constexpr int TILE = S * H;        // complex values per tile in LDS
+    static_assert(WPT * REG_WAVE_SLOTS == TILE, "a wave's rows are its exchange region");
+};
+
+// The k1 values lane j of a sequence takes in phase 2 (p = 0 .. P-1); each
+// set is closed under k1 -> (16 - k1) mod 16.
+template <int L>
+__device__ __forceinline__ int kappa(int j, int p) {
+    if (L == 8) return p == 0 ? j : (j == 0 ? 8 : 16 - j);
+    // L == 4: {j, 16 - j, 8 - j, 8 + j}, lane 0: {0, 8, 4, 12}
+    switch (p) {
+    case 0: return j;
+    case 1: return j == 0 ? 8 : 16 - j;
+    case 2: return 8 - j - (j == 0 ? 4 : 0);
+    default: return 8 + j + (j == 0 ? 4 : 0);
+    }
+}
+
+// Exchange slot of value (sequence sq, k1, n2) inside a wave's region.
+template <int L>
+__device__ __forceinline__ int xslot(int sq, int k1, int n2) {
+    constexpr int P = 16 / L;
+    return sq * 16 * L + (k1 ^ (sq & (P - 1))) * L + ((n2 + k1) & (L - 1));
+}
+
+// Column swizzle of tile row y.
+__device__ __forceinline__ int tswz(int y) { return ((y & 1) * 6) ^ (((y >> 1) & 1) * 8) ^ (((y >> 2) & 1) * 4); }
+
+// Register of the partner value Z[S - k] for the value at (p, k2): returns
+// (p', k2') for lanes j != 0 in (*pa, *ka) and for lane 0 in (*pb, *kb).
+template <int L>
+__device__ __forceinline__ void partner(int p, int k2, int *pa, int *ka, int *pb, int *kb) {
+    *ka = L - 1 - k2;
+    *pa = (L == 8) ? (1 - p) : (p ^ 1);
+    if (p == 0) {
+        *pb = 0;
+        *kb = (L - k2) & (L - 1);
+    } else if (L == 8 || p == 1) {
+        *pb = p;
+        *kb = L - 1 - k2;
+    } else {
+        *pb = p ^ 1;  // L == 4: 4 <-> 12
+        *kb = L - 1 - k2;
+    }
+}
+
+// zp[p][k2] = the value of z at S - k for k = kappa(q, p) + 16 k2.  Lane 0
+// of a sequence pairs its registers differently from the others, so the two
+// cases are separate branches (a select between array elements would send
+// z to scratch memory).
+template <int L, int P>
+__device__ __forceinline__ void partners(const float2 (&z)[P][L], bool lane0, float2 (&zp)[P][L]) {
+    if (lane0) {
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int k2 = 0; k2 < L; ++k2) {
+                int pa, ka, pb, kb;
+                partner<L>(p, k2, &pa, &ka, &pb, &kb);
+                zp[p][k2] = z[pb][kb];
+            }
+    } else {
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int k2 = 0; k2 < L; ++k2) {
+                int pa, ka, pb, kb;
+                partner<L>(p, k2, &pa, &ka, &pb, &kb);
+                zp[p][k2] = z[pa][ka];
+            }
+    }
+}
+
+}  // namespace
+
+// U8: pixels are single bytes (bpp 1, DDM_PIX_U8), loaded with constant
+// offsets; otherwise every pixel goes through load_px.
+template <int S, bool U8>
+__global__ __launch_bounds__(REG_NTH) void tile_fft_reg(PixSrc ps, RingDst rd, int tps, int tiles, int groups,
+                                                       const float2 *__restrict__ twg) {
+    using C = RegCfg<S>;
+    constexpr int H = C::H, L = C::L, P = C::P, Q = C::Q, WPT = C::WPT, TPW = C::TPW, TILE = C::TILE;
+    __shared__ float2 lds[TPW * TILE];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    // frames vary fastest across the grid: concurrent workgroups write
+    // neighbouring ring slots of the same element groups
+    const int nf = gridDim.x / groups;
+    const int f = blockIdx.x % nf;
+    const int t = w / WPT, wit = w % WPT;
+    const int tile = (blockIdx.x / nf) * TPW + t;
+    const bool valid = tile < tiles;
+    const int q = lane % L, sq = lane / L;  // lane within the sequence, sequence within the wave
+    float2 *reg = lds + w * REG_WAVE_SLOTS;  // this wave's region
+    float2 *tb = lds + t * TILE;             // this wave's tile
+
+    // twiddles W_S^(q k1), k1 = 1..15 (the same in both passes)
+    float2 tw[16];
+#pragma unroll
+    for (int k1 = 1; k1 < 16; ++k1) tw[k1] = twg[S + __mul24(q, k1)];
+
+    float2 v[16];
+    // ---------------- rows: sequence r = row pair ----------------
+    {
+        const int r = wit * Q + sq;
+        if (valid) {
+            const uint8_t *fr = ps.frames + (long long)f * ps.frame_bytes;
+            const int x0 = ps.x_off + (tile % tps) * S + q;
+            const int y0 = ps.y_off + (tile / tps) * S + 2 * r;
+            if constexpr (U8) {
+                const uint8_t *r0 = fr + (long long)y0 * ps.row_bytes + x0;
+                const uint8_t *r1 = r0 + ps.row_bytes;
+#pragma unroll
+                for (int n1 = 0; n1 < 16; ++n1) v[n1] = make_float2((float)r0[n1 * L], (float)r1[n1 * L]);
+            } else {
+#pragma unroll
+                for (int n1 = 0; n1 < 16; ++n1)
+                    v[n1] = make_float2(load_px(fr, ps, x0 + n1 * L, y0), load_px(fr, ps, x0 + n1 * L, y0 + 1));
+            }
+        } else {
+#pragma unroll
+            for (int n1 = 0; n1 < 16; ++n1) v[n1] = make_float2(0.f, 0.f);
+        }
+        dft16(v);
+#pragma unroll
+        for (int k1 = 1; k1 < 16; ++k1) v[k1] = cmul(v[k1], tw[k1]);
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) reg[xslot<L>(sq, k1, q)] = v[k1];
+        // phase 2: lane j = q takes k1 in kappa(j, .)
+        float2 z[P][L];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const int k1 = kappa<L>(q, p);
+#pragma unroll
+            for (int n2 = 0; n2 < L; ++n2) z[p][n2] = reg[xslot<L>(sq, k1, n2)];
+            dftL<L>(z[p]);  // z[p][k2] = Z[k1 + 16 k2]
+        }
+        // separate the real row pair; write rows 2r, 2r+1, columns k < H
+        const int ya = 2 * r, yb = 2 * r + 1;
+        const int sa = tswz(ya), sb = tswz(yb);
+        float2 zq[P][L];
+        partners<L, P>(z, q == 0, zq);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const int k1 = kappa<L>(q, p);
+#pragma unroll
+            for (int k2 = 0; k2 < H / 16; ++k2) {
+                const float2 zk = z[p][k2];
+                const float2 zp = zq[p][k2];
+                float2 xa = make_float2(0.5f * (zk.x + zp.x), 0.5f * (zk.y - zp.y));
+                float2 xb = make_float2(0.5f * (zk.y + zp.y), -0.5f * (zk.x - zp.x));
+                if (p == 0 && k2 == 0 && q == 0) {
+                    const float2 zh = z[0][H / 16];  // Z[H]: DC and Nyquist of both rows are real
+                    xa = make_float2(zk.x, zh.x);
+                    xb = make_float2(zk.y, zh.y);
+                }
+                const int c = k1 + 16 * k2;
+                tb[ya * H + (c ^ sa)] = xa;
+                tb[yb * H + (c ^ sb)] = xb;
+            }
+        }
+    }
+    __syncthreads();
+    // ---------------- columns: sequence c = packed column ----------------
+    const int c = wit * Q + sq;
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) {
+        const int y = n1 * L + q;
+        v[n1] = tb[y * H + (c ^ tswz(y))];
+    }
+    dft16(v);
+#pragma unroll
+    for (int k1 = 1; k1 < 16; ++k1) v[k1] = cmul(v[k1], tw[k1]);
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) reg[xslot<L>(sq, k1, q)] = v[k1];
+    float2 z[P][L];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const int k1 = kappa<L>(q, p);
+#pragma unroll
+        for (int n2 = 0; n2 < L; ++n2) z[p][n2] = reg[xslot<L>(sq, k1, n2)];
+        dftL<L>(z[p]);  // z[p][k2] = V[k1 + 16 k2], ky = k1 + 16 k2
+    }
+    if (!valid) return;
+    const TileOut out = tile_out(rd, f, tile, (long long)S * (H + 1));
+    if (c != 0) {
+        // rows ky = k1 + 16 k2 are D elements apart; walk the blocked ring
+        // address instead of recomputing it per store
+        constexpr int D = 16 * (H + 1), DQ = D / RING_GROUP, DR = D % RING_GROUP;
+        const long long gstride = (long long)rd.ring_slots * RING_GROUP;  // one element group further
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const long long e0 = out.base + (long long)kappa<L>(q, p) * (H + 1) + c;
+            int low = (int)(e0 & (RING_GROUP - 1));
+            float2 *ptr = rd.ring + ((e0 >> RING_GROUP_LOG2) * rd.ring_slots + out.slot) * RING_GROUP + low;
+#pragma unroll
+            for (int k2 = 0; k2 < L; ++k2) {
+                *ptr = z[p][k2];
+                low += DR;
+                ptr += DQ * gstride + DR;
+                if (low >= RING_GROUP) {
+                    low -= RING_GROUP;
+                    ptr += gstride - RING_GROUP;
+                }
+            }
+        }
+    } else {
+        // packed column: DC (kx = 0) and Nyquist (kx = H) of every row ky
+        float2 zq[P][L];
+        partners<L, P>(z, q == 0, zq);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const int k1 = kappa<L>(q, p);
+#pragma unroll
+            for (int k2 = 0; k2 < L; ++k2) {
+                const long long row = (long long)(k1 + 16 * k2) * (H + 1);
+                const float2 wv = z[p][k2], pv = zq[p][k2];  // V[ky], V[S - ky]
+                out[row] = make_float2(0.5f * (wv.x + pv.x), 0.5f * (wv.y - pv.y));
+                out[row + H] = make_float2(0.5f * (wv.y + pv.y), -0.5f * (wv.x - pv.x));
+            }
+        }
+    }
+}
+
+template <int S>
+static void launch_reg(const PixSrc &ps, const RingDst &rd, int tps, int tiles, int n_frames, const float2 *tw,
+                       hipStream_t st) {
+    constexpr int TPW = RegCfg<S>::TPW;
+    const int groups = (tiles + TPW - 1) / TPW;
+    const dim3 grid((unsigned)(groups * n_frames)), block(REG_NTH);
+    if (ps.fmt == DDM_PIX_U8 && ps.bpp == 1)
+        hipLaunchKernelGGL((tile_fft_reg<S, true>), grid, block, 0, st, ps, rd, tps, tiles, groups, tw);
+    else
+        hipLaunchKernelGGL((tile_fft_reg<S, false>), grid, block, 0, st, ps, rd, tps, tiles, groups, tw);
+}
+
+bool launch_tile_fft_reg(const PixSrc &ps, const RingDst &rd, int scale, int tps, int tiles, int n_frames,
+                         const float2 *tw, hipStream_t st) {
+    switch (scale) {
+    case 64: launch_reg<64>(ps, rd, tps, tiles, n_frames, tw, st); return true;
+    case 128: launch_reg<128>(ps, rd, tps, tiles, n_frames, tw, st); return true;
+    default: return false;
+    }
+}
+
+}  // namespace ddm

@@ -25,6 +25,7 @@
 #include <mutex>
 #include <vector>
 
+#include "ddm_device.h"
 #include "ddm_internal.h"
 #include "ddm_kernels.h"
 
@@ -32,16 +33,11 @@ namespace ddm {
 
 constexpr int NT = 256;  // threads per workgroup (4 waves of 64)
 
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 
 // LDS index with one padding slot per 32 complex values: Stockham writes
 // whose lanes step by 4 or 16 values then fall on distinct banks.
 __device__ __forceinline__ int lp(int i) { return i + (i >> 5); }
 constexpr int lds_len(int n) { return n + (n >> 5) + 1; }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
 template <int R>
 struct Bfly;
@@ -130,52 +126,6 @@ __device__ __forceinline__ void fft_passes(float2 *buf, const float2 *tw) {
     }
 }
 
-struct PixSrc {
-    const uint8_t *frames;
-    long long frame_bytes;
-    long long row_bytes;
-    int bpp, fmt, x_off, y_off;
-};
-
-__device__ __forceinline__ float load_px(const uint8_t *fr, const PixSrc &ps, int x, int y) {
-    const uint8_t *p = fr + (long long)y * ps.row_bytes + (long long)x * ps.bpp;
-    if (ps.fmt == DDM_PIX_U8) return (float)p[0];
-    const unsigned b0 = p[0], b1 = p[1];
-    return (float)(ps.fmt == DDM_PIX_U16LE ? (b0 | (b1 << 8)) : ((b0 << 8) | b1));
-}
-
-// Ring layout: the spectrum elements of a frame are cut into groups of
-// RING_GROUP consecutive elements, and each group keeps all ring slots
-// together: element e of slot s lives at ((e / RING_GROUP) * R + s) *
-// RING_GROUP + e % RING_GROUP.  A lag-engine workgroup (one group) then
-// walks the frame history as one contiguous stream.
-__device__ __forceinline__ long long ring_index(long long e, int slot, int R) {
-    return ((e >> RING_GROUP_LOG2) * R + slot) * RING_GROUP + (e & (RING_GROUP - 1));
-}
-
-struct RingDst {
-    float2 *ring;
-    long long elem_off;  // of this scale
-    int ring_slots;
-    int first_slot;      // slot of frame 0 of this launch
-};
-
-// Writes element `local` of tile `tile` of batch frame f.
-struct TileOut {
-    float2 *ring;
-    long long base;  // element index of the tile's first element
-    int slot, R;
-    __device__ __forceinline__ float2 &operator[](long long local) const {
-        return ring[ring_index(base + local, slot, R)];
-    }
-};
-
-__device__ __forceinline__ TileOut tile_out(const RingDst &rd, int f, int tile, long long tile_elems) {
-    int slot = rd.first_slot + f;
-    slot -= (slot >= rd.ring_slots) ? rd.ring_slots : 0;  // f < ring_slots
-    return TileOut{rd.ring, rd.elem_off + (long long)tile * tile_elems, slot, rd.ring_slots};
-}
-
 // ---------------------------------------------------------------------------
 // S <= 128: whole tiles in LDS.  TPW tiles per workgroup so that each
 // workgroup holds at least 1024 complex values.
@@ -197,8 +147,11 @@ __global__ __launch_bounds__(SmallCfg<S>::NTH) void tile_fft_small(PixSrc ps, Ri
     __shared__ float2 buf[lds_len(TPW * TILE)];
     __shared__ float2 tw[S];
     const int tid = threadIdx.x;
-    const int f = blockIdx.x / groups;
-    const int tile0 = (blockIdx.x % groups) * TPW;
+    // frames vary fastest across the grid: concurrent workgroups write
+    // neighbouring ring slots of the same element groups
+    const int nf = gridDim.x / groups;
+    const int f = blockIdx.x % nf;
+    const int tile0 = (blockIdx.x / nf) * TPW;
     for (int i = tid; i < S; i += NT) tw[i] = twg[S + i];
 
     // load: z[t][r][x] = x_{2r}[x] + i x_{2r+1}[x]; every pixel load is in
@@ -315,9 +268,10 @@ __global__ __launch_bounds__(NT) void tile_fft_rows(PixSrc ps, RingDst rd, int t
     __shared__ float2 buf[lds_len(P * S)];
     __shared__ float2 tw[S];
     const int tid = threadIdx.x;
-    const int g = blockIdx.x % GROUPS;
-    const int tile = (blockIdx.x / GROUPS) % tiles;
-    const int f = blockIdx.x / (GROUPS * tiles);
+    const int nf = gridDim.x / (GROUPS * tiles);  // frames vary fastest
+    const int f = blockIdx.x % nf;
+    const int g = (blockIdx.x / nf) % GROUPS;
+    const int tile = blockIdx.x / (nf * GROUPS);
     for (int i = tid; i < S; i += NT) tw[i] = twg[S + i];
     const uint8_t *fr = ps.frames + (long long)f * ps.frame_bytes;
     const int x0 = ps.x_off + (tile % tps) * S;
@@ -365,9 +319,10 @@ __global__ __launch_bounds__(LargeCfg<S>::NTH_COLS) void tile_fft_cols(RingDst r
     __shared__ float2 buf[lds_len(S * CB)];
     __shared__ float2 tw[S];
     const int tid = threadIdx.x;
-    const int cb = blockIdx.x % NB;
-    const int tile = (blockIdx.x / NB) % tiles;
-    const int f = blockIdx.x / (NB * tiles);
+    const int nf = gridDim.x / (NB * tiles);  // frames vary fastest
+    const int f = blockIdx.x % nf;
+    const int cb = (blockIdx.x / nf) % NB;
+    const int tile = blockIdx.x / (nf * NB);
     for (int i = tid; i < S; i += NT) tw[i] = twg[S + i];
     const TileOut t = tile_out(rd, f, tile, (long long)S * (H + 1));
     const int c0 = cb * CB;
@@ -499,6 +454,7 @@ hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st) {
               a.y_off};
     RingDst rd{(float2 *)a.ring, a.elem_off, a.ring_slots, a.first_slot};
     const int tps = a.largest_scale / a.scale, tiles = tps * tps;
+    if (launch_tile_fft_reg(ps, rd, a.scale, tps, tiles, a.n_frames, tw, st)) return hipGetLastError();
     switch (a.scale) {
     case 2: launch_small<2>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
     case 4: launch_small<4>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
