@@ -27,6 +27,21 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+
+def pmc_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summaries
+    (FETCH_SIZE and WRITE_SIZE passes, scaled by a calibration copy): lines
+    `traffic_bytes_per_launch <kernel> <bytes>` in profiles/pmc_traffic_*.md."""
+    import glob
+    here = os.path.dirname(os.path.abspath(__file__))
+    for path in sorted(glob.glob(os.path.join(here, "profiles", "pmc_traffic_*.md"))):
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    return float(parts[2]), os.path.relpath(path, here)
+    return None, None
+
 WORKLOAD = dict(
     frames=1024, size=512, scales=[512, 256, 128, 64], lag_max=512, lag_count=32,
     lam_lo=4.0, lam_hi=256.0, lam_count=30, tol=1.2, chunk=30,
@@ -266,6 +281,7 @@ def main():
     pipe_bytes = S * S + 8 * E * N + lag_bytes
     pipe_gbs = pipe_bytes / (elapsed / args.steps) / 1e9
 
+    traffic, traffic_src = pmc_traffic(f"lag_kernel<{lag_block}>")
     out = {
         "metric": "analysed frames/s (all scales x lags) + p99 chunk latency, at <=1e-4 err vs FP64",
         "value": round(fps, 2),
@@ -296,7 +312,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "algorithmic_bytes_per_launch": int(lag_bytes / lag_launches),
             "avg_launch_ms": round(lag_avg_ms, 4),
             "launches": lag_launches,

@@ -94,6 +94,39 @@ __device__ __forceinline__ void dft8(float2 (&u)[8]) {
     u[7] = csub(e3, o3);
 }
 
+// cos and sin of 2 pi k / 32, k = 0..7
+constexpr float W32C[8] = {1.0f, 0.98078528040323043f, 0.92387953251128674f, 0.83146961230254524f,
+                                    0.70710678118654757f, 0.55557023301960229f, 0.38268343236508984f,
+                                    0.19509032201612833f};
+
+// X[k] = sum_n v[n] exp(-2 pi i n k / 32): radix-2 split into two length-16
+// DFTs, natural order in and out.
+__device__ __forceinline__ void dft32(float2 (&v)[32]) {
+    float2 e[16], o[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        e[i] = v[2 * i];
+        o[i] = v[2 * i + 1];
+    }
+    dft16(e);
+    dft16(o);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        float2 t;
+        if (k == 0) {
+            t = o[0];
+        } else if (k == 8) {
+            t = mul_mi(o[8]);
+        } else if (k < 8) {
+            t = mulw(o[k], W32C[k], -W32C[8 - k]);  // W32^k = cos - i sin, sin(2 pi k/32) = cos(2 pi (8-k)/32)
+        } else {
+            t = mulw(o[k], -W32C[16 - k], -W32C[k - 8]);  // k = 8 + m: W32^k = -i W32^m
+        }
+        v[k] = cadd(e[k], t);
+        v[k + 16] = csub(e[k], t);
+    }
+}
+
 template <int L>
 __device__ __forceinline__ void dftL(float2 (&u)[L]);
 template <>
@@ -351,11 +384,208 @@ static void launch_reg(const PixSrc &ps, const RingDst &rd, int tps, int tiles, 
         hipLaunchKernelGGL((tile_fft_reg<S, false>), grid, block, 0, st, ps, rd, tps, tiles, groups, tw);
 }
 
+// ---------------------------------------------------------------------------
+// 256 x 256 and 512 x 512 tiles: rows and columns in two kernels, the row
+// half spectra parked in the ring slot in between (as tile_fft_rows /
+// tile_fft_cols of ddm_kernels.hip).  Sequences of length S = E1 x 16:
+// phase 1 on 16 lanes with E1 = S/16 values each (radix-16 or radix-32 in
+// registers), phase 2 on the same 16 lanes with E1/16 length-16 DFTs each.
+// The real-pair separation and the packed DC/Nyquist column go through LDS.
+// ---------------------------------------------------------------------------
+template <int S>
+struct BigCfg {
+    static constexpr int H = S / 2;
+    static constexpr int E1 = S / 16;  // phase-1 values per lane
+    static constexpr int P = E1 / 16;  // length-16 DFTs per lane in phase 2
+    static constexpr int NTH = 256;    // 4 waves, 4 sequences each
+    static constexpr int SEQ = NTH / 16;  // sequences per workgroup
+};
+
+template <int E>
+__device__ __forceinline__ void dftE(float2 (&v)[E]);
+template <>
+__device__ __forceinline__ void dftE<16>(float2 (&v)[16]) {
+    dft16(v);
+}
+template <>
+__device__ __forceinline__ void dftE<32>(float2 (&v)[32]) {
+    dft32(v);
+}
+
+// One length-S FFT per group of 16 lanes (q = lane & 15, sequence sq of the
+// wave): v holds x[n1 * 16 + q] on entry; on return the natural-order
+// spectrum of the sequence is at seqbuf[0 .. S) (LDS, private to the wave).
+// x is exchanged through seqbuf as [k1][n2] with a rotation of n2 by k1 so
+// that the 16 lanes of each access hit distinct 8-byte bank slots.
+template <int S>
+__device__ __forceinline__ void big_fft(float2 (&v)[S / 16], float2 *seqbuf, int q,
+                                        const float2 *__restrict__ twg) {
+    constexpr int E1 = S / 16, P = E1 / 16;
+    dftE<E1>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < E1; ++k1) v[k1] = cmul(v[k1], twg[S + __mul24(q, k1)]);
+#pragma unroll
+    for (int k1 = 0; k1 < E1; ++k1) seqbuf[k1 * 16 + ((q + k1) & 15)] = v[k1];
+    float2 u[P][16];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const int k1 = q + 16 * p;
+#pragma unroll
+        for (int n2 = 0; n2 < 16; ++n2) u[p][n2] = seqbuf[k1 * 16 + ((n2 + k1) & 15)];
+        dft16(u[p]);  // u[p][k2] = X[k1 + E1 k2]
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) seqbuf[q + 16 * p + E1 * k2] = u[p][k2];
+}
+
+// Rows: 16 row pairs per workgroup; writes columns 0 .. H-1 of rows 2r, 2r+1
+// of the tile's half spectrum into the ring slot (column 0 = DC + i Nyquist).
+template <int S, bool U8>
+__global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg(PixSrc ps, RingDst rd, int tps, int tiles,
+                                                                   const float2 *__restrict__ twg) {
+    using C = BigCfg<S>;
+    constexpr int H = C::H, E1 = C::E1, SEQ = C::SEQ, GROUPS = H / SEQ;
+    __shared__ float2 lds[SEQ * S];
+    const int tid = threadIdx.x, q = tid & 15, sq = tid >> 4;
+    const int nf = gridDim.x / (GROUPS * tiles);  // frames vary fastest
+    const int f = blockIdx.x % nf;
+    const int g = (blockIdx.x / nf) % GROUPS;
+    const int tile = blockIdx.x / (nf * GROUPS);
+    const int r = g * SEQ + sq;  // row pair within the tile
+    const uint8_t *fr = ps.frames + (long long)f * ps.frame_bytes;
+    const int x0 = ps.x_off + (tile % tps) * S + q;
+    const int y0 = ps.y_off + (tile / tps) * S + 2 * r;
+    float2 v[E1];
+    if constexpr (U8) {
+        const uint8_t *r0 = fr + (long long)y0 * ps.row_bytes + x0;
+        const uint8_t *r1 = r0 + ps.row_bytes;
+#pragma unroll
+        for (int n1 = 0; n1 < E1; ++n1) v[n1] = make_float2((float)r0[n1 * 16], (float)r1[n1 * 16]);
+    } else {
+#pragma unroll
+        for (int n1 = 0; n1 < E1; ++n1)
+            v[n1] = make_float2(load_px(fr, ps, x0 + n1 * 16, y0), load_px(fr, ps, x0 + n1 * 16, y0 + 1));
+    }
+    float2 *seqbuf = lds + sq * S;
+    big_fft<S>(v, seqbuf, q, twg);
+    // separate the row pair: lane q writes columns k = q + 16 m
+    const TileOut out = tile_out(rd, f, tile, (long long)S * (H + 1));
+    const long long ea = out.base + (long long)(2 * r) * (H + 1) + q;
+    const long long eb = ea + (H + 1);
+    const long long gstride = (long long)rd.ring_slots * RING_GROUP;
+    int la = (int)(ea & (RING_GROUP - 1)), lb = (int)(eb & (RING_GROUP - 1));
+    float2 *pa = rd.ring + ((ea >> RING_GROUP_LOG2) * rd.ring_slots + out.slot) * RING_GROUP + la;
+    float2 *pb = rd.ring + ((eb >> RING_GROUP_LOG2) * rd.ring_slots + out.slot) * RING_GROUP + lb;
+#pragma unroll
+    for (int m = 0; m < H / 16; ++m) {
+        const int k = q + 16 * m;
+        const float2 zk = seqbuf[k];
+        const float2 zp = seqbuf[(S - k) & (S - 1)];
+        float2 xa = make_float2(0.5f * (zk.x + zp.x), 0.5f * (zk.y - zp.y));
+        float2 xb = make_float2(0.5f * (zk.y + zp.y), -0.5f * (zk.x - zp.x));
+        if (k == 0) {
+            const float2 zh = seqbuf[H];  // DC and Nyquist of both rows are real
+            xa = make_float2(zk.x, zh.x);
+            xb = make_float2(zk.y, zh.y);
+        }
+        *pa = xa;
+        *pb = xb;
+        la += 16;
+        lb += 16;
+        pa += 16;
+        pb += 16;
+        if (la >= RING_GROUP) {
+            la -= RING_GROUP;
+            pa += gstride - RING_GROUP;
+        }
+        if (lb >= RING_GROUP) {
+            lb -= RING_GROUP;
+            pb += gstride - RING_GROUP;
+        }
+    }
+}
+
+// Columns: 16 consecutive packed columns per workgroup, read from and
+// written back to the ring slot; the packed column is separated into the
+// DC (kx = 0) and Nyquist (kx = H) columns on the way out.
+template <int S>
+__global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_cols_reg(RingDst rd, int tiles,
+                                                                   const float2 *__restrict__ twg) {
+    using C = BigCfg<S>;
+    constexpr int H = C::H, E1 = C::E1, SEQ = C::SEQ, NB = H / SEQ, NTH = C::NTH;
+    // tile block [y][c ^ (y & 15)], 16 columns per row; later the columns'
+    // spectra, S + 1 apart so that reading one row across them is conflict free
+    __shared__ float2 lds[SEQ * (S + 1)];
+    const int tid = threadIdx.x, q = tid & 15, sq = tid >> 4;
+    const int nf = gridDim.x / (NB * tiles);  // frames vary fastest
+    const int f = blockIdx.x % nf;
+    const int cb = (blockIdx.x / nf) % NB;
+    const int tile = blockIdx.x / (nf * NB);
+    const int c0 = cb * SEQ;
+    const TileOut t = tile_out(rd, f, tile, (long long)S * (H + 1));
+    {
+        // coalesced: 16 rows x 16 consecutive columns per pass
+        constexpr int K = S * SEQ / NTH;
+        float2 v[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] = t[(long long)(sq + 16 * k) * (H + 1) + c0 + q];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int y = sq + 16 * k;
+            lds[y * SEQ + (q ^ (y & 15))] = v[k];
+        }
+    }
+    __syncthreads();
+    // column sq of the block, lane q: rows n1 * 16 + q
+    float2 v[E1];
+#pragma unroll
+    for (int n1 = 0; n1 < E1; ++n1) {
+        const int y = n1 * 16 + q;
+        v[n1] = lds[y * SEQ + (sq ^ q)];
+    }
+    __syncthreads();
+    float2 *seqbuf = lds + sq * (S + 1);
+    big_fft<S>(v, seqbuf, q, twg);
+    __syncthreads();
+    // column sq's spectrum is at lds[sq * (S + 1) + ky]; write rows of 16 columns
+    if (c0 == 0) {
+        // packed column: DC and Nyquist of row ky from V[ky] and V[S - ky]
+        for (int ky = tid; ky < S; ky += NTH) {
+            const float2 w = lds[ky], pv = lds[(S - ky) & (S - 1)];
+            t[(long long)ky * (H + 1)] = make_float2(0.5f * (w.x + pv.x), 0.5f * (w.y - pv.y));
+            t[(long long)ky * (H + 1) + H] = make_float2(0.5f * (w.y + pv.y), -0.5f * (w.x - pv.x));
+        }
+    }
+    constexpr int K = S * SEQ / NTH;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int ky = sq + 16 * k;
+        if (c0 + q != 0) t[(long long)ky * (H + 1) + c0 + q] = lds[q * (S + 1) + ky];
+    }
+}
+
+template <int S>
+static void launch_big(const PixSrc &ps, const RingDst &rd, int tps, int tiles, int n_frames, const float2 *tw,
+                       hipStream_t st) {
+    using C = BigCfg<S>;
+    const dim3 g1((unsigned)((C::H / C::SEQ) * tiles * n_frames)), block(C::NTH);
+    if (ps.fmt == DDM_PIX_U8 && ps.bpp == 1)
+        hipLaunchKernelGGL((tile_fft_rows_reg<S, true>), g1, block, 0, st, ps, rd, tps, tiles, tw);
+    else
+        hipLaunchKernelGGL((tile_fft_rows_reg<S, false>), g1, block, 0, st, ps, rd, tps, tiles, tw);
+    const dim3 g2((unsigned)((C::H / C::SEQ) * tiles * n_frames));
+    hipLaunchKernelGGL(tile_fft_cols_reg<S>, g2, block, 0, st, rd, tiles, tw);
+}
+
 bool launch_tile_fft_reg(const PixSrc &ps, const RingDst &rd, int scale, int tps, int tiles, int n_frames,
                          const float2 *tw, hipStream_t st) {
     switch (scale) {
     case 64: launch_reg<64>(ps, rd, tps, tiles, n_frames, tw, st); return true;
     case 128: launch_reg<128>(ps, rd, tps, tiles, n_frames, tw, st); return true;
+    case 256: launch_big<256>(ps, rd, tps, tiles, n_frames, tw, st); return true;
+    case 512: launch_big<512>(ps, rd, tps, tiles, n_frames, tw, st); return true;
     default: return false;
     }
 }
