@@ -28,13 +28,14 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def pmc_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summaries
-    (FETCH_SIZE and WRITE_SIZE passes, scaled by a calibration copy): lines
-    `traffic_bytes_per_launch <kernel> <bytes>` in profiles/pmc_traffic_*.md."""
-    import glob
+def pmc_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3
+    PMC summary profiles/pmc_traffic_<workload>.md (FETCH_SIZE and WRITE_SIZE
+    passes, scaled by a calibration copy): its line
+    `traffic_bytes_per_launch <kernel> <bytes>`."""
     here = os.path.dirname(os.path.abspath(__file__))
-    for path in sorted(glob.glob(os.path.join(here, "profiles", "pmc_traffic_*.md"))):
+    path = os.path.join(here, "profiles", f"pmc_traffic_{workload}.md")
+    if os.path.exists(path):
         with open(path) as f:
             for line in f:
                 parts = line.split()
@@ -42,11 +43,24 @@ def pmc_traffic(kernel: str):
                     return float(parts[2]), os.path.relpath(path, here)
     return None, None
 
-WORKLOAD = dict(
-    frames=1024, size=512, scales=[512, 256, 128, 64], lag_max=512, lag_count=32,
-    lam_lo=4.0, lam_hi=256.0, lam_count=30, tol=1.2, chunk=30,
-    spots=2000, sigma=2.0, amplitude=80.0, diffusion=0.5, background=100.0, noise=5.0, seed=2026,
-)
+WORKLOADS = {
+    # the headline workload: 1024 frames of 512^2 uint8 Brownian video
+    "brownian512": dict(
+        frames=1024, size=512, bits=8, scales=[512, 256, 128, 64], lag_max=512, lag_count=32,
+        lam_lo=4.0, lam_hi=256.0, lam_count=30, tol=1.2, chunk=30, cpu_frames=0,
+        spots=2000, sigma=2.0, amplitude=80.0, diffusion=0.5, background=100.0, noise=5.0, seed=2026,
+        text="single-stream Brownian 1024x512^2 u8, scales 512/256/128/64, 32 log lags 1-512, "
+             "30 log lambdas 4-256 px, tol 20%, chunk 30, plain FP32"),
+    # 1000 frames of 2048^2 uint16 with 12-bit values; the analysed ROI is the
+    # largest scale (1024^2 at the origin)
+    "brownian2048_u16": dict(
+        frames=1000, size=2048, bits=12, scales=[1024, 512, 256, 128, 64, 32], lag_max=999, lag_count=64,
+        lam_lo=2.0, lam_hi=512.0, lam_count=40, tol=1.2, chunk=30, cpu_frames=60,
+        spots=8000, sigma=2.0, amplitude=80.0, diffusion=1.0, background=1000.0, noise=30.0, seed=2027,
+        text="single-stream Brownian 1000x2048^2 u16 (12-bit), ROI 1024^2, scales 1024..32, 64 log lags "
+             "1-999, 40 log lambdas 2-512 px, tol 20%, chunk 30, plain FP32"),
+}
+WORKLOAD = WORKLOADS["brownian512"]
 
 
 def distinct_frames(t0, n, lags):
@@ -189,8 +203,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--frames", type=int, default=WORKLOAD["frames"])
+    ap.add_argument("--workload", default="brownian512", choices=sorted(WORKLOADS))
+    ap.add_argument("--frames", type=int, default=0, help="frames to analyse (0 = the workload's count)")
     ap.add_argument("--lag-block", type=int, default=0, help="frames per lag-engine pass (0 = engine default)")
+    ap.add_argument("--host-input", action="store_true",
+                    help="frames start in pinned host memory (PCIe-inclusive rate; not the headline value)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=0, help="frames for the CPU baseline sample (0 = all)")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 tile checks")
@@ -216,23 +233,36 @@ def main():
 
     import cuda_multiddm_amd as ddm
 
-    W = dict(WORKLOAD)
-    W["frames"] = args.frames
+    W = dict(WORKLOADS[args.workload])
+    if args.frames:
+        W["frames"] = args.frames
     N, S = W["frames"], W["size"]
+    M = W["scales"][0]  # analysed ROI: M x M at the frame origin
+    bpp, pixfmt = (1, ddm.PIX_U8) if W["bits"] <= 8 else (2, ddm.PIX_U16LE)
     lags = ddm.log_lags(W["lag_max"], W["lag_count"])
     lams = ddm.log_lambdas(W["lam_lo"], W["lam_hi"], W["lam_count"])
     print(f"[bench] generating {N} frames {S}x{S}", file=sys.stderr, flush=True)
-    frames = ddm.synth_brownian(N, S, S, 8, W["spots"], W["sigma"], W["amplitude"], W["diffusion"],
+    frames = ddm.synth_brownian(N, S, S, W["bits"], W["spots"], W["sigma"], W["amplitude"], W["diffusion"],
                                 W["background"], W["noise"], W["seed"], min(os.cpu_count() or 1, 16))
+    raw = frames.reshape(-1).view(np.uint8)
+    print(f"[bench] frames ready ({raw.nbytes / 1e9:.2f} GB)", file=sys.stderr, flush=True)
     torch.cuda.set_device(0)
-    d_frames = torch.from_numpy(frames.reshape(-1).copy()).cuda()
+    if args.host_input:
+        h_frames = torch.empty(raw.size, dtype=torch.uint8, pin_memory=True).numpy()
+        h_frames[:] = raw
+    else:
+        d_frames = torch.from_numpy(raw).cuda()
     torch.cuda.synchronize()
 
     def make():
-        return ddm.Engine(S, S, W["scales"], lags, lams, W["tol"], W["chunk"], lag_block=args.lag_block)
+        return ddm.Engine(S, S, W["scales"], lags, lams, W["tol"], W["chunk"], bpp=bpp, pixfmt=pixfmt,
+                          lag_block=args.lag_block)
 
     def step(e):
-        e.submit_device(d_frames.data_ptr(), N)
+        if args.host_input:
+            e.submit_host(h_frames, N)  # ROI rows only cross PCIe
+        else:
+            e.submit_device(d_frames.data_ptr(), N)
         return e.results()
 
     for _ in range(args.warmup):
@@ -241,6 +271,7 @@ def main():
     torch.cuda.synchronize()
     elapsed = 0.0
     res = None
+    print("[bench] warmup done", file=sys.stderr, flush=True)
     for _ in range(args.steps):
         e = make()
         torch.cuda.synchronize()
@@ -278,10 +309,10 @@ def main():
     achieved = lag_bytes / (st.lag_ms * 1e-3) / 1e9
     # whole-pipeline algorithmic bytes per frame (ROI read, spectra write,
     # lag engine reads + accumulators), for context
-    pipe_bytes = S * S + 8 * E * N + lag_bytes
+    pipe_bytes = (M * M * bpp + 8 * E) * N + lag_bytes
     pipe_gbs = pipe_bytes / (elapsed / args.steps) / 1e9
 
-    traffic, traffic_src = pmc_traffic(f"lag_kernel<{lag_block}>")
+    traffic, traffic_src = pmc_traffic(args.workload, f"lag_kernel<{lag_block}>")
     out = {
         "metric": "analysed frames/s (all scales x lags) + p99 chunk latency, at <=1e-4 err vs FP64",
         "value": round(fps, 2),
@@ -296,8 +327,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"single-stream Brownian {N}x{S}^2 u8, scales 512/256/128/64, {len(lags)} log lags 1-512, "
-                        f"{len(lams)} log lambdas 4-256 px, tol 20%, chunk 30, plain FP32",
+            "workload": W["text"].replace(f"{WORKLOADS[args.workload]['frames']}x", f"{N}x"),
+            "name": args.workload,
+            "input": "pinned host memory (PCIe-inclusive)" if args.host_input else "device memory",
             "frames": N, "frame_size": S, "scales": W["scales"], "lags": len(lags), "lambdas": len(lams),
             "chunk": W["chunk"], "elems_per_frame": E, "ring_slots": info.ring_slots,
         },
@@ -323,7 +355,7 @@ def main():
 
     if not args.no_check:
         from oracle import oracle as o
-        cfg = o.Config(S, S, N, W["scales"], lags, lams, W["tol"])
+        cfg = o.Config(S, S, N, W["scales"], lags, lams, W["tol"], bpp, pixfmt)
         flat = res
         offs, off = {}, 0
         for s in W["scales"]:
@@ -333,22 +365,24 @@ def main():
         for si, s in enumerate(W["scales"]):
             if s > 256:
                 continue
-            ref = o.run(cfg, frames, "f64", nthreads=min(os.cpu_count() or 1, 16), only_scale=si, max_tiles=1)
+            ref = o.run(cfg, raw, "f64", nthreads=min(os.cpu_count() or 1, 16), only_scale=si, max_tiles=1)
             n1 = len(lams) * len(lags)
             errs[s] = o.max_rel_err(flat[offs[s]:offs[s] + n1], ref[offs[s]:offs[s] + n1])
+            print(f"[bench] FP64 check scale {s}: {errs[s]:.3e}", file=sys.stderr, flush=True)
         out["max_rel_err_vs_fp64"] = {f"scale{s}_tile0": float(f"{v:.3e}") for s, v in errs.items()}
 
     if not args.no_cpu_baseline:
         from oracle import oracle as o
         threads = min(os.cpu_count() or 1, 16)
-        nf = args.cpu_frames or N
-        cfg = o.Config(S, S, nf, W["scales"], lags, lams, W["tol"])
+        nf = min(args.cpu_frames or W["cpu_frames"] or N, N)
+        cfg = o.Config(S, S, nf, W["scales"], lags, lams, W["tol"], bpp, pixfmt)
         total = 0.0
         per = {}
         for si, s in enumerate(W["scales"]):
             t0 = time.perf_counter()
-            o.run(cfg, frames[:nf], "f32", nthreads=threads, only_scale=si, max_tiles=1)
+            o.run(cfg, raw[:nf * S * S * bpp], "f32", nthreads=threads, only_scale=si, max_tiles=1)
             dt = time.perf_counter() - t0
+            print(f"[bench] CPU baseline scale {s}: {dt:.2f} s", file=sys.stderr, flush=True)
             tiles = (W["scales"][0] // s) ** 2
             per[s] = dt
             total += dt * tiles

@@ -378,21 +378,35 @@ extern "C" int ddm_submit_host(ddm_engine *e, const void *frames, int n_frames) 
     const size_t roi_frame = row * p.M;
     const uint8_t *src = (const uint8_t *)frames;
     const size_t fb = (size_t)p.width * p.height * p.bpp;
-    // Frames that already sit in pinned memory and are exactly the ROI go
-    // to the device without the staging copy.
+    // Frames that already sit in pinned memory go to the device without the
+    // staging copy: the ROI rows are gathered by the copy engine (2-D copy).
     bool direct = false;
-    if (p.x_off == 0 && p.y_off == 0 && p.width == p.M && p.height == p.M && n_frames > 0) {
+    if (n_frames > 0) {
         hipPointerAttribute_t attr;
         direct = hipPointerGetAttributes(&attr, frames) == hipSuccess && attr.type == hipMemoryTypeHost;
         (void)hipGetLastError();  // pageable memory reports an error here; do not leave it behind
     }
+    const bool whole = p.x_off == 0 && p.y_off == 0 && p.width == p.M && p.height == p.M;
+    const bool rows_stack = p.y_off == 0 && p.height == p.M;  // ROI rows of consecutive frames are evenly spaced
     for (int i = 0; i < n_frames; i += p.chunk) {
         const int n = std::min(p.chunk, n_frames - i);
         const int b = e->next_stage;
         if (e->stage_used[b]) HIP_TRY(e, hipEventSynchronize(e->ev_free[b]));
-        if (direct) {
+        if (direct && whole) {
             HIP_TRY(e, hipMemcpyAsync(e->d_stage[b], src + (size_t)i * fb, roi_frame * n, hipMemcpyHostToDevice,
                                       e->st_copy));
+        } else if (direct) {
+            const size_t pitch = (size_t)p.width * p.bpp;
+            const uint8_t *roi0 = src + (size_t)i * fb + (size_t)p.x_off * p.bpp;
+            if (rows_stack) {
+                HIP_TRY(e, hipMemcpy2DAsync(e->d_stage[b], row, roi0, pitch, row, (size_t)p.M * n,
+                                            hipMemcpyHostToDevice, e->st_copy));
+            } else {
+                for (int f = 0; f < n; ++f)
+                    HIP_TRY(e, hipMemcpy2DAsync(e->d_stage[b] + (size_t)f * roi_frame, row,
+                                                roi0 + (size_t)f * fb + (size_t)p.y_off * pitch, pitch, row, p.M,
+                                                hipMemcpyHostToDevice, e->st_copy));
+            }
         } else {
             if (!e->h_stage[b]) {
                 HIP_TRY(e, hipHostMalloc((void **)&e->h_stage[b], roi_frame * p.chunk, hipHostMallocDefault));

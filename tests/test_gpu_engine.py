@@ -88,6 +88,40 @@ def test_host_and_device_submission_identical(gpu):
         assert o.max_rel_err(d, ref) <= TOL
 
 
+@pytest.mark.parametrize("w,h,x_off,y_off,bpp,fmt", [
+    (96, 64, 32, 0, 1, ddm.PIX_U8),      # ROI rows of consecutive frames evenly spaced: one 2-D copy
+    (80, 90, 8, 20, 2, ddm.PIX_U16LE),   # general ROI: one 2-D copy per frame
+    (64, 64, 0, 0, 3, ddm.PIX_U8),       # 3-byte pixels, ROI = frame
+])
+def test_pinned_host_frames_roi_upload(gpu, w, h, x_off, y_off, bpp, fmt):
+    """Frames in pinned host memory are uploaded by the copy engine (ROI rows
+    only); the results equal the pageable-host and device submissions."""
+    import torch
+    rng = np.random.default_rng(w + h + bpp)
+    n = 40
+    raw = rng.integers(0, 256, (n, h, w * bpp), dtype=np.uint8)
+    pinned = torch.empty(raw.size, dtype=torch.uint8, pin_memory=True).numpy()
+    pinned[:] = raw.reshape(-1)
+    scales, lags, lams = [64, 32], [1, 3, 8, 33], [3.0, 6.0]
+    outs = []
+    for src in ("pinned", "pageable", "device"):
+        with ddm.Engine(w, h, scales, lags, lams, 1.2, 16, bpp, fmt, x_off, y_off) as e:
+            print(f"roi upload {src} {w}x{h} off=({x_off},{y_off}) bpp={bpp}", flush=True)
+            if src == "pinned":
+                e.submit_host(pinned, n)
+            elif src == "pageable":
+                e.submit_host(raw.reshape(-1).copy(), n)
+            else:
+                t = torch.from_numpy(raw.reshape(-1).copy()).cuda()
+                torch.cuda.synchronize()
+                e.submit_device(t.data_ptr(), n)
+                e.sync()
+            outs.append(e.results())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[1], outs[2])
+    cfg = o.Config(w, h, n, scales, lags, lams, np.float32(1.2).item(), bpp, fmt, x_off, y_off)
+    assert o.max_rel_err(outs[0], o.run(cfg, raw)) <= TOL
+
+
 def test_results_mid_stream(gpu):
     """Reading results flushes a partial lag block; the stream then goes on
     and the final answer still equals a single pass."""
