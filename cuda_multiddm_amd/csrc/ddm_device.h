@@ -68,4 +68,9 @@ __device__ __forceinline__ TileOut tile_out(const RingDst &rd, int f, int tile, 
 bool launch_tile_fft_reg(const PixSrc &ps, const RingDst &rd, int scale, int tps, int tiles, int n_frames,
                          const float2 *tw, hipStream_t st);
 
+// 64 x 64 / 128 x 128 tile DFT on the matrix cores with split 16-bit
+// operands (ddm_dft_mfma.hip); precision is DDM_PREC_BF16X2 or _FP16X2.
+hipError_t launch_tile_dft_mfma(const PixSrc &ps, const RingDst &rd, int scale, int tps, int tiles, int n_frames,
+                                int precision, hipStream_t st);
+
 }  // namespace ddm

@@ -62,6 +62,7 @@ struct ddm_engine {
     size_t dev_bytes = 0, pinned_bytes = 0;
     std::string err;
     bool timing = false;
+    int precision = DDM_PREC_FP32;
     std::vector<Mark> pending;
     std::vector<hipEvent_t> pool;
     ddm_stage_stats stats{};
@@ -335,6 +336,7 @@ static int process_chunk(ddm_engine *e, const uint8_t *src, int n, long long fra
         a.first_slot = first_slot;
         a.elems_per_frame = p.elems_padded;
         a.elem_off = g.elem_off;
+        a.precision = e->precision;
         HIP_TRY(e, ddm::launch_tile_fft(a, e->st));
         e->stats.fft_launches += (g.s >= 256) ? 2 : 1;
     }
@@ -620,6 +622,14 @@ extern "C" int ddm_get_spectra(ddm_engine *e, long long frame, int scale_idx, fl
                            e->d_ring + ((g0 * p.ring_slots + slot) * G) * 2, (size_t)p.ring_slots * G * 2 * sizeof(float),
                            G * 2 * sizeof(float), g1 - g0, hipMemcpyDeviceToHost));
     std::memcpy(out, rows.data() + ((size_t)g.elem_off - g0 * G) * 2, n * 2 * sizeof(float));
+    return DDM_OK;
+}
+
+extern "C" int ddm_set_precision(ddm_engine *e, int mode) {
+    if (!e) return DDM_EINVAL;
+    if (mode != DDM_PREC_FP32 && mode != DDM_PREC_BF16X2 && mode != DDM_PREC_FP16X2)
+        return fail(e, DDM_EINVAL, "unknown precision mode");
+    e->precision = mode;
     return DDM_OK;
 }
 

@@ -20,6 +20,7 @@ struct TileFftArgs {
     int first_slot;            // slot of frame 0 of this batch
     long long elems_per_frame;
     long long elem_off;        // offset of this scale within a frame's spectrum
+    int precision = DDM_PREC_FP32;  // enum ddm_precision (64 / 128 tiles only)
 };
 
 struct LagArgs {

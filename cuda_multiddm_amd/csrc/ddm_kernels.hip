@@ -454,6 +454,8 @@ hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st) {
               a.y_off};
     RingDst rd{(float2 *)a.ring, a.elem_off, a.ring_slots, a.first_slot};
     const int tps = a.largest_scale / a.scale, tiles = tps * tps;
+    if (a.precision != DDM_PREC_FP32 && (a.scale == 64 || a.scale == 128))
+        return launch_tile_dft_mfma(ps, rd, a.scale, tps, tiles, a.n_frames, a.precision, st);
     if (launch_tile_fft_reg(ps, rd, a.scale, tps, tiles, a.n_frames, tw, st)) return hipGetLastError();
     switch (a.scale) {
     case 2: launch_small<2>(ps, rd, tps, tiles, a.n_frames, tw, st); break;

@@ -27,6 +27,7 @@ STATUS = {0: "DDM_OK", -1: "DDM_EINVAL", -2: "DDM_ENOMEM", -3: "DDM_EDEVICE", -4
 DDM_EBUSY = -10
 PIX_U8, PIX_U16LE, PIX_U16BE = 0, 1, 2
 NORM_FRAMES, NORM_PAIRS = 0, 1
+PREC_FP32, PREC_BF16X2, PREC_FP16X2 = 0, 1, 2
 
 # Every symbol include/ddm_amd.h declares.
 EXPORTED = [
@@ -37,7 +38,7 @@ EXPORTED = [
     "ddm_synth_uniform", "ddm_synth_brownian", "ddm_write_iqt_file", "ddm_flush",
     "ddm_server_create", "ddm_server_destroy", "ddm_server_open", "ddm_server_submit", "ddm_server_results",
     "ddm_server_engine", "ddm_server_drain", "ddm_server_close", "ddm_server_latencies", "ddm_server_stats",
-    "ddm_server_last_error", "ddm_fit_results", "ddm_k_fit",
+    "ddm_server_last_error", "ddm_fit_results", "ddm_k_fit", "ddm_set_precision",
 ]
 
 
@@ -117,6 +118,7 @@ def lib():
     L.ddm_get_accumulators.argtypes = [vp, ctypes.c_int, fp]
     L.ddm_get_spectra.argtypes = [vp, ll, ctypes.c_int, fp]
     L.ddm_set_timing.argtypes = [vp, ctypes.c_int]
+    L.ddm_set_precision.argtypes = [vp, ctypes.c_int]
     L.ddm_get_stage_stats.argtypes = [vp, ctypes.POINTER(StageStats)]
     L.ddm_get_chunk_times.argtypes = [vp, fp, ctypes.c_int]
     L.ddm_last_error.argtypes = [vp]
@@ -276,6 +278,10 @@ class Engine:
             D[s] = dco[off:off + T]
             off += T
         return params, D
+
+    def set_precision(self, mode: int):
+        """PREC_FP32 (default), PREC_BF16X2 or PREC_FP16X2 for the 64 / 128 tile transforms."""
+        self._check(lib().ddm_set_precision(self._h, int(mode)))
 
     def reset(self):
         self._check(lib().ddm_reset(self._h))

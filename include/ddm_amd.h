@@ -62,6 +62,13 @@ enum ddm_pixel_format { DDM_PIX_U8 = 0, DDM_PIX_U16LE = 1, DDM_PIX_U16BE = 2 };
  * (t, t + tau) summed for each lag instead. */
 enum ddm_norm_mode { DDM_NORM_FRAMES = 0, DDM_NORM_PAIRS = 1 };
 
+/* Arithmetic of the 64 x 64 and 128 x 128 tile transforms (ddm_set_precision).
+ * DDM_PREC_FP32: the FP32 FFT (default; every other scale always uses it).
+ * DDM_PREC_BF16X2 / DDM_PREC_FP16X2: the 2-D DFT as matrix products on the
+ * matrix cores, each operand split into a high and a low BF16 / FP16 part,
+ * FP32 accumulation. */
+enum ddm_precision { DDM_PREC_FP32 = 0, DDM_PREC_BF16X2 = 1, DDM_PREC_FP16X2 = 2 };
+
 typedef struct ddm_desc {
     int width;            /* frame width in pixels */
     int height;           /* frame height in pixels */
@@ -172,6 +179,11 @@ int ddm_get_accumulators(ddm_engine *eng, int scale_idx, float *out);
 int ddm_get_spectra(ddm_engine *eng, long long frame, int scale_idx, float *out);
 
 int ddm_set_timing(ddm_engine *eng, int on);
+
+/* Selects the arithmetic of the 64 x 64 and 128 x 128 tile transforms for the
+ * frames submitted from now on (enum ddm_precision).  No counterpart in the
+ * reference, which always uses cuFFT in FP32. */
+int ddm_set_precision(ddm_engine *eng, int mode);
 int ddm_get_stage_stats(ddm_engine *eng, ddm_stage_stats *stats);
 /* Device time of each submitted chunk (tiler+FFT+lag), in submission order;
  * returns the number written (<= max). */
