@@ -219,11 +219,44 @@ __device__ __forceinline__ void partners(const float2 (&z)[P][L], bool lane0, fl
     }
 }
 
+// Pixel access modes of the register FFT kernels (see pixel_mode).
+enum { PIX_ANY = 0, PIX_U8_1 = 1, PIX_U16LE_2 = 2 };
+
+// The u16 mode needs every pixel 2-byte aligned.
+__host__ __device__ inline int pixel_mode(const PixSrc &ps) {
+    if (ps.fmt == DDM_PIX_U8 && ps.bpp == 1) return PIX_U8_1;
+    if (ps.fmt == DDM_PIX_U16LE && ps.bpp == 2 && ((ps.row_bytes | ps.frame_bytes | (long long)(uintptr_t)ps.frames) & 1) == 0)
+        return PIX_U16LE_2;
+    return PIX_ANY;
+}
+
+// Pixels (x0 + n * STEP, y0) and (x0 + n * STEP, y0 + 1), n < N, as complex
+// values x_{2r} + i x_{2r+1}.
+template <int PIX, int N, int STEP>
+__device__ __forceinline__ void load_row_pair(const uint8_t *fr, const PixSrc &ps, int x0, int y0, float2 (&v)[N]) {
+    if constexpr (PIX == PIX_U8_1) {
+        const uint8_t *r0 = fr + (long long)y0 * ps.row_bytes + x0;
+        const uint8_t *r1 = r0 + ps.row_bytes;
+#pragma unroll
+        for (int n = 0; n < N; ++n) v[n] = make_float2((float)r0[n * STEP], (float)r1[n * STEP]);
+    } else if constexpr (PIX == PIX_U16LE_2) {
+        const uint16_t *r0 = reinterpret_cast<const uint16_t *>(fr + (long long)y0 * ps.row_bytes) + x0;
+        const uint16_t *r1 = reinterpret_cast<const uint16_t *>(reinterpret_cast<const uint8_t *>(r0) + ps.row_bytes);
+#pragma unroll
+        for (int n = 0; n < N; ++n) v[n] = make_float2((float)r0[n * STEP], (float)r1[n * STEP]);
+    } else {
+#pragma unroll
+        for (int n = 0; n < N; ++n)
+            v[n] = make_float2(load_px(fr, ps, x0 + n * STEP, y0), load_px(fr, ps, x0 + n * STEP, y0 + 1));
+    }
+}
+
 }  // namespace
 
-// U8: pixels are single bytes (bpp 1, DDM_PIX_U8), loaded with constant
-// offsets; otherwise every pixel goes through load_px.
-template <int S, bool U8>
+// PIX: PIX_U8_1 (bpp 1, DDM_PIX_U8) and PIX_U16LE_2 (bpp 2, DDM_PIX_U16LE)
+// load pixels with constant offsets from two row pointers; PIX_ANY sends
+// every pixel through load_px.
+template <int S, int PIX>
 __global__ __launch_bounds__(REG_NTH) void tile_fft_reg(PixSrc ps, RingDst rd, int tps, int tiles, int groups,
                                                        const float2 *__restrict__ twg) {
     using C = RegCfg<S>;
@@ -256,16 +289,7 @@ __global__ __launch_bounds__(REG_NTH) void tile_fft_reg(PixSrc ps, RingDst rd, i
             const uint8_t *fr = ps.frames + (long long)f * ps.frame_bytes;
             const int x0 = ps.x_off + (tile % tps) * S + q;
             const int y0 = ps.y_off + (tile / tps) * S + 2 * r;
-            if constexpr (U8) {
-                const uint8_t *r0 = fr + (long long)y0 * ps.row_bytes + x0;
-                const uint8_t *r1 = r0 + ps.row_bytes;
-#pragma unroll
-                for (int n1 = 0; n1 < 16; ++n1) v[n1] = make_float2((float)r0[n1 * L], (float)r1[n1 * L]);
-            } else {
-#pragma unroll
-                for (int n1 = 0; n1 < 16; ++n1)
-                    v[n1] = make_float2(load_px(fr, ps, x0 + n1 * L, y0), load_px(fr, ps, x0 + n1 * L, y0 + 1));
-            }
+            load_row_pair<PIX, 16, L>(fr, ps, x0, y0, v);
         } else {
 #pragma unroll
             for (int n1 = 0; n1 < 16; ++n1) v[n1] = make_float2(0.f, 0.f);
@@ -378,10 +402,16 @@ static void launch_reg(const PixSrc &ps, const RingDst &rd, int tps, int tiles, 
     constexpr int TPW = RegCfg<S>::TPW;
     const int groups = (tiles + TPW - 1) / TPW;
     const dim3 grid((unsigned)(groups * n_frames)), block(REG_NTH);
-    if (ps.fmt == DDM_PIX_U8 && ps.bpp == 1)
-        hipLaunchKernelGGL((tile_fft_reg<S, true>), grid, block, 0, st, ps, rd, tps, tiles, groups, tw);
-    else
-        hipLaunchKernelGGL((tile_fft_reg<S, false>), grid, block, 0, st, ps, rd, tps, tiles, groups, tw);
+    switch (pixel_mode(ps)) {
+    case PIX_U8_1:
+        hipLaunchKernelGGL((tile_fft_reg<S, PIX_U8_1>), grid, block, 0, st, ps, rd, tps, tiles, groups, tw);
+        break;
+    case PIX_U16LE_2:
+        hipLaunchKernelGGL((tile_fft_reg<S, PIX_U16LE_2>), grid, block, 0, st, ps, rd, tps, tiles, groups, tw);
+        break;
+    default:
+        hipLaunchKernelGGL((tile_fft_reg<S, PIX_ANY>), grid, block, 0, st, ps, rd, tps, tiles, groups, tw);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -442,7 +472,7 @@ __device__ __forceinline__ void big_fft(float2 (&v)[S / 16], float2 *seqbuf, int
 
 // Rows: 16 row pairs per workgroup; writes columns 0 .. H-1 of rows 2r, 2r+1
 // of the tile's half spectrum into the ring slot (column 0 = DC + i Nyquist).
-template <int S, bool U8>
+template <int S, int PIX>
 __global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg(PixSrc ps, RingDst rd, int tps, int tiles,
                                                                    const float2 *__restrict__ twg) {
     using C = BigCfg<S>;
@@ -458,16 +488,7 @@ __global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg(PixSrc ps, R
     const int x0 = ps.x_off + (tile % tps) * S + q;
     const int y0 = ps.y_off + (tile / tps) * S + 2 * r;
     float2 v[E1];
-    if constexpr (U8) {
-        const uint8_t *r0 = fr + (long long)y0 * ps.row_bytes + x0;
-        const uint8_t *r1 = r0 + ps.row_bytes;
-#pragma unroll
-        for (int n1 = 0; n1 < E1; ++n1) v[n1] = make_float2((float)r0[n1 * 16], (float)r1[n1 * 16]);
-    } else {
-#pragma unroll
-        for (int n1 = 0; n1 < E1; ++n1)
-            v[n1] = make_float2(load_px(fr, ps, x0 + n1 * 16, y0), load_px(fr, ps, x0 + n1 * 16, y0 + 1));
-    }
+    load_row_pair<PIX, E1, 16>(fr, ps, x0, y0, v);
     float2 *seqbuf = lds + sq * S;
     big_fft<S>(v, seqbuf, q, twg);
     // separate the row pair: lane q writes columns k = q + 16 m
@@ -571,10 +592,13 @@ static void launch_big(const PixSrc &ps, const RingDst &rd, int tps, int tiles, 
                        hipStream_t st) {
     using C = BigCfg<S>;
     const dim3 g1((unsigned)((C::H / C::SEQ) * tiles * n_frames)), block(C::NTH);
-    if (ps.fmt == DDM_PIX_U8 && ps.bpp == 1)
-        hipLaunchKernelGGL((tile_fft_rows_reg<S, true>), g1, block, 0, st, ps, rd, tps, tiles, tw);
-    else
-        hipLaunchKernelGGL((tile_fft_rows_reg<S, false>), g1, block, 0, st, ps, rd, tps, tiles, tw);
+    switch (pixel_mode(ps)) {
+    case PIX_U8_1: hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_U8_1>), g1, block, 0, st, ps, rd, tps, tiles, tw); break;
+    case PIX_U16LE_2:
+        hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_U16LE_2>), g1, block, 0, st, ps, rd, tps, tiles, tw);
+        break;
+    default: hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_ANY>), g1, block, 0, st, ps, rd, tps, tiles, tw);
+    }
     const dim3 g2((unsigned)((C::H / C::SEQ) * tiles * n_frames));
     hipLaunchKernelGGL(tile_fft_cols_reg<S>, g2, block, 0, st, rd, tiles, tw);
 }

@@ -123,6 +123,35 @@ def test_tile_fft_pixel_formats(gpu, fmt, bpp):
             assert np.max(np.abs(spec[f, tile] - ref)) / np.max(np.abs(ref)) < FFT_TOL
 
 
+@pytest.mark.parametrize("s", [64, 128, 256, 512])
+@pytest.mark.parametrize("fmt,bpp", [(ddm.PIX_U16LE, 2), (ddm.PIX_U16BE, 2), (ddm.PIX_U16LE, 4), (ddm.PIX_U8, 1)])
+def test_tile_fft_pixel_formats_large_tiles(gpu, s, fmt, bpp):
+    """The 64..512 tile kernels read 1-byte and 2-byte little-endian pixels
+    with their own loads; other layouts take the general pixel reader."""
+    rng = np.random.default_rng(s + fmt * 10 + bpp)
+    M, n, xo, yo = 2 * s, 2, 3, 1
+    hi = 256 if fmt == ddm.PIX_U8 else 4096
+    vals = rng.integers(0, hi, (n, M + yo, M + xo + 2)).astype(np.uint32)
+    raw = np.zeros(vals.shape + (bpp,), dtype=np.uint8)
+    if fmt == ddm.PIX_U8:
+        raw[..., 0] = vals
+    elif fmt == ddm.PIX_U16LE:
+        raw[..., 0], raw[..., 1] = vals & 0xFF, vals >> 8
+    else:
+        raw[..., 0], raw[..., 1] = vals >> 8, vals & 0xFF
+    if bpp > 2:
+        raw[..., 2:] = 77
+    spec = _run_tile_fft(gpu, raw.reshape(n, M + yo, -1), bpp, fmt, M, s, x_off=xo, y_off=yo, width=M + xo + 2)
+    spec = spec.reshape(n, 4, s, s // 2 + 1)
+    for f in range(n):
+        for tile in range(4):
+            t = _tile(vals[f], M, s, tile, xo, yo)
+            if t.sum() < 2 ** 24:
+                assert spec[f, tile, 0, 0] == t.sum()
+            ref = np.fft.rfft2(t)
+            assert np.max(np.abs(spec[f, tile] - ref)) / np.max(np.abs(ref)) < FFT_TOL
+
+
 def _lag_expected(ring, R, t0, n_new, lags):
     E = ring.shape[1]
     acc = np.zeros((len(lags), E), dtype=np.int64)
