@@ -35,7 +35,15 @@ struct LagCfg {
     static constexpr int SLOTS_MAX = WIDE ? 16 : 8;  // partial sums per wave
     static constexpr int UNROLL = WIDE ? 16 : 8;     // pairs per batch of loads
     static constexpr int WAVES_PER_EU = WIDE ? 4 : 8;
+    // history-major path for the other lags: at most HIST_LAGS of them, one
+    // register sum each, over blocks of HIST_BLOCK history frames
+    static constexpr int HIST_LAGS = WIDE ? 32 : 0;
+    static constexpr int HIST_BLOCK = 16;
 };
+
+// Relative cost of a pair in the register window and in the history-major
+// path, for the split of the waves.
+constexpr int COST_DENSE = 4, COST_HIST = 5;
 
 // sum_{k < count} |a[k*G] - b[k*G]|^2, G = RING_GROUP (consecutive ring slots
 // or consecutive LDS rows are RING_GROUP complex values apart).  Loads come
@@ -141,6 +149,102 @@ __device__ __forceinline__ void dense_sums(const float2 *g, const float2 *nl, in
     }
 }
 
+// Lags tau_0 < ... < tau_{ls-1} (lane l of `vtau` holds tau_l) by history
+// frame: frame p of the window (ring history for p < 0, LDS for p >= 0)
+// pairs with the new frames j = p + tau_l inside [0, n).  The wave takes
+// blocks of B consecutive p (block b = k, k + ws, ...), loads each block once
+// into registers and pairs it with the LDS rows of every lag that reaches it,
+// so a history frame is requested once per wave rather than once per lag.
+// s[l] collects lag l in a fixed block order.
+template <int B, int LSM>
+__device__ __forceinline__ void hist_sums(const float2 *g, const float2 *nl, int R, int slot0, int n, int dmax,
+                                          int vtau, int ls, int k, int ws, float (&s)[LSM]) {
+    const int tmin = __builtin_amdgcn_readlane(vtau, 0), tmax = __builtin_amdgcn_readlane(vtau, ls - 1);
+    const int pmin = max(-dmax, -tmax), pmax = n - 1 - tmin;
+    const int lane = threadIdx.x % RING_GROUP;
+    const bool live = lane < ls;
+    for (int p0 = pmin + k * B; p0 <= pmax; p0 += ws * B) {
+        float2 fp[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) fp[u] = window_frame(g, nl, R, slot0, min(p0 + u, pmax));
+        // the lags reaching the block: -p0 - B < tau < n - p0
+        const int la = __builtin_popcountll(__ballot(live && vtau <= -p0 - B));
+        const int lb = __builtin_popcountll(__ballot(live && vtau < n - p0));
+        for (int l = la; l < lb; ++l) {
+            const int j0 = p0 + __builtin_amdgcn_readlane(vtau, l);
+            const float2 *a = nl + j0 * RING_GROUP;
+            float t[4] = {0.f, 0.f, 0.f, 0.f};
+            if (j0 >= 0 && j0 + B <= n) {
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    const float2 v = a[u * RING_GROUP];
+                    const float dx = v.x - fp[u].x, dy = v.y - fp[u].y;
+                    t[u & 3] = fmaf(dx, dx, t[u & 3]);
+                    t[u & 3] = fmaf(dy, dy, t[u & 3]);
+                }
+            } else {
+                const int ua = -j0, ub = n - j0;  // valid u: ua <= u < ub
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    if (u >= ua && u < ub) {
+                        const float2 v = a[u * RING_GROUP];
+                        const float dx = v.x - fp[u].x, dy = v.y - fp[u].y;
+                        t[u & 3] = fmaf(dx, dx, t[u & 3]);
+                        t[u & 3] = fmaf(dy, dy, t[u & 3]);
+                    }
+                }
+            }
+            s[l] += (t[0] + t[1]) + (t[2] + t[3]);
+        }
+    }
+}
+
+// Partial sums of a launch with history-major lags meet in the staging buffer
+// (the new frames are no longer read) in rounds of 32 of those lags: in round
+// r, history wave k writes its sums of lags 32r + i to rows wd*SL + 32k + i,
+// and in round 0 dense wave w also writes its window sums to rows w*SL + i.
+// Each lag is then added up by one wave in a fixed order.
+template <int LSM, int SL, int DN, int SM>
+__device__ __forceinline__ void hist_reduce(float2 *nb, float *accg, long long E, const float (&s)[SM],
+                                            const float (&h)[LSM], int w, int wd, int ws, int sd, int dense, int ls) {
+    static_assert(SL <= SM, "window sums exceed the partial registers");
+    float *red = reinterpret_cast<float *>(nb);
+    const int lane = threadIdx.x % RING_GROUP;
+    const int hb = wd * SL;
+#pragma unroll
+    for (int r = 0; r < LSM / 32; ++r) {
+        if (r * 32 >= ls) break;
+        __syncthreads();
+        if (w < wd) {
+            if (r == 0) {
+#pragma unroll
+                for (int i = 0; i < SL; ++i) red[(w * SL + i) * RING_GROUP + lane] = s[i];
+            }
+        } else {
+            const int k = w - wd;
+#pragma unroll
+            for (int i = 0; i < 32; ++i)
+                if (r * 32 + i < ls) red[(hb + k * 32 + i) * RING_GROUP + lane] = h[r * 32 + i];
+        }
+        __syncthreads();
+        const int nd = r == 0 ? dense : 0, nh = min(32, ls - r * 32);
+        for (int it = w; it < nd + nh; it += LAG_WAVES) {
+            float t = 0.f;
+            int l;
+            if (it < nd) {
+                const int c = it / DN, i = it % DN;
+                for (int seg = 0; seg < sd; ++seg) t += red[((c * sd + seg) * SL + i) * RING_GROUP + lane];
+                l = it;
+            } else {
+                const int i = it - nd;
+                for (int k = 0; k < ws; ++k) t += red[(hb + k * 32 + i) * RING_GROUP + lane];
+                l = dense + r * 32 + i;
+            }
+            accg[(long long)l * E] += t;
+        }
+    }
+}
+
 // E: ring / accumulator row stride (padded), R: ring slots, slot0: slot of
 // t0, dmax = min(max lag, t0): oldest history frame the block may pair with.
 //
@@ -199,13 +303,17 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
         run = miss ? __builtin_ctzll(miss) : RING_GROUP;
         run = __builtin_amdgcn_readfirstlane(run);
     }
-    const int dense_lags = (SLOTS >= DENSE && run >= DENSE / 2) ? min(run, MAX_DENSE) : 0;
+    int dense_lags = (SLOTS >= DENSE && run >= DENSE / 2) ? min(run, MAX_DENSE) : 0;
+    if (C::HIST_LAGS && dense_lags > DENSE && dense_lags % DENSE < DENSE / 2)
+        dense_lags -= dense_lags % DENSE;  // a short last window costs more than its lags elsewhere
     const int nchunks = (dense_lags + DENSE - 1) / DENSE;
     const int ls = L - dense_lags;
+    const bool hist = C::HIST_LAGS && ls > 0 && ls <= C::HIST_LAGS;
     int wd = 0;  // dense waves
     if (nchunks) {
-        // a dense pair costs about 3/5 of a pair of the other lags
-        const int wdense = 3 * dense_lags, wother = 5 * ls;
+        // relative pair costs: the window computes whole chunks of DENSE lags
+        const int wdense = hist ? COST_DENSE * nchunks * DENSE : 3 * dense_lags;
+        const int wother = hist ? COST_HIST * ls : 5 * ls;
         wd = (LAG_WAVES * wdense + (wdense + wother) / 2) / (wdense + wother);
         wd = max(wd, nchunks);
         wd = min(wd, ls ? LAG_WAVES - 1 : LAG_WAVES);
@@ -214,17 +322,22 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
     const int sd = nchunks ? wd / nchunks : 1;
     const int ws = LAG_WAVES - wd;
     int ss = 0;  // segments per other lag; 0: one wave per lag, direct accumulation
-    for (int cand = NSEG; cand >= 1; cand /= 2)
-        if (ls * cand <= SLOTS * ws) {
-            ss = cand;
-            break;
-        }
+    if (!hist)
+        for (int cand = NSEG; cand >= 1; cand /= 2)
+            if (ls * cand <= SLOTS * ws) {
+                ss = cand;
+                break;
+            }
     __syncthreads();
     const float2 *nl = nb + lane;
     float *accg = acc + (long long)blockIdx.x * RING_GROUP + lane;
     float s[C::SLOTS_MAX];
 #pragma unroll
     for (int i = 0; i < C::SLOTS_MAX; ++i) s[i] = 0.f;
+    constexpr int HL = C::HIST_LAGS > 0 ? C::HIST_LAGS : 1;
+    float h[HL];  // history-major lag sums
+#pragma unroll
+    for (int i = 0; i < HL; ++i) h[i] = 0.f;
     if (w < wd) {
         const int c = w / sd, seg = w % sd;
         const int la = c * DENSE, kc = min(DENSE, dense_lags - la);
@@ -241,6 +354,9 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
                     if (i < kc) s[i] = lag_range_sum<U>(g, nl, R, slot0, dmax, tau_a + i, j0, j1);
             }
         }
+    } else if (hist) {
+        const int vtau = lags[dense_lags + (lane < ls ? lane : 0)];
+        hist_sums<C::HIST_BLOCK>(g, nl, R, slot0, n, dmax, vtau, ls, w - wd, ws, h);
     } else if (ss) {
         const int k = w - wd, units = ls * ss, len = (n + ss - 1) / ss;
 #pragma unroll 1
@@ -255,6 +371,12 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
     } else {
         for (int l = dense_lags + (w - wd); l < L; l += ws)
             accg[(long long)l * E] += lag_range_sum<U>(g, nl, R, slot0, dmax, lags[l], 0, n);
+    }
+    if constexpr (C::HIST_LAGS > 0) {
+        if (hist) {
+            hist_reduce<C::HIST_LAGS, SLOTS, DENSE>(nb, accg, E, s, h, w, wd, ws, sd, dense_lags, ls);
+            return;
+        }
     }
     if (wd == 0 && ss == 0) return;
     // ---- combine the partials: [wave][slot][lane] in the staging buffer ----

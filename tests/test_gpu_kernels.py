@@ -171,7 +171,8 @@ def _lag_expected(ring, R, t0, n_new, lags):
 def test_lag_engine_bit_exact(gpu, J, t0, n_new, nlags):
     """Integer spectra: every |dF|^2 and every partial sum is an integer below
     2^24, so FP32 gives the exact answer whatever the summation order.
-    40 lags use the 64-register variant, 70 lags two passes."""
+    With 256-frame blocks, 10 lags take the history-major path and 40 or 70
+    the per-lag units."""
     import torch
     n_new = J if n_new is None else n_new
     base = [0, 1, 2, 5, 15, 16, 17, 31, 33, 40, 63, 64, 65, 127, 128, 129, 150]
@@ -202,6 +203,10 @@ DENSE_LAG_SETS = {
     "zero4": [0, 1, 2, 3, 9],                 # shortest run that is dense, lag 0 included
     "short3": [1, 2, 3, 50],                  # run too short: all single
     "run16": list(range(2, 18)),               # exactly one full chunk
+    # 33-run: 32 window lags, the 33rd joins the 31 log lags, 32 history-major lags in all
+    "log64": list(range(1, 34)) + [37, 42, 46, 52, 58, 64, 72, 80, 90, 100, 112, 124, 139, 155, 173, 193, 215,
+                                   240, 268, 299, 334, 372, 416, 464, 517, 577, 644, 719, 802, 895, 999],
+    "sparse33": [3 * k + 1 for k in range(33)],  # no run, one lag too many for the history-major path
 }
 
 
