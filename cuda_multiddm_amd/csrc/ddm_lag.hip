@@ -23,6 +23,10 @@ namespace ddm {
 constexpr int LAG_NT = 1024;                    // threads per workgroup: 16 waves
 constexpr int LAG_WAVES = LAG_NT / RING_GROUP;  // waves sharing one ring group
 constexpr int DENSE_SEG = 32;                   // frames per segment of a lag's units
+// Word 3 of a raw (untyped, stride 0) buffer resource on gfx9-family parts:
+// a valid 32-bit data format, so dword loads return memory.
+constexpr int BUFFER_RSRC_WORD3 = 0x00020000;
+constexpr int DENSE_BATCH = 4;                  // register-window steps scheduled together
 
 // Per block size: J = 256 stages 128 KB of new frames, so one workgroup (4
 // waves per SIMD) runs per CU and each wave has room for a wider window,
@@ -39,26 +43,66 @@ struct LagCfg {
     // register sum each, over blocks of HIST_BLOCK history frames
     static constexpr int HIST_LAGS = WIDE ? 32 : 0;
     static constexpr int HIST_BLOCK = 16;
+    static constexpr bool HIST_PREFETCH = false;
 };
 
 // Relative cost of a pair in the register window and in the history-major
 // path, for the split of the waves.
-constexpr int COST_DENSE = 4, COST_HIST = 5;
+constexpr int COST_DENSE = 4, COST_HIST = 9;
 
-// sum_{k < count} |a[k*G] - b[k*G]|^2, G = RING_GROUP (consecutive ring slots
-// or consecutive LDS rows are RING_GROUP complex values apart).  Loads come
+// The R ring slots of one ring group, read with buffer loads: the lane's
+// offset in the vector operand, the wave-uniform slot in the scalar one.  The
+// buffer's record count bounds every read to the group's own slots, and LDS
+// and ring reads stay distinct instructions (no generic-address loads).
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+// The LLVM intrinsic itself: this compiler's raw_buffer_load_b64 builtin
+// lowers to a single-dword load.
+__device__ f32x2_t raw_buffer_load_f32x2(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset, int aux) __asm(
+    "llvm.amdgcn.raw.ptr.buffer.load.v2f32");
+
+struct GroupRing {
+    __amdgpu_buffer_rsrc_t rs;
+    int voff;  // lane * sizeof(float2)
+    static constexpr int SLOT_BYTES = RING_GROUP * sizeof(float2);
+    // slot + k, k a compile-time constant: k % 8 goes into the instruction's
+    // 12-bit offset, the rest into the scalar offset, so runs of loads share
+    // one or two scalar registers
+    __device__ __forceinline__ float2 load(int slot, int k) const {
+        const f32x2_t v = raw_buffer_load_f32x2(rs, voff + (k & 7) * SLOT_BYTES, (slot + (k & ~7)) * SLOT_BYTES, 0);
+        return make_float2(v.x, v.y);
+    }
+    __device__ __forceinline__ float2 operator[](int slot) const { return load(slot, 0); }
+};
+
+// Row k of a run of consecutive frames: ring slots from `slot` on, or LDS
+// rows RING_GROUP complex values apart.
+struct RingRun {
+    GroupRing g;
+    int slot;
+    __device__ __forceinline__ float2 operator[](int k) const { return g.load(slot, k); }
+    __device__ __forceinline__ RingRun shift(int k) const { return RingRun{g, slot + k}; }
+};
+struct LdsRun {
+    const float2 *p;
+    __device__ __forceinline__ float2 operator[](int k) const { return p[k * RING_GROUP]; }
+    __device__ __forceinline__ LdsRun shift(int k) const { return LdsRun{p + k * RING_GROUP}; }
+};
+
+// sum_{k < count} |a[k] - b[k]|^2 over two runs of frames.  Loads come
 // in batches of U pairs; each pair costs two subtractions and two fused
 // multiply-adds into one of four independent partial sums.
-template <int U>
-__device__ __forceinline__ float run_sum(const float2 *a, const float2 *b, int count) {
+template <int U, class A>
+__device__ __forceinline__ float run_sum(A a, LdsRun b, int count) {
     float s[4] = {0.f, 0.f, 0.f, 0.f};
     int k = 0;
     for (; k + U <= count; k += U) {
+        const A ak = a.shift(k);
+        const LdsRun bk = b.shift(k);
         float2 va[U], vb[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            va[u] = a[(k + u) * RING_GROUP];
-            vb[u] = b[(k + u) * RING_GROUP];
+            va[u] = ak[u];
+            vb[u] = bk[u];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -68,7 +112,7 @@ __device__ __forceinline__ float run_sum(const float2 *a, const float2 *b, int c
         }
     }
     for (; k < count; ++k) {
-        const float2 va = a[k * RING_GROUP], vb = b[k * RING_GROUP];
+        const float2 va = a.shift(k)[0], vb = b.shift(k)[0];
         const float dx = va.x - vb.x, dy = va.y - vb.y;
         s[0] = fmaf(dx, dx, s[0]);
         s[0] = fmaf(dy, dy, s[0]);
@@ -80,7 +124,7 @@ __device__ __forceinline__ float run_sum(const float2 *a, const float2 *b, int c
 // partner is valid when j - tau >= -dmax; it is in the ring history for
 // j < tau and in LDS for j >= tau.
 template <int U>
-__device__ __forceinline__ float lag_range_sum(const float2 *g, const float2 *nl, int R, int slot0, int dmax,
+__device__ __forceinline__ float lag_range_sum(const GroupRing &g, const float2 *nl, int R, int slot0, int dmax,
                                                int tau, int ja, int jb) {
     float sum = 0.f;
     const int jh0 = max(tau - dmax, ja), jh1 = min(tau, jb);
@@ -88,12 +132,12 @@ __device__ __forceinline__ float lag_range_sum(const float2 *g, const float2 *nl
         int jw = tau - slot0;  // j < jw: the partner slot wrapped
         jw = jw < jh0 ? jh0 : (jw > jh1 ? jh1 : jw);
         if (jh0 < jw)
-            sum += run_sum<U>(g + (long long)(slot0 + jh0 - tau + R) * RING_GROUP, nl + jh0 * RING_GROUP, jw - jh0);
+            sum += run_sum<U>(RingRun{g, slot0 + jh0 - tau + R}, LdsRun{nl + jh0 * RING_GROUP}, jw - jh0);
         if (jw < jh1)
-            sum += run_sum<U>(g + (long long)(slot0 + jw - tau) * RING_GROUP, nl + jw * RING_GROUP, jh1 - jw);
+            sum += run_sum<U>(RingRun{g, slot0 + jw - tau}, LdsRun{nl + jw * RING_GROUP}, jh1 - jw);
     }
     const int ji = max(tau, ja);
-    if (ji < jb) sum += run_sum<U>(nl + (ji - tau) * RING_GROUP, nl + ji * RING_GROUP, jb - ji);
+    if (ji < jb) sum += run_sum<U>(LdsRun{nl + (ji - tau) * RING_GROUP}, LdsRun{nl + ji * RING_GROUP}, jb - ji);
     return sum;
 }
 
@@ -101,7 +145,7 @@ __device__ __forceinline__ float lag_range_sum(const float2 *g, const float2 *nl
 // Frames of valid pairs are at most max lag <= R - n back, one wrap away;
 // older ones (read only for discarded lags) are clamped to slot 0 so the
 // load stays inside the ring.
-__device__ __forceinline__ float2 window_frame(const float2 *g, const float2 *nl, int R, int slot0, int r) {
+__device__ __forceinline__ float2 window_frame(const GroupRing &g, const float2 *nl, int R, int slot0, int r) {
     float2 v;
     if (r >= 0) {
         v = nl[r * RING_GROUP];
@@ -109,9 +153,30 @@ __device__ __forceinline__ float2 window_frame(const float2 *g, const float2 *nl
         int s = slot0 + r;
         s += (s < 0) ? R : 0;
         s = s < 0 ? 0 : s;
-        v = g[(long long)s * RING_GROUP];
+        v = g[s];
     }
     return v;
+}
+
+// D steps of the register window: step u loads the entering partner pb[u]
+// and the new frame at LDS row u of pa, and forms D pairs.
+template <int D, int S, class A>
+__device__ __forceinline__ void dense_block(A pb, const float2 *pa, float2 (&win)[D], float (&s)[S]) {
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+        win[u] = pb[u];
+        const float2 a = pa[u * RING_GROUP];
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            const float2 b = win[(u - i) & (D - 1)];
+            const float dx = a.x - b.x, dy = a.y - b.y;
+            s[i] = fmaf(dx, dx, s[i]);
+            s[i] = fmaf(dy, dy, s[i]);
+        }
+        // loads of at most DENSE_BATCH steps ahead: keeps the window within
+        // the register budget
+        if ((u + 1) % DENSE_BATCH == 0) __builtin_amdgcn_sched_barrier(0);
+    }
 }
 
 // D consecutive lags tau_a .. tau_a + D - 1 over new frames j in [j0, j1).
@@ -122,7 +187,7 @@ __device__ __forceinline__ float2 window_frame(const float2 *g, const float2 *nl
 // constant.  Sums of lags beyond the caller's count come from in-bounds
 // (possibly stale) slots and are discarded by the caller.
 template <int D, int S>
-__device__ __forceinline__ void dense_sums(const float2 *g, const float2 *nl, int R, int slot0, int tau_a, int j0,
+__device__ __forceinline__ void dense_sums(const GroupRing &g, const float2 *nl, int R, int slot0, int tau_a, int j0,
                                            int j1, float (&s)[S]) {
     static_assert(D <= S && (D & (D - 1)) == 0, "window must be a power of two within the partial sums");
     float2 win[D];
@@ -131,7 +196,18 @@ __device__ __forceinline__ void dense_sums(const float2 *g, const float2 *nl, in
 #pragma unroll
     for (int i = 0; i < D; ++i) s[i] = 0.f;
     for (int jb = j0; jb < j1; jb += D) {
-        const int cnt = j1 - jb;
+        const int cnt = j1 - jb, r0 = jb - tau_a;  // r0: entering partner of step 0
+        // A full block of D steps whose entering partners all sit in LDS, or
+        // all in the ring history without a wrap, reads them from one base
+        // with constant offsets; other blocks go frame by frame.
+        if (cnt >= D && r0 >= 0) {
+            dense_block<D>(LdsRun{nl + r0 * RING_GROUP}, nl + jb * RING_GROUP, win, s);
+            continue;
+        }
+        if (cnt >= D && r0 + D <= 0 && slot0 + r0 >= 0) {
+            dense_block<D>(RingRun{g, slot0 + r0}, nl + jb * RING_GROUP, win, s);
+            continue;
+        }
 #pragma unroll
         for (int u = 0; u < D; ++u) {
             if (u < cnt) {
@@ -149,52 +225,104 @@ __device__ __forceinline__ void dense_sums(const float2 *g, const float2 *nl, in
     }
 }
 
+// History frames p0 .. p0 + B - 1 of the window, clamped to pmax (frames
+// beyond it pair with no new frame).
+template <int B>
+__device__ __forceinline__ void hist_load(const GroupRing &g, const float2 *nl, int R, int slot0, int p0, int pmax,
+                                          float2 (&fp)[B]) {
+    const int pe = p0 + B - 1;  // last frame of the block
+    if (pe <= pmax) {
+        // whole block in LDS, or in the history as one run of slots
+        int base = slot0 + p0;
+        base += (base < 0) ? R : 0;
+        if (p0 >= 0) {
+#pragma unroll
+            for (int u = 0; u < B; ++u) fp[u] = nl[(p0 + u) * RING_GROUP];
+            return;
+        }
+        if (pe < 0 && base + B <= R) {
+#pragma unroll
+            for (int u = 0; u < B; ++u) fp[u] = g.load(base, u);
+            return;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) fp[u] = window_frame(g, nl, R, slot0, min(p0 + u, pmax));
+}
+
+// Pairs of history frames p0 + u (in fp) with the new frames p0 + u + tau_l
+// inside [0, n), for every lag that reaches the block: -p0 - B < tau < n - p0.
+template <int B, int LSM>
+__device__ __forceinline__ void hist_block(const float2 *nl, const float2 (&fp)[B], int p0, int n, int vtau, bool live,
+                                           float (&s)[LSM]) {
+    const int la = __builtin_popcountll(__ballot(live && vtau <= -p0 - B));
+    const int lb = __builtin_popcountll(__ballot(live && vtau < n - p0));
+    for (int l = la; l < lb; ++l) {
+        const int j0 = p0 + __builtin_amdgcn_readlane(vtau, l);
+        const float2 *a = nl + j0 * RING_GROUP;
+        float t[4] = {0.f, 0.f, 0.f, 0.f};
+        if (j0 >= 0 && j0 + B <= n) {
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                const float2 v = a[u * RING_GROUP];
+                const float dx = v.x - fp[u].x, dy = v.y - fp[u].y;
+                t[u & 3] = fmaf(dx, dx, t[u & 3]);
+                t[u & 3] = fmaf(dy, dy, t[u & 3]);
+            }
+        } else {
+            const int ua = -j0, ub = n - j0;  // valid u: ua <= u < ub
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                if (u >= ua && u < ub) {
+                    const float2 v = a[u * RING_GROUP];
+                    const float dx = v.x - fp[u].x, dy = v.y - fp[u].y;
+                    t[u & 3] = fmaf(dx, dx, t[u & 3]);
+                    t[u & 3] = fmaf(dy, dy, t[u & 3]);
+                }
+            }
+        }
+        s[l] += (t[0] + t[1]) + (t[2] + t[3]);
+    }
+}
+
 // Lags tau_0 < ... < tau_{ls-1} (lane l of `vtau` holds tau_l) by history
 // frame: frame p of the window (ring history for p < 0, LDS for p >= 0)
 // pairs with the new frames j = p + tau_l inside [0, n).  The wave takes
 // blocks of B consecutive p (block b = k, k + ws, ...), loads each block once
 // into registers and pairs it with the LDS rows of every lag that reaches it,
 // so a history frame is requested once per wave rather than once per lag.
+// With PF the next block's loads are issued before the current block's pairs.
 // s[l] collects lag l in a fixed block order.
-template <int B, int LSM>
-__device__ __forceinline__ void hist_sums(const float2 *g, const float2 *nl, int R, int slot0, int n, int dmax,
+template <int B, bool PF, int LSM>
+__device__ __forceinline__ void hist_sums(const GroupRing &g, const float2 *nl, int R, int slot0, int n, int dmax,
                                           int vtau, int ls, int k, int ws, float (&s)[LSM]) {
     const int tmin = __builtin_amdgcn_readlane(vtau, 0), tmax = __builtin_amdgcn_readlane(vtau, ls - 1);
     const int pmin = max(-dmax, -tmax), pmax = n - 1 - tmin;
-    const int lane = threadIdx.x % RING_GROUP;
-    const bool live = lane < ls;
-    for (int p0 = pmin + k * B; p0 <= pmax; p0 += ws * B) {
-        float2 fp[B];
-#pragma unroll
-        for (int u = 0; u < B; ++u) fp[u] = window_frame(g, nl, R, slot0, min(p0 + u, pmax));
-        // the lags reaching the block: -p0 - B < tau < n - p0
-        const int la = __builtin_popcountll(__ballot(live && vtau <= -p0 - B));
-        const int lb = __builtin_popcountll(__ballot(live && vtau < n - p0));
-        for (int l = la; l < lb; ++l) {
-            const int j0 = p0 + __builtin_amdgcn_readlane(vtau, l);
-            const float2 *a = nl + j0 * RING_GROUP;
-            float t[4] = {0.f, 0.f, 0.f, 0.f};
-            if (j0 >= 0 && j0 + B <= n) {
-#pragma unroll
-                for (int u = 0; u < B; ++u) {
-                    const float2 v = a[u * RING_GROUP];
-                    const float dx = v.x - fp[u].x, dy = v.y - fp[u].y;
-                    t[u & 3] = fmaf(dx, dx, t[u & 3]);
-                    t[u & 3] = fmaf(dy, dy, t[u & 3]);
-                }
-            } else {
-                const int ua = -j0, ub = n - j0;  // valid u: ua <= u < ub
-#pragma unroll
-                for (int u = 0; u < B; ++u) {
-                    if (u >= ua && u < ub) {
-                        const float2 v = a[u * RING_GROUP];
-                        const float dx = v.x - fp[u].x, dy = v.y - fp[u].y;
-                        t[u & 3] = fmaf(dx, dx, t[u & 3]);
-                        t[u & 3] = fmaf(dy, dy, t[u & 3]);
-                    }
-                }
-            }
-            s[l] += (t[0] + t[1]) + (t[2] + t[3]);
+    const bool live = (int)(threadIdx.x % RING_GROUP) < ls;
+    const int step = ws * B;
+    int p0 = pmin + k * B;
+    if (p0 > pmax) return;
+    float2 fa[B];
+    hist_load<B>(g, nl, R, slot0, p0, pmax, fa);
+    if constexpr (!PF) {
+        for (;;) {
+            hist_block<B>(nl, fa, p0, n, vtau, live, s);
+            p0 += step;
+            if (p0 > pmax) break;
+            hist_load<B>(g, nl, R, slot0, p0, pmax, fa);
+        }
+    } else {
+        float2 fb[B];
+        for (;;) {
+            const int p1 = p0 + step;
+            if (p1 <= pmax) hist_load<B>(g, nl, R, slot0, p1, pmax, fb);
+            hist_block<B>(nl, fa, p0, n, vtau, live, s);
+            if (p1 > pmax) break;
+            const int p2 = p1 + step;
+            if (p2 <= pmax) hist_load<B>(g, nl, R, slot0, p2, pmax, fa);
+            hist_block<B>(nl, fb, p1, n, vtau, live, s);
+            if (p2 > pmax) break;
+            p0 = p2;
         }
     }
 }
@@ -274,7 +402,10 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
     const int tid = threadIdx.x;
     const int lane = tid % RING_GROUP;
     const int w = __builtin_amdgcn_readfirstlane(tid / RING_GROUP);
-    const float2 *g = ring + (long long)blockIdx.x * R * RING_GROUP + lane;  // slot s at g[s * RING_GROUP]
+    GroupRing g;
+    g.rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<float2 *>(ring + (long long)blockIdx.x * R * RING_GROUP), 0,
+                                             R * (int)(RING_GROUP * sizeof(float2)), BUFFER_RSRC_WORD3);
+    g.voff = lane * (int)sizeof(float2);
     {
         // stage the new frames: all loads in flight before the LDS stores
         constexpr int K = (J + LAG_WAVES - 1) / LAG_WAVES;
@@ -285,7 +416,7 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
             if (j < n) {
                 int s = slot0 + j;
                 s -= (s >= R) ? R : 0;
-                v[k] = g[(long long)s * RING_GROUP];
+                v[k] = g[s];
             }
         }
 #pragma unroll
@@ -356,7 +487,7 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
         }
     } else if (hist) {
         const int vtau = lags[dense_lags + (lane < ls ? lane : 0)];
-        hist_sums<C::HIST_BLOCK>(g, nl, R, slot0, n, dmax, vtau, ls, w - wd, ws, h);
+        hist_sums<C::HIST_BLOCK, C::HIST_PREFETCH>(g, nl, R, slot0, n, dmax, vtau, ls, w - wd, ws, h);
     } else if (ss) {
         const int k = w - wd, units = ls * ss, len = (n + ss - 1) / ss;
 #pragma unroll 1
