@@ -308,8 +308,9 @@ static int flush_lags(ddm_engine *e) {
     return pending > 0 ? lag_block(e, (int)pending) : DDM_OK;
 }
 
-// Tile + FFT for n <= chunk frames at device address src, then the lag
-// engine over every full block of lag_block frames now waiting.
+// Tile + FFT for n frames at device address src (n <= the ring room left,
+// see batch_room), then the lag engine over every full block of lag_block
+// frames now waiting.
 static int process_chunk(ddm_engine *e, const uint8_t *src, int n, long long frame_bytes, int width,
                          int x_off, int y_off, hipEvent_t consumed) {
     const Params &p = e->p;
@@ -358,16 +359,35 @@ static int process_chunk(ddm_engine *e, const uint8_t *src, int n, long long fra
     return DDM_OK;
 }
 
+// Frames of device-resident input transformed per batch: whole chunks, up to
+// about FFT_BATCH_FRAMES.  A 30-frame chunk of one scale is too little work
+// to fill the device; ~128 frames is, while the 512-tile row spectra of a
+// batch (1 MB a frame) still fit in the last-level cache for the column pass.
+constexpr int FFT_BATCH_FRAMES = 128;
+
+// Frames the next batch may hold without overwriting a ring slot still
+// needed: frame t goes to slot t % R, and the slots of frames not yet lag
+// processed, with their max-lag history, must survive.  Since the ring holds
+// chunk + lag_block + max lag slots and fewer than lag_block frames wait for
+// the lag engine between batches, this is at least chunk frames.
+static int batch_room(const ddm_engine *e) {
+    const Params &p = e->p;
+    const long long waiting = e->frames - e->frames_lagged;
+    return (int)std::max<long long>(1, (long long)p.ring_slots - p.max_lag - waiting);
+}
+
 extern "C" int ddm_submit_device(ddm_engine *e, const void *frames, int n_frames) {
     if (!e) return DDM_EINVAL;
     if (n_frames < 0 || (n_frames > 0 && !frames)) return fail(e, DDM_EINVAL, "bad frame buffer");
     const Params &p = e->p;
     const long long fb = (long long)p.width * p.height * p.bpp;
     const uint8_t *src = (const uint8_t *)frames;
-    for (int i = 0; i < n_frames; i += p.chunk) {
-        const int n = std::min(p.chunk, n_frames - i);
+    const int batch = p.chunk * std::max(1, FFT_BATCH_FRAMES / p.chunk);
+    for (int i = 0; i < n_frames;) {
+        const int n = std::min({batch, n_frames - i, batch_room(e)});
         int rc = process_chunk(e, src + (long long)i * fb, n, fb, p.width, p.x_off, p.y_off, nullptr);
         if (rc) return rc;
+        i += n;
     }
     return DDM_OK;
 }

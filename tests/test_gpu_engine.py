@@ -88,6 +88,22 @@ def test_host_and_device_submission_identical(gpu):
         assert o.max_rel_err(d, ref) <= TOL
 
 
+@pytest.mark.parametrize("chunk,lb,lags", [
+    (7, 32, [1, 2, 3, 5, 9, 20, 60]),       # batch of 126 limited by the ring room (39 - waiting)
+    (30, 256, [1, 4, 17, 90, 200]),         # 120-frame batches, lag blocks straddling them
+    (13, 64, [1, 2, 3, 4, 5, 6, 7, 8, 40]),  # batches of 117, room 77 - waiting
+])
+def test_device_batches_match_chunked_host_submission(gpu, chunk, lb, lags):
+    """Device-resident frames are transformed several chunks per batch, as
+    many as the ring's free slots allow; the results equal a chunk-by-chunk
+    host submission bit for bit, and the oracle."""
+    frames = ddm.synth_uniform(300, 64, 64, 8, seed=chunk + lb)
+    a, ref = _both(frames, [64, 32], lags, [2.0, 5.0, 9.0], 1.2, chunk, lag_block=lb)
+    b, _ = _both(frames, [64, 32], lags, [2.0, 5.0, 9.0], 1.2, chunk, lag_block=lb, device_input=True)
+    assert np.array_equal(a, b)
+    assert o.max_rel_err(b, ref) <= TOL
+
+
 @pytest.mark.parametrize("w,h,x_off,y_off,bpp,fmt", [
     (96, 64, 32, 0, 1, ddm.PIX_U8),      # ROI rows of consecutive frames evenly spaced: one 2-D copy
     (80, 90, 8, 20, 2, ddm.PIX_U16LE),   # general ROI: one 2-D copy per frame
