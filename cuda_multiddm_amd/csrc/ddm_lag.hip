@@ -27,6 +27,7 @@ constexpr int DENSE_SEG = 32;                   // frames per segment of a lag's
 // a valid 32-bit data format, so dword loads return memory.
 constexpr int BUFFER_RSRC_WORD3 = 0x00020000;
 constexpr int DENSE_BATCH = 4;                  // register-window steps scheduled together
+constexpr int ACC_PREFETCH = 4;                 // accumulators each wave reads early
 
 // Per block size: J = 256 stages 128 KB of new frames, so one workgroup (4
 // waves per SIMD) runs per CU and each wave has room for a wider window,
@@ -43,7 +44,6 @@ struct LagCfg {
     // register sum each, over blocks of HIST_BLOCK history frames
     static constexpr int HIST_LAGS = WIDE ? 32 : 0;
     static constexpr int HIST_BLOCK = 16;
-    static constexpr bool HIST_PREFETCH = false;
 };
 
 // Relative cost of a pair in the register window and in the history-major
@@ -291,40 +291,51 @@ __device__ __forceinline__ void hist_block(const float2 *nl, const float2 (&fp)[
 // blocks of B consecutive p (block b = k, k + ws, ...), loads each block once
 // into registers and pairs it with the LDS rows of every lag that reaches it,
 // so a history frame is requested once per wave rather than once per lag.
-// With PF the next block's loads are issued before the current block's pairs.
 // s[l] collects lag l in a fixed block order.
-template <int B, bool PF, int LSM>
+template <int B, int LSM>
 __device__ __forceinline__ void hist_sums(const GroupRing &g, const float2 *nl, int R, int slot0, int n, int dmax,
                                           int vtau, int ls, int k, int ws, float (&s)[LSM]) {
     const int tmin = __builtin_amdgcn_readlane(vtau, 0), tmax = __builtin_amdgcn_readlane(vtau, ls - 1);
     const int pmin = max(-dmax, -tmax), pmax = n - 1 - tmin;
     const bool live = (int)(threadIdx.x % RING_GROUP) < ls;
-    const int step = ws * B;
     int p0 = pmin + k * B;
     if (p0 > pmax) return;
     float2 fa[B];
     hist_load<B>(g, nl, R, slot0, p0, pmax, fa);
-    if constexpr (!PF) {
-        for (;;) {
-            hist_block<B>(nl, fa, p0, n, vtau, live, s);
-            p0 += step;
-            if (p0 > pmax) break;
-            hist_load<B>(g, nl, R, slot0, p0, pmax, fa);
-        }
-    } else {
-        float2 fb[B];
-        for (;;) {
-            const int p1 = p0 + step;
-            if (p1 <= pmax) hist_load<B>(g, nl, R, slot0, p1, pmax, fb);
-            hist_block<B>(nl, fa, p0, n, vtau, live, s);
-            if (p1 > pmax) break;
-            const int p2 = p1 + step;
-            if (p2 <= pmax) hist_load<B>(g, nl, R, slot0, p2, pmax, fa);
-            hist_block<B>(nl, fb, p1, n, vtau, live, s);
-            if (p2 > pmax) break;
-            p0 = p2;
-        }
+    for (;;) {
+        hist_block<B>(nl, fa, p0, n, vtau, live, s);
+        p0 += ws * B;
+        if (p0 > pmax) break;
+        hist_load<B>(g, nl, R, slot0, p0, pmax, fa);
     }
+}
+
+// The accumulators of the first PF lags wave w adds (lags w, w + 16, ...),
+// requested when the workgroup starts so that they have arrived by the time
+// the partial sums are added.
+template <int PF>
+__device__ __forceinline__ void load_acc(const float *accg, long long E, int w, int nred, float (&pre)[PF]) {
+#pragma unroll
+    for (int m = 0; m < PF; ++m) {
+        const int l = w + m * LAG_WAVES;
+        pre[m] = l < nred ? accg[(long long)l * E] : 0.f;
+    }
+}
+
+// acc[l] += t for the m-th lag this wave adds (l = w + 16 m), from the early
+// read when there is one.
+template <int PF>
+__device__ __forceinline__ void add_acc(float *accg, long long E, int l, int m, float t, const float (&pre)[PF]) {
+    float a = 0.f;
+    bool have = false;
+#pragma unroll
+    for (int i = 0; i < PF; ++i)
+        if (i == m) {
+            a = pre[i];
+            have = true;
+        }
+    float *p = accg + (long long)l * E;
+    *p = (have ? a : *p) + t;
 }
 
 // Partial sums of a launch with history-major lags meet in the staging buffer
@@ -332,9 +343,10 @@ __device__ __forceinline__ void hist_sums(const GroupRing &g, const float2 *nl, 
 // r, history wave k writes its sums of lags 32r + i to rows wd*SL + 32k + i,
 // and in round 0 dense wave w also writes its window sums to rows w*SL + i.
 // Each lag is then added up by one wave in a fixed order.
-template <int LSM, int SL, int DN, int SM>
+template <int LSM, int SL, int DN, int SM, int PF>
 __device__ __forceinline__ void hist_reduce(float2 *nb, float *accg, long long E, const float (&s)[SM],
-                                            const float (&h)[LSM], int w, int wd, int ws, int sd, int dense, int ls) {
+                                            const float (&h)[LSM], int w, int wd, int ws, int sd, int dense, int ls,
+                                            const float (&pre)[PF]) {
     static_assert(SL <= SM, "window sums exceed the partial registers");
     float *red = reinterpret_cast<float *>(nb);
     const int lane = threadIdx.x % RING_GROUP;
@@ -368,7 +380,7 @@ __device__ __forceinline__ void hist_reduce(float2 *nb, float *accg, long long E
                 for (int k = 0; k < ws; ++k) t += red[(hb + k * 32 + i) * RING_GROUP + lane];
                 l = dense + r * 32 + i;
             }
-            accg[(long long)l * E] += t;
+            add_acc(accg, E, l, (l - w) / LAG_WAVES, t, pre);
         }
     }
 }
@@ -459,9 +471,14 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
                 ss = cand;
                 break;
             }
+    // the accumulators this wave will add to are requested beside the staging
+    // loads, so they have arrived when the partial sums are added
+    float *accg = acc + (long long)blockIdx.x * RING_GROUP + lane;
+    const int nred = hist ? L : (ss ? L : dense_lags);
+    float pre[ACC_PREFETCH];
+    load_acc(accg, E, w, nred, pre);
     __syncthreads();
     const float2 *nl = nb + lane;
-    float *accg = acc + (long long)blockIdx.x * RING_GROUP + lane;
     float s[C::SLOTS_MAX];
 #pragma unroll
     for (int i = 0; i < C::SLOTS_MAX; ++i) s[i] = 0.f;
@@ -487,7 +504,7 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
         }
     } else if (hist) {
         const int vtau = lags[dense_lags + (lane < ls ? lane : 0)];
-        hist_sums<C::HIST_BLOCK, C::HIST_PREFETCH>(g, nl, R, slot0, n, dmax, vtau, ls, w - wd, ws, h);
+        hist_sums<C::HIST_BLOCK>(g, nl, R, slot0, n, dmax, vtau, ls, w - wd, ws, h);
     } else if (ss) {
         const int k = w - wd, units = ls * ss, len = (n + ss - 1) / ss;
 #pragma unroll 1
@@ -505,7 +522,7 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
     }
     if constexpr (C::HIST_LAGS > 0) {
         if (hist) {
-            hist_reduce<C::HIST_LAGS, SLOTS, DENSE>(nb, accg, E, s, h, w, wd, ws, sd, dense_lags, ls);
+            hist_reduce<C::HIST_LAGS, SLOTS, DENSE>(nb, accg, E, s, h, w, wd, ws, sd, dense_lags, ls, pre);
             return;
         }
     }
@@ -517,7 +534,6 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
     for (int i = 0; i < SLOTS; ++i) red[(w * SLOTS + i) * RING_GROUP + lane] = s[i];
     __syncthreads();
     // one lag per wave and step, so every index below is wave-uniform
-    const int nred = ss ? L : dense_lags;
     for (int l = w; l < nred; l += LAG_WAVES) {
         float t = 0.f;
         if (l < dense_lags) {
@@ -534,7 +550,7 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
                 }
             }
         }
-        accg[(long long)l * E] += t;
+        add_acc(accg, E, l, (l - w) / LAG_WAVES, t, pre);
     }
 }
 
