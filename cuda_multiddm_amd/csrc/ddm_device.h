@@ -64,9 +64,11 @@ __device__ __forceinline__ TileOut tile_out(const RingDst &rd, int f, int tile, 
 }
 
 // Register-resident tile FFT (ddm_fft.hip) for the scales it covers;
-// returns false for other scales.
+// returns false for other scales.  xt (optional) holds the row spectra of
+// the 256 and 512 tiles between their two passes: n_frames * (M/s)^2 * s^2/2
+// complex values (see big_fft_scratch_elems).
 bool launch_tile_fft_reg(const PixSrc &ps, const RingDst &rd, int scale, int tps, int tiles, int n_frames,
-                         const float2 *tw, hipStream_t st);
+                         const float2 *tw, float2 *xt, hipStream_t st);
 
 // 64 x 64 / 128 x 128 tile DFT on the matrix cores with split 16-bit
 // operands (ddm_dft_mfma.hip); precision is DDM_PREC_BF16X2 or _FP16X2.

@@ -50,6 +50,8 @@ struct ddm_engine {
     int *d_qoff = nullptr, *d_qcnt = nullptr, *d_idx = nullptr;
     double *d_norm = nullptr, *d_out = nullptr;
     double *d_tau = nullptr, *d_fit = nullptr;  // allocated by the first ddm_fit_results
+    float *d_xt = nullptr;    // row spectra of 256 / 512 tiles between FFT passes
+    long long xt_elems = 0;   // complex values d_xt holds (grown on demand)
     uint8_t *d_stage[2] = {nullptr, nullptr};
     uint8_t *h_stage[2] = {nullptr, nullptr};
     bool stage_used[2] = {false, false};
@@ -149,7 +151,7 @@ static void release(ddm_engine *e) {
     // device buffers come from the stream-ordered pool: freeing them does not
     // synchronise the device (other engines keep running)
     void *dev[] = {e->d_ring, e->d_acc, e->d_lags,     e->d_az,       e->d_qoff,  e->d_qcnt, e->d_idx,
-                   e->d_norm, e->d_out, e->d_stage[0], e->d_stage[1], e->d_tau,   e->d_fit};
+                   e->d_norm, e->d_out, e->d_stage[0], e->d_stage[1], e->d_tau,   e->d_fit, e->d_xt};
     for (void *ptr : dev)
         if (ptr && e->st) (void)hipFreeAsync(ptr, e->st);
     if (e->st) (void)hipStreamSynchronize(e->st);
@@ -308,6 +310,25 @@ static int flush_lags(ddm_engine *e) {
     return pending > 0 ? lag_block(e, (int)pending) : DDM_OK;
 }
 
+// The between-pass buffer of the 256 / 512 tile FFTs, large enough for a
+// batch of n frames.  Grown on the engine's stream, so work already queued
+// on it keeps the old buffer until it is done.
+static int ensure_xt(ddm_engine *e, int n) {
+    long long need = 0;
+    for (const ddm::ScaleGeom &g : e->p.geom) need = std::max(need, ddm::big_fft_scratch_elems(e->p.M, g.s, n));
+    if (need <= e->xt_elems) return DDM_OK;
+    if (e->d_xt) {
+        HIP_TRY(e, hipFreeAsync(e->d_xt, e->st));
+        e->dev_bytes -= (size_t)e->xt_elems * 2 * sizeof(float);
+        e->d_xt = nullptr;
+        e->xt_elems = 0;
+    }
+    int rc = dev_alloc(e, &e->d_xt, (size_t)need * 2, "FFT pass buffer");
+    if (rc) return rc;
+    e->xt_elems = need;
+    return DDM_OK;
+}
+
 // Tile + FFT for n frames at device address src (n <= the ring room left,
 // see batch_room), then the lag engine over every full block of lag_block
 // frames now waiting.
@@ -317,6 +338,8 @@ static int process_chunk(ddm_engine *e, const uint8_t *src, int n, long long fra
     const long long t0 = e->frames;
     const int R = p.ring_slots;
     const int first_slot = (int)(t0 % R);
+    int rc = ensure_xt(e, n);
+    if (rc) return rc;
     Mark chunk, fft;
     mark_begin(e, &chunk, MK_CHUNK);
     mark_begin(e, &fft, MK_FFT);
@@ -338,6 +361,7 @@ static int process_chunk(ddm_engine *e, const uint8_t *src, int n, long long fra
         a.elems_per_frame = p.elems_padded;
         a.elem_off = g.elem_off;
         a.precision = e->precision;
+        a.xt = e->d_xt;
         HIP_TRY(e, ddm::launch_tile_fft(a, e->st));
         e->stats.fft_launches += (g.s >= 256) ? 2 : 1;
     }
@@ -716,7 +740,14 @@ extern "C" int ddm_k_tile_fft(const void *frames, int n_frames, long long frame_
     a.first_slot = (int)(first_frame % ring_slots);
     a.elems_per_frame = elems_per_frame;
     a.elem_off = elem_off;
-    return ddm::launch_tile_fft(a, (hipStream_t)stream) == hipSuccess ? DDM_OK : DDM_EDEVICE;
+    // the between-pass buffer of 256 / 512 tiles lives for this call only
+    // (stream-ordered allocation and release)
+    const hipStream_t st = (hipStream_t)stream;
+    const long long xt = ddm::big_fft_scratch_elems(largest_scale, scale, n_frames);
+    if (xt && hipMallocAsync((void **)&a.xt, (size_t)xt * 2 * sizeof(float), st) != hipSuccess) return DDM_ENOMEM;
+    const hipError_t rc = ddm::launch_tile_fft(a, st);
+    if (a.xt) (void)hipFreeAsync(a.xt, st);
+    return rc == hipSuccess ? DDM_OK : DDM_EDEVICE;
 }
 
 extern "C" int ddm_k_lag_accumulate(const float *ring, int ring_slots, long long elems_per_frame,

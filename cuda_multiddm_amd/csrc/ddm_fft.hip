@@ -470,10 +470,15 @@ __device__ __forceinline__ void big_fft(float2 (&v)[S / 16], float2 *seqbuf, int
         for (int k2 = 0; k2 < 16; ++k2) seqbuf[q + 16 * p + E1 * k2] = u[p][k2];
 }
 
-// Rows: 16 row pairs per workgroup; writes columns 0 .. H-1 of rows 2r, 2r+1
-// of the tile's half spectrum into the ring slot (column 0 = DC + i Nyquist).
+// Rows: 16 row pairs per workgroup; columns 0 .. H-1 of rows 2r, 2r+1 of the
+// tile's half spectrum (column 0 = DC + i Nyquist) go to the batch's
+// column-block buffer xt: frame f, tile t, column block b = kx / 16, row ky,
+// column kx % 16 at xt[(((f * tiles + t) * H / 16 + b) * S + ky) * 16 + kx % 16],
+// so the lanes of a row piece write 128 aligned bytes and the column pass
+// reads each block as one contiguous run.
 template <int S, int PIX>
-__global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg(PixSrc ps, RingDst rd, int tps, int tiles,
+__global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg(PixSrc ps, int tps, int tiles,
+                                                                   float2 *__restrict__ xt,
                                                                    const float2 *__restrict__ twg) {
     using C = BigCfg<S>;
     constexpr int H = C::H, E1 = C::E1, SEQ = C::SEQ, GROUPS = H / SEQ;
@@ -492,47 +497,31 @@ __global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg(PixSrc ps, R
     float2 *seqbuf = lds + sq * S;
     big_fft<S>(v, seqbuf, q, twg);
     // separate the row pair: lane q writes columns k = q + 16 m
-    const TileOut out = tile_out(rd, f, tile, (long long)S * (H + 1));
-    const long long ea = out.base + (long long)(2 * r) * (H + 1) + q;
-    const long long eb = ea + (H + 1);
-    const long long gstride = (long long)rd.ring_slots * RING_GROUP;
-    int la = (int)(ea & (RING_GROUP - 1)), lb = (int)(eb & (RING_GROUP - 1));
-    float2 *pa = rd.ring + ((ea >> RING_GROUP_LOG2) * rd.ring_slots + out.slot) * RING_GROUP + la;
-    float2 *pb = rd.ring + ((eb >> RING_GROUP_LOG2) * rd.ring_slots + out.slot) * RING_GROUP + lb;
+    float2 *xa = xt + ((long long)f * tiles + tile) * ((long long)S * H) + (2 * r) * 16 + q;
 #pragma unroll
     for (int m = 0; m < H / 16; ++m) {
         const int k = q + 16 * m;
         const float2 zk = seqbuf[k];
         const float2 zp = seqbuf[(S - k) & (S - 1)];
-        float2 xa = make_float2(0.5f * (zk.x + zp.x), 0.5f * (zk.y - zp.y));
-        float2 xb = make_float2(0.5f * (zk.y + zp.y), -0.5f * (zk.x - zp.x));
+        float2 a = make_float2(0.5f * (zk.x + zp.x), 0.5f * (zk.y - zp.y));
+        float2 b = make_float2(0.5f * (zk.y + zp.y), -0.5f * (zk.x - zp.x));
         if (k == 0) {
             const float2 zh = seqbuf[H];  // DC and Nyquist of both rows are real
-            xa = make_float2(zk.x, zh.x);
-            xb = make_float2(zk.y, zh.y);
+            a = make_float2(zk.x, zh.x);
+            b = make_float2(zk.y, zh.y);
         }
-        *pa = xa;
-        *pb = xb;
-        la += 16;
-        lb += 16;
-        pa += 16;
-        pb += 16;
-        if (la >= RING_GROUP) {
-            la -= RING_GROUP;
-            pa += gstride - RING_GROUP;
-        }
-        if (lb >= RING_GROUP) {
-            lb -= RING_GROUP;
-            pb += gstride - RING_GROUP;
-        }
+        xa[(long long)m * S * 16] = a;
+        xa[(long long)m * S * 16 + 16] = b;
     }
 }
 
-// Columns: 16 consecutive packed columns per workgroup, read from and
-// written back to the ring slot; the packed column is separated into the
-// DC (kx = 0) and Nyquist (kx = H) columns on the way out.
+// Columns: 16 consecutive packed columns per workgroup, read from their
+// contiguous column block of xt and written to the ring slot; the packed
+// column is separated into the DC (kx = 0) and Nyquist (kx = H) columns on
+// the way out.
 template <int S>
 __global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_cols_reg(RingDst rd, int tiles,
+                                                                   const float2 *__restrict__ xt,
                                                                    const float2 *__restrict__ twg) {
     using C = BigCfg<S>;
     constexpr int H = C::H, E1 = C::E1, SEQ = C::SEQ, NB = H / SEQ, NTH = C::NTH;
@@ -550,8 +539,9 @@ __global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_cols_reg(RingDst rd, 
         // coalesced: 16 rows x 16 consecutive columns per pass
         constexpr int K = S * SEQ / NTH;
         float2 v[K];
+        const float2 *xb = xt + (((long long)f * tiles + tile) * (H / 16) + cb) * ((long long)S * 16) + sq * 16 + q;
 #pragma unroll
-        for (int k = 0; k < K; ++k) v[k] = t[(long long)(sq + 16 * k) * (H + 1) + c0 + q];
+        for (int k = 0; k < K; ++k) v[k] = xb[k * 256];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int y = sq + 16 * k;
@@ -589,27 +579,35 @@ __global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_cols_reg(RingDst rd, 
 
 template <int S>
 static void launch_big(const PixSrc &ps, const RingDst &rd, int tps, int tiles, int n_frames, const float2 *tw,
-                       hipStream_t st) {
+                       float2 *xt, hipStream_t st) {
     using C = BigCfg<S>;
     const dim3 g1((unsigned)((C::H / C::SEQ) * tiles * n_frames)), block(C::NTH);
     switch (pixel_mode(ps)) {
-    case PIX_U8_1: hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_U8_1>), g1, block, 0, st, ps, rd, tps, tiles, tw); break;
-    case PIX_U16LE_2:
-        hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_U16LE_2>), g1, block, 0, st, ps, rd, tps, tiles, tw);
+    case PIX_U8_1:
+        hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_U8_1>), g1, block, 0, st, ps, tps, tiles, xt, tw);
         break;
-    default: hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_ANY>), g1, block, 0, st, ps, rd, tps, tiles, tw);
+    case PIX_U16LE_2:
+        hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_U16LE_2>), g1, block, 0, st, ps, tps, tiles, xt, tw);
+        break;
+    default: hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_ANY>), g1, block, 0, st, ps, tps, tiles, xt, tw);
     }
     const dim3 g2((unsigned)((C::H / C::SEQ) * tiles * n_frames));
-    hipLaunchKernelGGL(tile_fft_cols_reg<S>, g2, block, 0, st, rd, tiles, tw);
+    hipLaunchKernelGGL(tile_fft_cols_reg<S>, g2, block, 0, st, rd, tiles, xt, tw);
 }
 
 bool launch_tile_fft_reg(const PixSrc &ps, const RingDst &rd, int scale, int tps, int tiles, int n_frames,
-                         const float2 *tw, hipStream_t st) {
+                         const float2 *tw, float2 *xt, hipStream_t st) {
     switch (scale) {
     case 64: launch_reg<64>(ps, rd, tps, tiles, n_frames, tw, st); return true;
     case 128: launch_reg<128>(ps, rd, tps, tiles, n_frames, tw, st); return true;
-    case 256: launch_big<256>(ps, rd, tps, tiles, n_frames, tw, st); return true;
-    case 512: launch_big<512>(ps, rd, tps, tiles, n_frames, tw, st); return true;
+    case 256:
+        if (!xt) return false;
+        launch_big<256>(ps, rd, tps, tiles, n_frames, tw, xt, st);
+        return true;
+    case 512:
+        if (!xt) return false;
+        launch_big<512>(ps, rd, tps, tiles, n_frames, tw, xt, st);
+        return true;
     default: return false;
     }
 }

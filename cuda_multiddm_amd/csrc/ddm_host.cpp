@@ -129,7 +129,11 @@ size_t device_footprint(const Params &p) {
     bytes += 2 * (size_t)p.chunk * p.M * p.M * p.bpp;                  // upload staging
     bytes += (size_t)p.result_count * 8 + L * 12;                      // results, lags, norms
     for (const ScaleGeom &g : p.geom) bytes += Q * ((size_t)g.tile_elems * 4 + 8);  // ring tables (bound)
-    return bytes;
+    // between-pass buffer of 256 / 512 tile FFTs for one chunk (host input;
+    // device input batches several chunks and grows it)
+    size_t xt = 0;
+    for (const ScaleGeom &g : p.geom) xt = std::max(xt, (size_t)big_fft_scratch_elems(p.M, g.s, p.chunk) * 8);
+    return bytes + xt;
 }
 
 // Member lists of one (scale, lambda, tolerance) ring, cached for the life of

@@ -22,6 +22,13 @@ constexpr int RING_GROUP_LOG2 = 6;
 constexpr int RING_GROUP = 1 << RING_GROUP_LOG2;
 inline long long pad_elems(long long e) { return (e + RING_GROUP - 1) / RING_GROUP * RING_GROUP; }
 
+// Complex values of the tile FFT's between-pass buffer (TileFftArgs::xt) a
+// batch of n frames needs for tiles of size `scale` cut from an M x M region
+// (0 for scales that do not use it).
+inline long long big_fft_scratch_elems(int M, int scale, int n) {
+    return (scale == 256 || scale == 512) ? (long long)n * M * M / 2 : 0;
+}
+
 // Frames per lag-engine pass when the descriptor leaves it to the engine.
 constexpr int kDefaultLagBlock = 256;
 // Lag values above this are rejected (ring memory would be absurd).

@@ -21,7 +21,12 @@ struct TileFftArgs {
     long long elems_per_frame;
     long long elem_off;        // offset of this scale within a frame's spectrum
     int precision = DDM_PREC_FP32;  // enum ddm_precision (64 / 128 tiles only)
+    // optional buffer for the row spectra of 256 / 512 tiles between their
+    // row and column passes (big_fft_scratch_elems values); without it the
+    // row spectra go through the ring slot
+    float *xt = nullptr;
 };
+
 
 struct LagArgs {
     const float *ring;

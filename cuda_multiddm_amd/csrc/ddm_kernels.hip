@@ -456,7 +456,8 @@ hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st) {
     const int tps = a.largest_scale / a.scale, tiles = tps * tps;
     if (a.precision != DDM_PREC_FP32 && (a.scale == 64 || a.scale == 128))
         return launch_tile_dft_mfma(ps, rd, a.scale, tps, tiles, a.n_frames, a.precision, st);
-    if (launch_tile_fft_reg(ps, rd, a.scale, tps, tiles, a.n_frames, tw, st)) return hipGetLastError();
+    if (launch_tile_fft_reg(ps, rd, a.scale, tps, tiles, a.n_frames, tw, (float2 *)a.xt, st))
+        return hipGetLastError();
     switch (a.scale) {
     case 2: launch_small<2>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
     case 4: launch_small<4>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
