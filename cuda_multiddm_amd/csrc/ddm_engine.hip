@@ -524,6 +524,18 @@ static int compute_results(ddm_engine *e) {
     a.lag_count = (int)L;
     a.out = e->d_out;
     a.result_count = p.result_count;
+    // scales come largest first; those with a ring of AZ_WIDE_RING elements
+    // or more take a workgroup per value
+    a.wide_items = 0;
+    {
+        const size_t Q = p.lambdas.size();
+        for (size_t si = 0; si < p.geom.size(); ++si) {
+            int big = 0;
+            for (size_t q = 0; q < Q; ++q) big = std::max(big, e->rt.cnt[si * Q + q]);
+            if (big < ddm::AZ_WIDE_RING) break;
+            a.wide_items += (long long)p.geom[si].tiles * (long long)Q * (long long)L;
+        }
+    }
     Mark az;
     mark_begin(e, &az, MK_AZ);
     HIP_TRY(e, ddm::launch_azimuthal(a, e->st));

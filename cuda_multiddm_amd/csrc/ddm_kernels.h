@@ -58,7 +58,13 @@ struct AzArgs {
     int lambda_count, lag_count;
     double *out;               // device [result_count]
     long long result_count;
+    // values [0, wide_items) (the scales with large rings, which come first)
+    // take a workgroup each, the others a wave each
+    long long wide_items = 0;
 };
+
+// Ring size from which a scale's averages take a whole workgroup per value.
+constexpr int AZ_WIDE_RING = 2048;
 
 hipError_t twiddle_table(const float2 **out);
 

@@ -355,20 +355,29 @@ __global__ __launch_bounds__(LargeCfg<S>::NTH_COLS) void tile_fft_cols(RingDst r
 }
 
 // ---------------------------------------------------------------------------
-// Masked azimuthal average.  One workgroup per output value
-// results[scale][tile][q][lag]; the q ring's element list is summed in FP64
-// and normalised: I = 2 * sum * s^-4 / count * norm[lag].
+// Masked azimuthal average of results[scale][tile][q][lag]: the q ring's
+// elements summed in FP64 in a fixed order and normalised,
+// I = 2 * sum * s^-4 / count * norm[lag].  WPR waves per value: four (the
+// whole workgroup) for scales with large rings, one for the rest, so that a
+// workgroup then takes AZ_WAVES consecutive values sharing the ring's
+// element list.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void azimuthal_kernel(const float *__restrict__ acc, long long E,
-                                                      const AzScale *__restrict__ scales, int S,
-                                                      const int *__restrict__ q_off,
-                                                      const int *__restrict__ q_cnt,
-                                                      const int *__restrict__ idx,
-                                                      const double *__restrict__ norm, int Q, int L,
-                                                      double *__restrict__ out, long long first_item) {
-    __shared__ double part[NT / 64];
-    const int tid = threadIdx.x;
-    const long long item = first_item + blockIdx.x;
+constexpr int AZ_WAVES = 4;
+
+template <int WPR>
+__global__ __launch_bounds__(AZ_WAVES * 64) void azimuthal_kernel(const float *__restrict__ acc, long long E,
+                                                                  const AzScale *__restrict__ scales, int S,
+                                                                  const int *__restrict__ q_off,
+                                                                  const int *__restrict__ q_cnt,
+                                                                  const int *__restrict__ idx,
+                                                                  const double *__restrict__ norm, int Q, int L,
+                                                                  double *__restrict__ out, long long first_item,
+                                                                  long long count) {
+    constexpr int NTR = WPR * 64;  // threads per value
+    __shared__ double part[AZ_WAVES];
+    const int tid = threadIdx.x % NTR;
+    const long long item = first_item + (long long)blockIdx.x * (AZ_WAVES / WPR) + threadIdx.x / NTR;
+    if (item >= first_item + count) return;  // whole values only (the WPR = 4 grid is exact)
     int si = 0;
     while (si + 1 < S && item >= scales[si + 1].out_base) ++si;
     const AzScale sc = scales[si];
@@ -384,17 +393,20 @@ __global__ __launch_bounds__(NT) void azimuthal_kernel(const float *__restrict__
     const int *ix = idx + q_off[si * Q + q];
     const float *a = acc + (long long)l * E + sc.elem_off + tile * sc.tile_elems;
     double sum = 0.0;
-    for (int i = tid; i < cnt; i += NT) sum += (double)a[ix[i]];
+    for (int i = tid; i < cnt; i += NTR) sum += (double)a[ix[i]];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) sum += __shfl_down(sum, o, 64);
-    if ((tid & 63) == 0) part[tid >> 6] = sum;
-    __syncthreads();
-    if (tid == 0) {
-        double s = 0.0;
+    if constexpr (WPR > 1) {
+        if ((tid & 63) == 0) part[tid >> 6] = sum;
+        __syncthreads();
+        if (tid != 0) return;
+        sum = 0.0;
 #pragma unroll
-        for (int w = 0; w < NT / 64; ++w) s += part[w];
-        out[item] = 2.0 * s * sc.inv_s4 / (double)cnt * norm[l];
+        for (int w = 0; w < WPR; ++w) sum += part[w];
+    } else if (tid != 0) {
+        return;
     }
+    out[item] = 2.0 * sum * sc.inv_s4 / (double)cnt * norm[l];
 }
 
 // ---------------------------------------------------------------------------
@@ -475,15 +487,24 @@ hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_azimuthal(const AzArgs &a, hipStream_t st) {
+template <int WPR>
+static void launch_az_range(const AzArgs &a, long long first, long long end, hipStream_t st) {
+    constexpr long long per_block = AZ_WAVES / WPR;
     // grid.x is limited to 2^31 - 1 workgroups; split larger result sets
-    const long long maxg = 1LL << 30;
-    for (long long first = 0; first < a.result_count; first += maxg) {
-        const long long n = (a.result_count - first) < maxg ? (a.result_count - first) : maxg;
-        hipLaunchKernelGGL(azimuthal_kernel, dim3((unsigned)n), dim3(NT), 0, st, a.acc, a.elems_per_frame,
-                           a.scales, a.scale_count, a.q_off, a.q_cnt, a.idx, a.norm, a.lambda_count,
-                           a.lag_count, a.out, first);
+    const long long maxn = (1LL << 30) * per_block;
+    for (; first < end; first += maxn) {
+        const long long n = (end - first) < maxn ? (end - first) : maxn;
+        const unsigned blocks = (unsigned)((n + per_block - 1) / per_block);
+        hipLaunchKernelGGL(azimuthal_kernel<WPR>, dim3(blocks), dim3(AZ_WAVES * 64), 0, st, a.acc,
+                           a.elems_per_frame, a.scales, a.scale_count, a.q_off, a.q_cnt, a.idx, a.norm,
+                           a.lambda_count, a.lag_count, a.out, first, n);
     }
+}
+
+hipError_t launch_azimuthal(const AzArgs &a, hipStream_t st) {
+    const long long split = a.wide_items < a.result_count ? a.wide_items : a.result_count;
+    if (split > 0) launch_az_range<AZ_WAVES>(a, 0, split, st);
+    if (split < a.result_count) launch_az_range<1>(a, split, a.result_count, st);
     return hipGetLastError();
 }
 
