@@ -210,6 +210,8 @@ def main():
                     help="frames start in pinned host memory (PCIe-inclusive rate; not the headline value)")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x2", "fp16x2"],
                     help="64/128 tile transforms: FP32 FFT or split-operand DFT on the matrix cores")
+    ap.add_argument("--autotune", action="store_true",
+                    help="let the engine's autotuner pick the lag block and FFT batch (ddm_autotune)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=0, help="frames for the CPU baseline sample (0 = all)")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 tile checks")
@@ -258,10 +260,20 @@ def main():
 
     prec = {"fp32": ddm.PREC_FP32, "bf16x2": ddm.PREC_BF16X2, "fp16x2": ddm.PREC_FP16X2}[args.precision]
 
+    tune_cache = None
+    if args.autotune:
+        import tempfile
+        tune_cache = os.path.join(tempfile.mkdtemp(prefix="ddm_tune_"), "schedule.txt")
+    schedule = {}
+
     def make():
         e = ddm.Engine(S, S, W["scales"], lags, lams, W["tol"], W["chunk"], bpp=bpp, pixfmt=pixfmt,
                        lag_block=args.lag_block)
         e.set_precision(prec)
+        if args.autotune:
+            # timed on the first engine, read back from the cache by the others
+            # (before any frame, outside the timed region)
+            schedule["lag_block"], schedule["fft_batch_frames"] = e.autotune(tune_cache)
         return e
 
     def step(e):
@@ -341,7 +353,9 @@ def main():
             # device-resident frames are transformed whole chunks at a time, up to
             # ~128 frames per batch (the engine's FFT batch); host input goes chunk
             # by chunk.  Chunk latencies below are per batch.
-            "fft_batch_frames": W["chunk"] if args.host_input else W["chunk"] * max(1, 128 // W["chunk"]),
+            "fft_batch_frames": W["chunk"] if args.host_input else
+            schedule.get("fft_batch_frames", W["chunk"] * max(1, 128 // W["chunk"])),
+            "lag_block": lag_block, "schedule": "autotuned" if args.autotune else "default",
         },
         "p50_chunk_ms": percentile(chunk_ms, 50),
         "p99_chunk_ms": percentile(chunk_ms, 99),

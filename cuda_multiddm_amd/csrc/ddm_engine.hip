@@ -36,6 +36,13 @@ struct Mark {
     int kind;
 };
 
+// Frames of device-resident input transformed per batch by default: whole
+// chunks, up to about 128 frames.  A 30-frame chunk of one scale is too
+// little work to fill the device; ~128 frames is, while the 512-tile row
+// spectra of a batch (1 MB a frame) still fit in the last-level cache for
+// the column pass.  ddm_autotune may choose another multiple of the chunk.
+int default_fft_batch(int chunk) { return chunk * std::max(1, 128 / chunk); }
+
 }  // namespace
 
 struct ddm_engine {
@@ -43,6 +50,7 @@ struct ddm_engine {
     ddm::RingTables rt;
     int device = 0;
     int lag_block = 32;
+    int fft_batch = 30;  // frames of device-resident input transformed per batch
     float *d_ring = nullptr;
     float *d_acc = nullptr;
     int *d_lags = nullptr;
@@ -218,6 +226,7 @@ extern "C" int ddm_create(const ddm_desc *desc, ddm_engine **out) {
     } while (0)
     HTRY(hipGetDevice(&e->device));
     e->lag_block = p.lag_block;
+    e->fft_batch = default_fft_batch(p.chunk);
     TRY(ddm::build_ring_tables(p, &e->rt));
     const long long E = p.elems_padded;
     const size_t L = p.lags.size();
@@ -383,11 +392,6 @@ static int process_chunk(ddm_engine *e, const uint8_t *src, int n, long long fra
     return DDM_OK;
 }
 
-// Frames of device-resident input transformed per batch: whole chunks, up to
-// about FFT_BATCH_FRAMES.  A 30-frame chunk of one scale is too little work
-// to fill the device; ~128 frames is, while the 512-tile row spectra of a
-// batch (1 MB a frame) still fit in the last-level cache for the column pass.
-constexpr int FFT_BATCH_FRAMES = 128;
 
 // Frames the next batch may hold without overwriting a ring slot still
 // needed: frame t goes to slot t % R, and the slots of frames not yet lag
@@ -406,9 +410,8 @@ extern "C" int ddm_submit_device(ddm_engine *e, const void *frames, int n_frames
     const Params &p = e->p;
     const long long fb = (long long)p.width * p.height * p.bpp;
     const uint8_t *src = (const uint8_t *)frames;
-    const int batch = p.chunk * std::max(1, FFT_BATCH_FRAMES / p.chunk);
     for (int i = 0; i < n_frames;) {
-        const int n = std::min({batch, n_frames - i, batch_room(e)});
+        const int n = std::min({e->fft_batch, n_frames - i, batch_room(e)});
         int rc = process_chunk(e, src + (long long)i * fb, n, fb, p.width, p.x_off, p.y_off, nullptr);
         if (rc) return rc;
         i += n;
@@ -620,7 +623,7 @@ extern "C" int ddm_get_info(const ddm_engine *e, ddm_info *info) {
     info->lag_count = (int)p.lags.size();
     info->lambda_count = (int)p.lambdas.size();
     info->ring_slots = p.ring_slots;
-    info->lag_block = p.lag_block;
+    info->lag_block = e->lag_block;  // the descriptor's, or the autotuner's choice
     info->elems_per_frame = p.elems_per_frame;
     info->elems_per_frame_padded = p.elems_padded;
     info->result_count = p.result_count;
@@ -686,6 +689,182 @@ extern "C" int ddm_set_precision(ddm_engine *e, int mode) {
     if (mode != DDM_PREC_FP32 && mode != DDM_PREC_BF16X2 && mode != DDM_PREC_FP16X2)
         return fail(e, DDM_EINVAL, "unknown precision mode");
     e->precision = mode;
+    return DDM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: the lag block and the FFT batch of device-resident input are
+// timed on this engine's own buffers (synthetic frames, the real geometry)
+// and the fastest per frame is kept.  Results are cached per device and
+// geometry in a text file, one "key<TAB>lag_block fft_batch" line each.
+// ---------------------------------------------------------------------------
+static std::string tune_key(const ddm_engine *e) {
+    const Params &p = e->p;
+    hipDeviceProp_t prop{};
+    (void)hipGetDeviceProperties(&prop, e->device);
+    uint64_t h = 1469598103934665603ull;  // FNV-1a of the lag list
+    for (int l : p.lags) {
+        h ^= (uint64_t)(uint32_t)l;
+        h *= 1099511628211ull;
+    }
+    std::string k = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount) +
+                    " M" + std::to_string(p.M) + " bpp" + std::to_string(p.bpp) + " fmt" + std::to_string(p.fmt) +
+                    " C" + std::to_string(p.chunk) + " J" + std::to_string(p.lag_block) + " S";
+    for (const ddm::ScaleGeom &g : p.geom) k += std::to_string(g.s) + ",";
+    char hx[32];
+    std::snprintf(hx, sizeof hx, " L%zu:%016llx", p.lags.size(), (unsigned long long)h);
+    return k + hx;
+}
+
+static bool tune_lookup(const char *path, const std::string &key, int *J, int *B) {
+    if (!path) return false;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return false;
+    char line[4096];
+    bool found = false;
+    while (std::fgets(line, sizeof line, f)) {
+        const char *tab = std::strchr(line, '\t');
+        if (!tab || std::string(line, tab - line) != key) continue;
+        int j = 0, b = 0;
+        if (std::sscanf(tab + 1, "%d %d", &j, &b) == 2) {  // the last matching line wins
+            *J = j;
+            *B = b;
+            found = true;
+        }
+    }
+    std::fclose(f);
+    return found;
+}
+
+// Milliseconds of `launch` on the engine's stream: best of three after one
+// untimed run.
+template <class F>
+static int time_ms(ddm_engine *e, F launch, float *best) {
+    hipEvent_t a = take_event(e), b = take_event(e);
+    if (!a || !b) return fail(e, DDM_EDEVICE, "event creation failed");
+    *best = 1e30f;
+    for (int rep = 0; rep < 4; ++rep) {
+        HIP_TRY(e, hipEventRecord(a, e->st));
+        HIP_TRY(e, launch());
+        HIP_TRY(e, hipEventRecord(b, e->st));
+        HIP_TRY(e, hipEventSynchronize(b));
+        float ms = 0.f;
+        HIP_TRY(e, hipEventElapsedTime(&ms, a, b));
+        if (rep) *best = std::min(*best, ms);
+    }
+    e->pool.push_back(a);
+    e->pool.push_back(b);
+    return DDM_OK;
+}
+
+extern "C" int ddm_autotune(ddm_engine *e, const char *cache_path, int *lag_block_out, int *fft_batch_out) {
+    if (!e) return DDM_EINVAL;
+    if (e->frames != 0) return fail(e, DDM_ESTATE, "ddm_autotune must run before the first frame");
+    int rc = ddm_sync(e);
+    if (rc) return rc;
+    const Params &p = e->p;
+    const std::string key = tune_key(e);
+    int J = 0, B = 0;
+    const bool cached = tune_lookup(cache_path, key, &J, &B) && J >= 32 && J <= p.lag_block && (J & (J - 1)) == 0 &&
+                        B >= p.chunk && B % p.chunk == 0 && B <= p.chunk + p.lag_block;
+    if (!cached) {
+        // lag block: J frames against the full max-lag history, per frame; a
+        // candidate must beat the descriptor's block by 2 % to replace it
+        float best = 1e30f, def_lag = 1e30f;
+        for (int cand : {64, 128, 256}) {
+            if (cand > p.lag_block) continue;
+            ddm::LagArgs a;
+            a.ring = e->d_ring;
+            a.acc = e->d_acc;
+            a.elems_per_frame = p.elems_padded;
+            a.ring_slots = p.ring_slots;
+            a.slot0 = 0;
+            a.n_new = cand;
+            a.dmax = p.max_lag;
+            a.lags = e->d_lags;
+            a.lag_count = (int)p.lags.size();
+            a.lag_block = cand;
+            float ms = 0.f;
+            rc = time_ms(e, [&] { return ddm::launch_lag(a, e->st); }, &ms);
+            if (rc) return rc;
+            if (cand == p.lag_block) def_lag = ms / cand;
+            if (ms / cand < best) {
+                best = ms / cand;
+                J = cand;
+            }
+        }
+        if (!J || (def_lag < 1e30f && best > 0.98f * def_lag)) J = p.lag_block;
+        // FFT batch: the same bmax synthetic frames through every scale in
+        // batches of 1, 2, 4 or 8 chunks (per-launch costs included); a
+        // candidate must beat the default by 2 % to replace it
+        int bmax = p.chunk;
+        for (int k : {2, 4, 8})
+            if (p.chunk * k <= p.chunk + p.lag_block) bmax = p.chunk * k;
+        const long long fb = (long long)p.width * p.height * p.bpp;
+        uint8_t *frames = nullptr;
+        rc = dev_alloc(e, &frames, (size_t)(fb * bmax), "autotune frames");
+        if (rc) return rc;
+        HIP_TRY(e, hipMemsetAsync(frames, 0x5a, (size_t)(fb * bmax), e->st));
+        rc = ensure_xt(e, bmax);
+        if (rc) return rc;
+        best = 1e30f;
+        float def_ms = 1e30f;
+        const int def_b = std::min(default_fft_batch(p.chunk), bmax);
+        for (int k : {1, 2, 4, 8}) {
+            const int n = p.chunk * k;
+            if (n > bmax) break;
+            auto launch = [&]() -> hipError_t {
+                for (int done = 0; done < bmax; done += n)
+                for (const ddm::ScaleGeom &g : p.geom) {
+                    ddm::TileFftArgs a;
+                    a.frames = frames + (long long)done * fb;
+                    a.n_frames = std::min(n, bmax - done);
+                    a.frame_bytes = fb;
+                    a.width = p.width;
+                    a.bpp = p.bpp;
+                    a.fmt = p.fmt;
+                    a.x_off = p.x_off;
+                    a.y_off = p.y_off;
+                    a.largest_scale = p.M;
+                    a.scale = g.s;
+                    a.ring = e->d_ring;
+                    a.ring_slots = p.ring_slots;
+                    a.first_slot = 0;
+                    a.elems_per_frame = p.elems_padded;
+                    a.elem_off = g.elem_off;
+                    a.precision = e->precision;
+                    a.xt = e->d_xt;
+                    const hipError_t err = ddm::launch_tile_fft(a, e->st);
+                    if (err != hipSuccess) return err;
+                }
+                return hipSuccess;
+            };
+            float ms = 0.f;
+            rc = time_ms(e, launch, &ms);
+            if (rc) return rc;
+            if (n == def_b) def_ms = ms;
+            if (ms < best) {
+                best = ms;
+                B = n;
+            }
+        }
+        if (def_ms < 1e30f && best > 0.98f * def_ms) B = def_b;
+        HIP_TRY(e, hipFreeAsync(frames, e->st));
+        e->dev_bytes -= (size_t)(fb * bmax);
+        // the timing runs accumulated into the lag sums
+        HIP_TRY(e, hipMemsetAsync(e->d_acc, 0, p.lags.size() * (size_t)p.elems_padded * sizeof(float), e->st));
+        HIP_TRY(e, hipStreamSynchronize(e->st));
+        if (cache_path) {
+            if (FILE *f = std::fopen(cache_path, "a")) {
+                std::fprintf(f, "%s\t%d %d\n", key.c_str(), J, B);
+                std::fclose(f);
+            }
+        }
+    }
+    e->lag_block = J;
+    e->fft_batch = B;
+    if (lag_block_out) *lag_block_out = J;
+    if (fft_batch_out) *fft_batch_out = B;
     return DDM_OK;
 }
 

@@ -38,7 +38,7 @@ EXPORTED = [
     "ddm_synth_uniform", "ddm_synth_brownian", "ddm_write_iqt_file", "ddm_flush",
     "ddm_server_create", "ddm_server_destroy", "ddm_server_open", "ddm_server_submit", "ddm_server_results",
     "ddm_server_engine", "ddm_server_drain", "ddm_server_close", "ddm_server_latencies", "ddm_server_stats",
-    "ddm_server_last_error", "ddm_fit_results", "ddm_k_fit", "ddm_set_precision",
+    "ddm_server_last_error", "ddm_fit_results", "ddm_k_fit", "ddm_set_precision", "ddm_autotune",
 ]
 
 
@@ -112,6 +112,7 @@ def lib():
     L.ddm_results.argtypes = [vp, dp]
     L.ddm_fit_results.argtypes = [vp, ctypes.c_float, dp, dp]
     L.ddm_k_fit.argtypes = [vp, vp, ctypes.c_int, ll, ctypes.c_int, vp, vp]
+    L.ddm_autotune.argtypes = [vp, ctypes.c_char_p, ip, ip]
     L.ddm_reset.argtypes = [vp]
     L.ddm_get_info.argtypes = [vp, ctypes.POINTER(Info)]
     L.ddm_get_params.argtypes = [vp, ip, ip, fp]
@@ -282,6 +283,14 @@ class Engine:
     def set_precision(self, mode: int):
         """PREC_FP32 (default), PREC_BF16X2 or PREC_FP16X2 for the 64 / 128 tile transforms."""
         self._check(lib().ddm_set_precision(self._h, int(mode)))
+
+    def autotune(self, cache_path: Optional[str] = None):
+        """Times the lag block and the FFT batch on this engine (before the
+        first frame) and keeps the fastest; returns (lag_block, fft_batch_frames)."""
+        j, b = ctypes.c_int(0), ctypes.c_int(0)
+        path = cache_path.encode() if cache_path else None
+        self._check(lib().ddm_autotune(self._h, path, ctypes.byref(j), ctypes.byref(b)))
+        return j.value, b.value
 
     def reset(self):
         self._check(lib().ddm_reset(self._h))

@@ -92,7 +92,7 @@ typedef struct ddm_info {
     int largest_scale;         /* M */
     int scale_count, lag_count, lambda_count;
     int ring_slots;            /* spectra kept resident: chunk_frames + lag_block + max lag */
-    int lag_block;             /* frames per lag-engine pass */
+    int lag_block;             /* frames per lag-engine pass (after ddm_autotune: its choice) */
     long long elems_per_frame; /* complex spectrum elements per frame, all scales */
     long long elems_per_frame_padded; /* rounded up to a multiple of 64: ring and accumulator row stride */
     long long result_count;    /* sum over scales of tiles * lambda_count * lag_count */
@@ -184,6 +184,17 @@ int ddm_set_timing(ddm_engine *eng, int on);
  * frames submitted from now on (enum ddm_precision).  No counterpart in the
  * reference, which always uses cuFFT in FP32. */
 int ddm_set_precision(ddm_engine *eng, int mode);
+
+/* Autotuner: before the first frame, times the lag-engine block (64, 128 or
+ * 256 frames, at most the descriptor's lag_block) and the FFT batch of
+ * device-resident input (1, 2, 4 or 8 chunks) on this engine's buffers with
+ * synthetic frames, and keeps the fastest per frame.  cache_path (may be
+ * NULL): text file of earlier choices, keyed by device and geometry; a hit
+ * skips the timing, a miss appends the new choice.  Writes the choice to
+ * *lag_block / *fft_batch_frames (either may be NULL).  Results do not depend
+ * on the choice.  No counterpart in the reference, whose launch shapes are
+ * compile-time constants (src/constants.hpp:11-13). */
+int ddm_autotune(ddm_engine *eng, const char *cache_path, int *lag_block, int *fft_batch_frames);
 int ddm_get_stage_stats(ddm_engine *eng, ddm_stage_stats *stats);
 /* Device time of each submitted chunk (tiler+FFT+lag), in submission order;
  * returns the number written (<= max). */

@@ -214,3 +214,34 @@ def test_create_rejects_invalid_descriptors(gpu):
         ddm.Engine(64, 64, [64], [1], [4.0], x_off=1)
     with pytest.raises(ddm.DDMError):
         ddm.Engine(64, 64, [64], [2, 2], [4.0])
+
+
+def test_autotune_schedule_and_cache(gpu, tmp_path):
+    """The autotuner picks a lag block and an FFT batch from its candidates,
+    results still match the oracle, a second engine of the same geometry
+    reads the choice from the cache, and tuning after the first frame is
+    refused."""
+    import torch
+    frames = ddm.synth_uniform(200, 128, 128, 8, seed=21)
+    scales, lags, lams = [128, 64], [1, 2, 3, 5, 9, 17, 40], [4.0, 8.0, 16.0]
+    cache = str(tmp_path / "schedule.txt")
+    raw = np.ascontiguousarray(frames).view(np.uint8).reshape(-1)
+    t = torch.from_numpy(raw.copy()).cuda()
+    torch.cuda.synchronize()
+    with ddm.Engine(128, 128, scales, lags, lams, 1.2, 20) as e:
+        j, b = e.autotune(cache)
+        print(f"autotune: lag block {j}, FFT batch {b}", flush=True)
+        assert j in (64, 128, 256) and b in (20, 40, 80, 160)
+        assert e.info().lag_block == j
+        e.submit_device(t.data_ptr(), frames.shape[0])
+        e.sync()
+        got = e.results()
+        with pytest.raises(ddm.DDMError):
+            e.autotune(cache)
+    ref = o.run(o.Config(128, 128, frames.shape[0], scales, lags, lams, np.float32(1.2).item()), frames)
+    assert o.max_rel_err(got, ref) <= TOL
+    lines = open(cache).read().splitlines()
+    assert len(lines) == 1 and lines[0].endswith(f"\t{j} {b}")
+    with ddm.Engine(128, 128, scales, lags, lams, 1.2, 20) as e2:
+        assert e2.autotune(cache) == (j, b)
+    assert len(open(cache).read().splitlines()) == 1  # a hit appends nothing
