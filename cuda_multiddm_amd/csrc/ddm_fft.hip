@@ -415,20 +415,23 @@ static void launch_reg(const PixSrc &ps, const RingDst &rd, int tps, int tiles, 
 }
 
 // ---------------------------------------------------------------------------
-// 256 x 256 and 512 x 512 tiles: rows and columns in two kernels, the row
-// half spectra parked in the ring slot in between (as tile_fft_rows /
-// tile_fft_cols of ddm_kernels.hip).  Sequences of length S = E1 x 16:
-// phase 1 on 16 lanes with E1 = S/16 values each (radix-16 or radix-32 in
-// registers), phase 2 on the same 16 lanes with E1/16 length-16 DFTs each.
-// The real-pair separation and the packed DC/Nyquist column go through LDS.
+// 256, 512 and 1024 tiles: rows and columns in two kernels, the row half
+// spectra parked in the batch's pass buffer in between.  Sequences of length
+// S = E1 x L on L lanes (L = 16, or 32 for S = 1024): phase 1 a radix-E1
+// DFT of each lane's E1 values at stride L (radix-16 or radix-32 in
+// registers) and the twiddles W_S^(q k1); phase 2 after an exchange through
+// LDS private to the sequence: E1/L length-L DFTs per lane.  The real-pair
+// separation and the packed DC/Nyquist column go through LDS.
 // ---------------------------------------------------------------------------
 template <int S>
 struct BigCfg {
     static constexpr int H = S / 2;
-    static constexpr int E1 = S / 16;  // phase-1 values per lane
-    static constexpr int P = E1 / 16;  // length-16 DFTs per lane in phase 2
-    static constexpr int NTH = 256;    // 4 waves, 4 sequences each
-    static constexpr int SEQ = NTH / 16;  // sequences per workgroup
+    static constexpr int L = S >= 1024 ? 32 : 16;  // lanes per sequence
+    static constexpr int E1 = S / L;               // phase-1 values per lane
+    static constexpr int P = E1 / L;               // length-L DFTs per lane in phase 2
+    static constexpr int NTH = 256;                // 4 waves
+    static constexpr int SEQ = NTH / L;            // sequences (rows pairs / columns) per workgroup
+    static_assert(E1 == 16 || E1 == 32, "phase 1 is radix 16 or 32");
 };
 
 template <int E>
@@ -442,48 +445,48 @@ __device__ __forceinline__ void dftE<32>(float2 (&v)[32]) {
     dft32(v);
 }
 
-// One length-S FFT per group of 16 lanes (q = lane & 15, sequence sq of the
-// wave): v holds x[n1 * 16 + q] on entry; on return the natural-order
-// spectrum of the sequence is at seqbuf[0 .. S) (LDS, private to the wave).
-// x is exchanged through seqbuf as [k1][n2] with a rotation of n2 by k1 so
-// that the 16 lanes of each access hit distinct 8-byte bank slots.
+// One length-S FFT per group of L lanes (q = lane index in the group): v
+// holds x[n1 * L + q] on entry; on return the natural-order spectrum of the
+// sequence is at seqbuf[0 .. S) (LDS, private to the sequence).  x is
+// exchanged through seqbuf as [k1][n2] with a rotation of n2 by k1 so that
+// the lanes of each access hit distinct 8-byte bank slots.
 template <int S>
-__device__ __forceinline__ void big_fft(float2 (&v)[S / 16], float2 *seqbuf, int q,
+__device__ __forceinline__ void big_fft(float2 (&v)[BigCfg<S>::E1], float2 *seqbuf, int q,
                                         const float2 *__restrict__ twg) {
-    constexpr int E1 = S / 16, P = E1 / 16;
+    constexpr int L = BigCfg<S>::L, E1 = BigCfg<S>::E1, P = BigCfg<S>::P;
     dftE<E1>(v);
 #pragma unroll
     for (int k1 = 1; k1 < E1; ++k1) v[k1] = cmul(v[k1], twg[S + __mul24(q, k1)]);
 #pragma unroll
-    for (int k1 = 0; k1 < E1; ++k1) seqbuf[k1 * 16 + ((q + k1) & 15)] = v[k1];
-    float2 u[P][16];
+    for (int k1 = 0; k1 < E1; ++k1) seqbuf[k1 * L + ((q + k1) & (L - 1))] = v[k1];
+    float2 u[P][L];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-        const int k1 = q + 16 * p;
+        const int k1 = q + L * p;
 #pragma unroll
-        for (int n2 = 0; n2 < 16; ++n2) u[p][n2] = seqbuf[k1 * 16 + ((n2 + k1) & 15)];
-        dft16(u[p]);  // u[p][k2] = X[k1 + E1 k2]
+        for (int n2 = 0; n2 < L; ++n2) u[p][n2] = seqbuf[k1 * L + ((n2 + k1) & (L - 1))];
+        dftE<L>(u[p]);  // u[p][k2] = X[k1 + E1 k2]
     }
 #pragma unroll
     for (int p = 0; p < P; ++p)
 #pragma unroll
-        for (int k2 = 0; k2 < 16; ++k2) seqbuf[q + 16 * p + E1 * k2] = u[p][k2];
+        for (int k2 = 0; k2 < L; ++k2) seqbuf[q + L * p + E1 * k2] = u[p][k2];
 }
 
-// Rows: 16 row pairs per workgroup; columns 0 .. H-1 of rows 2r, 2r+1 of the
+// Rows: SEQ row pairs per workgroup; columns 0 .. H-1 of rows 2r, 2r+1 of the
 // tile's half spectrum (column 0 = DC + i Nyquist) go to the batch's
 // column-block buffer xt: frame f, tile t, column block b = kx / 16, row ky,
 // column kx % 16 at xt[(((f * tiles + t) * H / 16 + b) * S + ky) * 16 + kx % 16],
-// so the lanes of a row piece write 128 aligned bytes and the column pass
-// reads each block as one contiguous run.
+// so every 16 lanes of a row piece write 128 aligned bytes and the column
+// pass reads each block as one contiguous run.
 template <int S, int PIX>
 __global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg(PixSrc ps, int tps, int tiles,
                                                                    float2 *__restrict__ xt,
                                                                    const float2 *__restrict__ twg) {
     using C = BigCfg<S>;
-    constexpr int H = C::H, E1 = C::E1, SEQ = C::SEQ, GROUPS = H / SEQ;
+    constexpr int H = C::H, L = C::L, E1 = C::E1, SEQ = C::SEQ, GROUPS = H / SEQ;
     __shared__ float2 lds[SEQ * S];
-    const int tid = threadIdx.x, q = tid & 15, sq = tid >> 4;
+    const int tid = threadIdx.x, q = tid & (L - 1), sq = tid / L;
     const int nf = gridDim.x / (GROUPS * tiles);  // frames vary fastest
     const int f = blockIdx.x % nf;
     const int g = (blockIdx.x / nf) % GROUPS;
@@ -493,14 +496,16 @@ __global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg(PixSrc ps, i
     const int x0 = ps.x_off + (tile % tps) * S + q;
     const int y0 = ps.y_off + (tile / tps) * S + 2 * r;
     float2 v[E1];
-    load_row_pair<PIX, E1, 16>(fr, ps, x0, y0, v);
+    load_row_pair<PIX, E1, L>(fr, ps, x0, y0, v);
     float2 *seqbuf = lds + sq * S;
     big_fft<S>(v, seqbuf, q, twg);
-    // separate the row pair: lane q writes columns k = q + 16 m
-    float2 *xa = xt + ((long long)f * tiles + tile) * ((long long)S * H) + (2 * r) * 16 + q;
+    // separate the row pair: lane q writes columns k = q + L m, block
+    // k / 16 = (L / 16) m + q / 16, column q % 16 of it
+    float2 *xa = xt + ((long long)f * tiles + tile) * ((long long)S * H) + (2 * r) * 16 + (q & 15) +
+                 (long long)(q >> 4) * S * 16;
 #pragma unroll
-    for (int m = 0; m < H / 16; ++m) {
-        const int k = q + 16 * m;
+    for (int m = 0; m < H / L; ++m) {
+        const int k = q + L * m;
         const float2 zk = seqbuf[k];
         const float2 zp = seqbuf[(S - k) & (S - 1)];
         float2 a = make_float2(0.5f * (zk.x + zp.x), 0.5f * (zk.y - zp.y));
@@ -510,57 +515,58 @@ __global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg(PixSrc ps, i
             a = make_float2(zk.x, zh.x);
             b = make_float2(zk.y, zh.y);
         }
-        xa[(long long)m * S * 16] = a;
-        xa[(long long)m * S * 16 + 16] = b;
+        xa[(long long)m * (L / 16) * S * 16] = a;
+        xa[(long long)m * (L / 16) * S * 16 + 16] = b;
     }
 }
 
-// Columns: 16 consecutive packed columns per workgroup, read from their
-// contiguous column block of xt and written to the ring slot; the packed
-// column is separated into the DC (kx = 0) and Nyquist (kx = H) columns on
-// the way out.
+// Columns: SEQ consecutive packed columns per workgroup, read from the pass
+// buffer's column block and written to the ring slot; the packed column is
+// separated into the DC (kx = 0) and Nyquist (kx = H) columns on the way out.
 template <int S>
 __global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_cols_reg(RingDst rd, int tiles,
                                                                    const float2 *__restrict__ xt,
                                                                    const float2 *__restrict__ twg) {
     using C = BigCfg<S>;
-    constexpr int H = C::H, E1 = C::E1, SEQ = C::SEQ, NB = H / SEQ, NTH = C::NTH;
-    // tile block [y][c ^ (y & 15)], 16 columns per row; later the columns'
+    constexpr int H = C::H, L = C::L, E1 = C::E1, SEQ = C::SEQ, NB = H / SEQ, NTH = C::NTH;
+    constexpr int RP = NTH / SEQ;  // rows per pass of the staging loops (= L)
+    // tile block [y][c ^ (y % SEQ)], SEQ columns per row; later the columns'
     // spectra, S + 1 apart so that reading one row across them is conflict free
     __shared__ float2 lds[SEQ * (S + 1)];
-    const int tid = threadIdx.x, q = tid & 15, sq = tid >> 4;
+    const int tid = threadIdx.x;
     const int nf = gridDim.x / (NB * tiles);  // frames vary fastest
     const int f = blockIdx.x % nf;
     const int cb = (blockIdx.x / nf) % NB;
     const int tile = blockIdx.x / (nf * NB);
     const int c0 = cb * SEQ;
     const TileOut t = tile_out(rd, f, tile, (long long)S * (H + 1));
+    const int col = tid % SEQ, row = tid / SEQ;  // staging: SEQ consecutive columns of RP rows per pass
     {
-        // coalesced: 16 rows x 16 consecutive columns per pass
-        constexpr int K = S * SEQ / NTH;
-        float2 v[K];
-        const float2 *xb = xt + (((long long)f * tiles + tile) * (H / 16) + cb) * ((long long)S * 16) + sq * 16 + q;
+        float2 v[E1];
+        const float2 *xb = xt + (((long long)f * tiles + tile) * (H / 16) + c0 / 16) * ((long long)S * 16) +
+                           row * 16 + (c0 & 15) + col;
 #pragma unroll
-        for (int k = 0; k < K; ++k) v[k] = xb[k * 256];
+        for (int k = 0; k < E1; ++k) v[k] = xb[k * RP * 16];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int y = sq + 16 * k;
-            lds[y * SEQ + (q ^ (y & 15))] = v[k];
+        for (int k = 0; k < E1; ++k) {
+            const int y = row + RP * k;
+            lds[y * SEQ + (col ^ (y & (SEQ - 1)))] = v[k];
         }
     }
     __syncthreads();
-    // column sq of the block, lane q: rows n1 * 16 + q
+    // column sq of the block, lane q: rows n1 * L + q
+    const int q = tid & (L - 1), sq = tid / L;
     float2 v[E1];
 #pragma unroll
     for (int n1 = 0; n1 < E1; ++n1) {
-        const int y = n1 * 16 + q;
-        v[n1] = lds[y * SEQ + (sq ^ q)];
+        const int y = n1 * L + q;
+        v[n1] = lds[y * SEQ + (sq ^ (y & (SEQ - 1)))];
     }
     __syncthreads();
     float2 *seqbuf = lds + sq * (S + 1);
     big_fft<S>(v, seqbuf, q, twg);
     __syncthreads();
-    // column sq's spectrum is at lds[sq * (S + 1) + ky]; write rows of 16 columns
+    // column c's spectrum is at lds[c * (S + 1) + ky]; write rows of SEQ columns
     if (c0 == 0) {
         // packed column: DC and Nyquist of row ky from V[ky] and V[S - ky]
         for (int ky = tid; ky < S; ky += NTH) {
@@ -569,11 +575,10 @@ __global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_cols_reg(RingDst rd, 
             t[(long long)ky * (H + 1) + H] = make_float2(0.5f * (w.y + pv.y), -0.5f * (w.x - pv.x));
         }
     }
-    constexpr int K = S * SEQ / NTH;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int ky = sq + 16 * k;
-        if (c0 + q != 0) t[(long long)ky * (H + 1) + c0 + q] = lds[q * (S + 1) + ky];
+    for (int k = 0; k < E1; ++k) {
+        const int ky = row + RP * k;
+        if (c0 + col != 0) t[(long long)ky * (H + 1) + c0 + col] = lds[col * (S + 1) + ky];
     }
 }
 
@@ -607,6 +612,10 @@ bool launch_tile_fft_reg(const PixSrc &ps, const RingDst &rd, int scale, int tps
     case 512:
         if (!xt) return false;
         launch_big<512>(ps, rd, tps, tiles, n_frames, tw, xt, st);
+        return true;
+    case 1024:
+        if (!xt) return false;
+        launch_big<1024>(ps, rd, tps, tiles, n_frames, tw, xt, st);
         return true;
     default: return false;
     }

@@ -26,7 +26,7 @@ inline long long pad_elems(long long e) { return (e + RING_GROUP - 1) / RING_GRO
 // batch of n frames needs for tiles of size `scale` cut from an M x M region
 // (0 for scales that do not use it).
 inline long long big_fft_scratch_elems(int M, int scale, int n) {
-    return (scale == 256 || scale == 512) ? (long long)n * M * M / 2 : 0;
+    return (scale == 256 || scale == 512 || scale == 1024) ? (long long)n * M * M / 2 : 0;
 }
 
 // Frames per lag-engine pass when the descriptor leaves it to the engine.

@@ -123,10 +123,10 @@ def test_tile_fft_pixel_formats(gpu, fmt, bpp):
             assert np.max(np.abs(spec[f, tile] - ref)) / np.max(np.abs(ref)) < FFT_TOL
 
 
-@pytest.mark.parametrize("s", [64, 128, 256, 512])
+@pytest.mark.parametrize("s", [64, 128, 256, 512, 1024])
 @pytest.mark.parametrize("fmt,bpp", [(ddm.PIX_U16LE, 2), (ddm.PIX_U16BE, 2), (ddm.PIX_U16LE, 4), (ddm.PIX_U8, 1)])
 def test_tile_fft_pixel_formats_large_tiles(gpu, s, fmt, bpp):
-    """The 64..512 tile kernels read 1-byte and 2-byte little-endian pixels
+    """The 64..1024 tile kernels read 1-byte and 2-byte little-endian pixels
     with their own loads; other layouts take the general pixel reader."""
     rng = np.random.default_rng(s + fmt * 10 + bpp)
     M, n, xo, yo = 2 * s, 2, 3, 1
