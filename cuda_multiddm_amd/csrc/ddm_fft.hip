@@ -415,6 +415,147 @@ static void launch_reg(const PixSrc &ps, const RingDst &rd, int tps, int tiles, 
 }
 
 // ---------------------------------------------------------------------------
+// 8, 16 and 32 tiles: one lane per row pair, then one lane per packed column.
+// A tile takes R = S/2 lanes, a wave 64/R tiles, a workgroup SMALL_NTH/R.
+// Row phase: lane r holds z[n] = x_{2r}[n] + i x_{2r+1}[n] and does the
+// whole length-S DFT in registers, separates the two real rows' spectra and
+// writes columns 0 .. R-1 of rows 2r, 2r+1 (column 0 = DC + i Nyquist) to the
+// tile's LDS block.  Column phase: lane c reads packed column c, transforms
+// it in registers and writes it to the ring; lane 0 separates the packed
+// column into the DC and Nyquist columns.
+// ---------------------------------------------------------------------------
+constexpr int SMALL_NTH = 256;
+
+template <int S>
+__device__ __forceinline__ void dftS(float2 (&v)[S]);
+template <>
+__device__ __forceinline__ void dftS<8>(float2 (&v)[8]) {
+    dft8(v);
+}
+template <>
+__device__ __forceinline__ void dftS<16>(float2 (&v)[16]) {
+    dft16(v);
+}
+template <>
+__device__ __forceinline__ void dftS<32>(float2 (&v)[32]) {
+    dft32(v);
+}
+
+// S consecutive pixels of row y from x0: with A4 every row start is 4-byte
+// aligned, so 1- and 2-byte pixels come in 4-byte words.
+template <int PIX, int S, bool A4>
+__device__ __forceinline__ void load_row(const uint8_t *fr, const PixSrc &ps, int x0, int y, float (&out)[S]) {
+    const uint8_t *row = fr + (long long)y * ps.row_bytes;
+    if constexpr (PIX == PIX_U8_1 && A4 && S % 4 == 0) {
+        const uint32_t *w = reinterpret_cast<const uint32_t *>(row + x0);
+#pragma unroll
+        for (int i = 0; i < S / 4; ++i) {
+            const uint32_t u = w[i];
+            out[4 * i] = (float)(u & 0xff);
+            out[4 * i + 1] = (float)((u >> 8) & 0xff);
+            out[4 * i + 2] = (float)((u >> 16) & 0xff);
+            out[4 * i + 3] = (float)(u >> 24);
+        }
+    } else if constexpr (PIX == PIX_U16LE_2 && A4 && S % 2 == 0) {
+        const uint32_t *w = reinterpret_cast<const uint32_t *>(row + 2 * x0);
+#pragma unroll
+        for (int i = 0; i < S / 2; ++i) {
+            const uint32_t u = w[i];
+            out[2 * i] = (float)(u & 0xffff);
+            out[2 * i + 1] = (float)(u >> 16);
+        }
+    } else if constexpr (PIX == PIX_U8_1) {
+#pragma unroll
+        for (int i = 0; i < S; ++i) out[i] = (float)row[x0 + i];
+    } else if constexpr (PIX == PIX_U16LE_2) {
+        const uint16_t *h = reinterpret_cast<const uint16_t *>(row) + x0;
+#pragma unroll
+        for (int i = 0; i < S; ++i) out[i] = (float)h[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < S; ++i) out[i] = load_px(fr, ps, x0 + i, y);
+    }
+}
+
+template <int S, int PIX, bool A4>
+__global__ __launch_bounds__(SMALL_NTH) void tile_fft_lane(PixSrc ps, RingDst rd, int tps, int tiles, int groups) {
+    constexpr int R = S / 2, TPG = SMALL_NTH / R;  // lanes per tile, tiles per workgroup
+    constexpr int LS = R + 1;                      // LDS row stride (complex values)
+    __shared__ float2 lds[TPG * S * LS];
+    const int tid = threadIdx.x, lt = tid / R, r = tid % R;
+    const int nf = gridDim.x / groups;  // frames vary fastest
+    const int f = blockIdx.x % nf;
+    const int tile = (blockIdx.x / nf) * TPG + lt;
+    const bool live = tile < tiles;
+    float2 *blk = lds + lt * S * LS;
+    if (live) {
+        const uint8_t *fr = ps.frames + (long long)f * ps.frame_bytes;
+        const int x0 = ps.x_off + (tile % tps) * S;
+        const int y0 = ps.y_off + (tile / tps) * S + 2 * r;
+        float2 z[S];
+        {
+            float a[S], b[S];
+            load_row<PIX, S, A4>(fr, ps, x0, y0, a);
+            load_row<PIX, S, A4>(fr, ps, x0, y0 + 1, b);
+#pragma unroll
+            for (int n = 0; n < S; ++n) z[n] = make_float2(a[n], b[n]);
+        }
+        dftS<S>(z);
+        // rows 2r (x_a) and 2r+1 (x_b): X_a = (Z_k + conj Z_{S-k}) / 2,
+        // X_b = -i (Z_k - conj Z_{S-k}) / 2; column 0 packs DC + i Nyquist
+        blk[(2 * r) * LS] = make_float2(z[0].x, z[R].x);
+        blk[(2 * r + 1) * LS] = make_float2(z[0].y, z[R].y);
+#pragma unroll
+        for (int k = 1; k < R; ++k) {
+            const float2 zk = z[k], zp = z[S - k];
+            blk[(2 * r) * LS + k] = make_float2(0.5f * (zk.x + zp.x), 0.5f * (zk.y - zp.y));
+            blk[(2 * r + 1) * LS + k] = make_float2(0.5f * (zk.y + zp.y), -0.5f * (zk.x - zp.x));
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    // column c = r of the tile
+    float2 v[S];
+#pragma unroll
+    for (int ky = 0; ky < S; ++ky) v[ky] = blk[ky * LS + r];
+    dftS<S>(v);
+    const TileOut t = tile_out(rd, f, tile, (long long)S * (R + 1));
+    if (r != 0) {
+#pragma unroll
+        for (int ky = 0; ky < S; ++ky) t[(long long)ky * (R + 1) + r] = v[ky];
+    } else {
+        // packed column: DC (kx = 0) and Nyquist (kx = R) columns
+#pragma unroll
+        for (int ky = 0; ky < S; ++ky) {
+            const float2 w = v[ky], pv = v[(S - ky) & (S - 1)];
+            t[(long long)ky * (R + 1)] = make_float2(0.5f * (w.x + pv.x), 0.5f * (w.y - pv.y));
+            t[(long long)ky * (R + 1) + R] = make_float2(0.5f * (w.y + pv.y), -0.5f * (w.x - pv.x));
+        }
+    }
+}
+
+template <int S>
+static void launch_lane(const PixSrc &ps, const RingDst &rd, int tps, int tiles, int n_frames, hipStream_t st) {
+    constexpr int TPG = SMALL_NTH / (S / 2);
+    const int groups = (tiles + TPG - 1) / TPG;
+    const dim3 grid((unsigned)(groups * n_frames)), block(SMALL_NTH);
+    const int mode = pixel_mode(ps);
+    // every row start 4-byte aligned: frames, frame stride, row stride and the ROI's first byte
+    const bool a4 = ((ps.row_bytes | ps.frame_bytes | (long long)(uintptr_t)ps.frames |
+                      (long long)ps.x_off * ps.bpp) & 3) == 0;
+    if (mode == PIX_U8_1 && a4)
+        hipLaunchKernelGGL((tile_fft_lane<S, PIX_U8_1, true>), grid, block, 0, st, ps, rd, tps, tiles, groups);
+    else if (mode == PIX_U8_1)
+        hipLaunchKernelGGL((tile_fft_lane<S, PIX_U8_1, false>), grid, block, 0, st, ps, rd, tps, tiles, groups);
+    else if (mode == PIX_U16LE_2 && a4)
+        hipLaunchKernelGGL((tile_fft_lane<S, PIX_U16LE_2, true>), grid, block, 0, st, ps, rd, tps, tiles, groups);
+    else if (mode == PIX_U16LE_2)
+        hipLaunchKernelGGL((tile_fft_lane<S, PIX_U16LE_2, false>), grid, block, 0, st, ps, rd, tps, tiles, groups);
+    else
+        hipLaunchKernelGGL((tile_fft_lane<S, PIX_ANY, false>), grid, block, 0, st, ps, rd, tps, tiles, groups);
+}
+
+// ---------------------------------------------------------------------------
 // 256, 512 and 1024 tiles: rows and columns in two kernels, the row half
 // spectra parked in the batch's pass buffer in between.  Sequences of length
 // S = E1 x L on L lanes (L = 16, or 32 for S = 1024): phase 1 a radix-E1
@@ -603,6 +744,9 @@ static void launch_big(const PixSrc &ps, const RingDst &rd, int tps, int tiles, 
 bool launch_tile_fft_reg(const PixSrc &ps, const RingDst &rd, int scale, int tps, int tiles, int n_frames,
                          const float2 *tw, float2 *xt, hipStream_t st) {
     switch (scale) {
+    case 8: launch_lane<8>(ps, rd, tps, tiles, n_frames, st); return true;
+    case 16: launch_lane<16>(ps, rd, tps, tiles, n_frames, st); return true;
+    case 32: launch_lane<32>(ps, rd, tps, tiles, n_frames, st); return true;
     case 64: launch_reg<64>(ps, rd, tps, tiles, n_frames, tw, st); return true;
     case 128: launch_reg<128>(ps, rd, tps, tiles, n_frames, tw, st); return true;
     case 256:
