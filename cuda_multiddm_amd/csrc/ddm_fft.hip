@@ -261,7 +261,10 @@ __global__ __launch_bounds__(REG_NTH) void tile_fft_reg(PixSrc ps, RingDst rd, i
                                                        const float2 *__restrict__ twg) {
     using C = RegCfg<S>;
     constexpr int H = C::H, L = C::L, P = C::P, Q = C::Q, WPT = C::WPT, TPW = C::TPW, TILE = C::TILE;
-    __shared__ float2 lds[TPW * TILE];
+    constexpr int TE = S * (H + 1);  // spectrum elements per tile
+    // row/column transpose and exchange regions; at the end the tiles'
+    // spectra in ring element order
+    __shared__ float2 lds[TPW * TE];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = tid >> 6;
@@ -355,28 +358,16 @@ __global__ __launch_bounds__(REG_NTH) void tile_fft_reg(PixSrc ps, RingDst rd, i
         for (int n2 = 0; n2 < L; ++n2) z[p][n2] = reg[xslot<L>(sq, k1, n2)];
         dftL<L>(z[p]);  // z[p][k2] = V[k1 + 16 k2], ky = k1 + 16 k2
     }
-    if (!valid) return;
-    const TileOut out = tile_out(rd, f, tile, (long long)S * (H + 1));
+    // the tile's spectrum [ky][kx] into LDS, then out to the ring in element
+    // order: whole 512-byte element groups per wave store
+    __syncthreads();
+    float2 *ob = lds + t * TE;
     if (c != 0) {
-        // rows ky = k1 + 16 k2 are D elements apart; walk the blocked ring
-        // address instead of recomputing it per store
-        constexpr int D = 16 * (H + 1), DQ = D / RING_GROUP, DR = D % RING_GROUP;
-        const long long gstride = (long long)rd.ring_slots * RING_GROUP;  // one element group further
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            const long long e0 = out.base + (long long)kappa<L>(q, p) * (H + 1) + c;
-            int low = (int)(e0 & (RING_GROUP - 1));
-            float2 *ptr = rd.ring + ((e0 >> RING_GROUP_LOG2) * rd.ring_slots + out.slot) * RING_GROUP + low;
+            const int k1 = kappa<L>(q, p);
 #pragma unroll
-            for (int k2 = 0; k2 < L; ++k2) {
-                *ptr = z[p][k2];
-                low += DR;
-                ptr += DQ * gstride + DR;
-                if (low >= RING_GROUP) {
-                    low -= RING_GROUP;
-                    ptr += gstride - RING_GROUP;
-                }
-            }
+            for (int k2 = 0; k2 < L; ++k2) ob[(k1 + 16 * k2) * (H + 1) + c] = z[p][k2];
         }
     } else {
         // packed column: DC (kx = 0) and Nyquist (kx = H) of every row ky
@@ -387,12 +378,22 @@ __global__ __launch_bounds__(REG_NTH) void tile_fft_reg(PixSrc ps, RingDst rd, i
             const int k1 = kappa<L>(q, p);
 #pragma unroll
             for (int k2 = 0; k2 < L; ++k2) {
-                const long long row = (long long)(k1 + 16 * k2) * (H + 1);
+                const int row = (k1 + 16 * k2) * (H + 1);
                 const float2 wv = z[p][k2], pv = zq[p][k2];  // V[ky], V[S - ky]
-                out[row] = make_float2(0.5f * (wv.x + pv.x), 0.5f * (wv.y - pv.y));
-                out[row + H] = make_float2(0.5f * (wv.y + pv.y), -0.5f * (wv.x - pv.x));
+                ob[row] = make_float2(0.5f * (wv.x + pv.x), 0.5f * (wv.y - pv.y));
+                ob[row + H] = make_float2(0.5f * (wv.y + pv.y), -0.5f * (wv.x - pv.x));
             }
         }
+    }
+    __syncthreads();
+    const int first = (blockIdx.x / nf) * TPW;  // the workgroup's first tile
+    const int ntile = min(TPW, tiles - first);
+    int slot = rd.first_slot + f;
+    slot -= (slot >= rd.ring_slots) ? rd.ring_slots : 0;
+    const long long gstride = (long long)rd.ring_slots * RING_GROUP;
+    for (int i = tid; i < ntile * TE; i += REG_NTH) {
+        const long long e = rd.elem_off + (long long)first * TE + i;  // the workgroup's tiles are consecutive
+        rd.ring[(e >> RING_GROUP_LOG2) * gstride + (long long)slot * RING_GROUP + (e & (RING_GROUP - 1))] = lds[i];
     }
 }
 
