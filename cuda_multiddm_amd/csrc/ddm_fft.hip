@@ -422,8 +422,9 @@ static void launch_reg(const PixSrc &ps, const RingDst &rd, int tps, int tiles, 
 // whole length-S DFT in registers, separates the two real rows' spectra and
 // writes columns 0 .. R-1 of rows 2r, 2r+1 (column 0 = DC + i Nyquist) to the
 // tile's LDS block.  Column phase: lane c reads packed column c, transforms
-// it in registers and writes it to the ring; lane 0 separates the packed
-// column into the DC and Nyquist columns.
+// it in registers and puts it back (lane 0 separates the packed column into
+// the DC and Nyquist columns); the workgroup's spectra then go to the ring
+// in element order.
 // ---------------------------------------------------------------------------
 constexpr int SMALL_NTH = 256;
 
@@ -514,24 +515,36 @@ __global__ __launch_bounds__(SMALL_NTH) void tile_fft_lane(PixSrc ps, RingDst rd
         }
     }
     __syncthreads();
-    if (!live) return;
     // column c = r of the tile
     float2 v[S];
 #pragma unroll
     for (int ky = 0; ky < S; ++ky) v[ky] = blk[ky * LS + r];
     dftS<S>(v);
-    const TileOut t = tile_out(rd, f, tile, (long long)S * (R + 1));
+    __syncthreads();
+    // the spectrum [ky][kx <= R] back into the tile's block (LS = R + 1 is
+    // its row length), then out to the ring in element order
     if (r != 0) {
 #pragma unroll
-        for (int ky = 0; ky < S; ++ky) t[(long long)ky * (R + 1) + r] = v[ky];
+        for (int ky = 0; ky < S; ++ky) blk[ky * LS + r] = v[ky];
     } else {
         // packed column: DC (kx = 0) and Nyquist (kx = R) columns
 #pragma unroll
         for (int ky = 0; ky < S; ++ky) {
             const float2 w = v[ky], pv = v[(S - ky) & (S - 1)];
-            t[(long long)ky * (R + 1)] = make_float2(0.5f * (w.x + pv.x), 0.5f * (w.y - pv.y));
-            t[(long long)ky * (R + 1) + R] = make_float2(0.5f * (w.y + pv.y), -0.5f * (w.x - pv.x));
+            blk[ky * LS] = make_float2(0.5f * (w.x + pv.x), 0.5f * (w.y - pv.y));
+            blk[ky * LS + R] = make_float2(0.5f * (w.y + pv.y), -0.5f * (w.x - pv.x));
         }
+    }
+    __syncthreads();
+    constexpr int TE = S * LS;  // spectrum elements per tile
+    const int first = (blockIdx.x / nf) * TPG;
+    const int ntile = min(TPG, tiles - first);
+    int slot = rd.first_slot + f;
+    slot -= (slot >= rd.ring_slots) ? rd.ring_slots : 0;
+    const long long gstride = (long long)rd.ring_slots * RING_GROUP;
+    for (int i = tid; i < ntile * TE; i += SMALL_NTH) {
+        const long long e = rd.elem_off + (long long)first * TE + i;  // the workgroup's tiles are consecutive
+        rd.ring[(e >> RING_GROUP_LOG2) * gstride + (long long)slot * RING_GROUP + (e & (RING_GROUP - 1))] = lds[i];
     }
 }
 
