@@ -6,13 +6,16 @@
 //
 // Mapping: one workgroup per ring group (RING_GROUP = 64 consecutive
 // spectrum elements, one per lane), 16 waves.  The block's new frames are
-// staged once in LDS.  For a lag tau, new frame j pairs with frame j - tau:
-// for j < tau that frame is in the ring history and the partner slots of
-// consecutive j are consecutive (one wrap at most), so a range of j reads
-// its partners as at most two contiguous runs; for j >= tau both frames are
-// in LDS.  The leading run of consecutive lags (1, 2, 3, ... in the usual
-// lag lists) is handled with a register window instead: one new frame and
-// one entering partner per step serve DENSE pairs.
+// staged once in LDS; the group's ring history is read with buffer loads.
+// Three ways to form the pairs (new frame j, frame j - tau):
+//  * the leading run of consecutive lags (1, 2, 3, ... in the usual lag
+//    lists): a register window, one new frame and one entering partner per
+//    step serving DENSE pairs;
+//  * up to HIST_LAGS other lags, history-major: a block of consecutive
+//    history frames in registers is paired with the LDS rows of every lag
+//    that reaches it, so each history frame is read once per wave;
+//  * more lags than that: each lag over ranges of j, whose partners are at
+//    most two contiguous runs of ring slots (one wrap) or LDS rows.
 #include <hip/hip_runtime.h>
 
 #include "ddm_internal.h"
