@@ -49,6 +49,15 @@ struct LagCfg {
     static constexpr int HIST_BLOCK = 16;
 };
 
+// Workgroup barrier ordering LDS only.  Waves never exchange data through
+// global memory here, so loads still in flight (the early accumulator reads)
+// need not complete before it, as they would under __syncthreads().
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Relative cost of a pair in the register window and in the history-major
 // path, for the split of the waves.
 constexpr int COST_DENSE = 4, COST_HIST = 9;
@@ -162,15 +171,16 @@ __device__ __forceinline__ float2 window_frame(const GroupRing &g, const float2 
 }
 
 // D steps of the register window: step u loads the entering partner pb[u]
-// and the new frame at LDS row u of pa, and forms D pairs.
-template <int D, int S, class A>
+// and the new frame at LDS row u of pa, and forms K pairs (the first K lags
+// of the window).
+template <int D, int K, int S, class A>
 __device__ __forceinline__ void dense_block(A pb, const float2 *pa, float2 (&win)[D], float (&s)[S]) {
 #pragma unroll
     for (int u = 0; u < D; ++u) {
         win[u] = pb[u];
         const float2 a = pa[u * RING_GROUP];
 #pragma unroll
-        for (int i = 0; i < D; ++i) {
+        for (int i = 0; i < K; ++i) {
             const float2 b = win[(u - i) & (D - 1)];
             const float dx = a.x - b.x, dy = a.y - b.y;
             s[i] = fmaf(dx, dx, s[i]);
@@ -182,20 +192,23 @@ __device__ __forceinline__ void dense_block(A pb, const float2 *pa, float2 (&win
     }
 }
 
-// D consecutive lags tau_a .. tau_a + D - 1 over new frames j in [j0, j1).
-// The partners of frame j are the D frames j - tau_a - i, a sliding window
-// kept in registers: each step loads one new frame and one entering partner
-// and forms D pairs.  Window slot (u - i) & (D - 1) holds the partner of
-// step u for lag tau_a + i, so every register index is a compile-time
-// constant.  Sums of lags beyond the caller's count come from in-bounds
-// (possibly stale) slots and are discarded by the caller.
-template <int D, int S>
+// K <= D consecutive lags tau_a .. tau_a + K - 1 over new frames j in
+// [j0, j1).  The partners of frame j are the frames j - tau_a - i, a sliding
+// window of D registers: each step loads one new frame and one entering
+// partner and forms K pairs.  Window slot (u - i) & (D - 1) holds the
+// partner of step u for lag tau_a + i, so every register index is a
+// compile-time constant.  Sums of lags beyond the caller's count (K rounds
+// it up) come from in-bounds (possibly stale) slots and are discarded by the
+// caller.
+template <int D, int K, int S>
 __device__ __forceinline__ void dense_sums(const GroupRing &g, const float2 *nl, int R, int slot0, int tau_a, int j0,
                                            int j1, float (&s)[S]) {
-    static_assert(D <= S && (D & (D - 1)) == 0, "window must be a power of two within the partial sums");
+    static_assert(D <= S && (D & (D - 1)) == 0 && K <= D, "window must be a power of two within the partial sums");
     float2 win[D];
+    // slots read before a step writes them: the partners of lags 1 .. K - 1
+    // of the first steps
 #pragma unroll
-    for (int k = 1; k < D; ++k) win[k] = window_frame(g, nl, R, slot0, j0 - tau_a - (D - k));
+    for (int k = D - K + 1; k < D; ++k) win[k] = window_frame(g, nl, R, slot0, j0 - tau_a - (D - k));
 #pragma unroll
     for (int i = 0; i < D; ++i) s[i] = 0.f;
     for (int jb = j0; jb < j1; jb += D) {
@@ -204,11 +217,11 @@ __device__ __forceinline__ void dense_sums(const GroupRing &g, const float2 *nl,
         // all in the ring history without a wrap, reads them from one base
         // with constant offsets; other blocks go frame by frame.
         if (cnt >= D && r0 >= 0) {
-            dense_block<D>(LdsRun{nl + r0 * RING_GROUP}, nl + jb * RING_GROUP, win, s);
+            dense_block<D, K>(LdsRun{nl + r0 * RING_GROUP}, nl + jb * RING_GROUP, win, s);
             continue;
         }
         if (cnt >= D && r0 + D <= 0 && slot0 + r0 >= 0) {
-            dense_block<D>(RingRun{g, slot0 + r0}, nl + jb * RING_GROUP, win, s);
+            dense_block<D, K>(RingRun{g, slot0 + r0}, nl + jb * RING_GROUP, win, s);
             continue;
         }
 #pragma unroll
@@ -217,7 +230,7 @@ __device__ __forceinline__ void dense_sums(const GroupRing &g, const float2 *nl,
                 win[u] = window_frame(g, nl, R, slot0, jb + u - tau_a);
                 const float2 a = nl[(jb + u) * RING_GROUP];
 #pragma unroll
-                for (int i = 0; i < D; ++i) {
+                for (int i = 0; i < K; ++i) {
                     const float2 b = win[(u - i) & (D - 1)];
                     const float dx = a.x - b.x, dy = a.y - b.y;
                     s[i] = fmaf(dx, dx, s[i]);
@@ -226,6 +239,33 @@ __device__ __forceinline__ void dense_sums(const GroupRing &g, const float2 *nl,
             }
         }
     }
+}
+
+// The window for kc live lags: K = D / 2, 3D / 4, 7D / 8 or D, the smallest
+// that covers kc (a wider window computes lags that are thrown away).
+template <int D>
+__device__ __forceinline__ int window_width(int kc) {
+    if (D < 16) return D;
+    return kc <= D / 2 ? D / 2 : (kc <= 3 * D / 4 ? 3 * D / 4 : (kc <= 7 * D / 8 ? 7 * D / 8 : D));
+}
+template <int D, int S>
+__device__ __forceinline__ void dense_sums_live(int kc, const GroupRing &g, const float2 *nl, int R, int slot0,
+                                                int tau_a, int j0, int j1, float (&s)[S]) {
+    if constexpr (D >= 16) {
+        if (kc <= D / 2) {
+            dense_sums<D, D / 2>(g, nl, R, slot0, tau_a, j0, j1, s);
+            return;
+        }
+        if (kc <= 3 * D / 4) {
+            dense_sums<D, 3 * D / 4>(g, nl, R, slot0, tau_a, j0, j1, s);
+            return;
+        }
+        if (kc <= 7 * D / 8) {
+            dense_sums<D, 7 * D / 8>(g, nl, R, slot0, tau_a, j0, j1, s);
+            return;
+        }
+    }
+    dense_sums<D, D>(g, nl, R, slot0, tau_a, j0, j1, s);
 }
 
 // History frames p0 .. p0 + B - 1 of the window, clamped to pmax (frames
@@ -357,7 +397,7 @@ __device__ __forceinline__ void hist_reduce(float2 *nb, float *accg, long long E
 #pragma unroll
     for (int r = 0; r < LSM / 32; ++r) {
         if (r * 32 >= ls) break;
-        __syncthreads();
+        lds_barrier();
         if (w < wd) {
             if (r == 0) {
 #pragma unroll
@@ -369,7 +409,7 @@ __device__ __forceinline__ void hist_reduce(float2 *nb, float *accg, long long E
             for (int i = 0; i < 32; ++i)
                 if (r * 32 + i < ls) red[(hb + k * 32 + i) * RING_GROUP + lane] = h[r * 32 + i];
         }
-        __syncthreads();
+        lds_barrier();
         const int nd = r == 0 ? dense : 0, nh = min(32, ls - r * 32);
         for (int it = w; it < nd + nh; it += LAG_WAVES) {
             float t = 0.f;
@@ -458,7 +498,8 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
     int wd = 0;  // dense waves
     if (nchunks) {
         // relative pair costs: the window computes whole chunks of DENSE lags
-        const int wdense = hist ? COST_DENSE * nchunks * DENSE : 3 * dense_lags;
+        const int wdense = hist ? COST_DENSE * ((nchunks - 1) * DENSE + window_width<DENSE>(dense_lags - (nchunks - 1) * DENSE))
+                                : 3 * dense_lags;
         const int wother = hist ? COST_HIST * ls : 5 * ls;
         wd = (LAG_WAVES * wdense + (wdense + wother) / 2) / (wdense + wother);
         wd = max(wd, nchunks);
@@ -480,7 +521,7 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
     const int nred = hist ? L : (ss ? L : dense_lags);
     float pre[ACC_PREFETCH];
     load_acc(accg, E, w, nred, pre);
-    __syncthreads();
+    lds_barrier();
     const float2 *nl = nb + lane;
     float s[C::SLOTS_MAX];
 #pragma unroll
@@ -497,7 +538,7 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
         const int j0 = seg * len, j1 = min(j0 + len, n);
         if (j0 < j1) {
             if (j0 - (tau_a + kc - 1) >= -dmax) {
-                dense_sums<DENSE>(g, nl, R, slot0, tau_a, j0, j1, s);
+                dense_sums_live<DENSE>(kc, g, nl, R, slot0, tau_a, j0, j1, s);
             } else {
                 // near the start of the stream some partners do not exist yet
 #pragma unroll
@@ -531,11 +572,11 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
     }
     if (wd == 0 && ss == 0) return;
     // ---- combine the partials: [wave][slot][lane] in the staging buffer ----
-    __syncthreads();
+    lds_barrier();
     float *red = reinterpret_cast<float *>(nb);
 #pragma unroll
     for (int i = 0; i < SLOTS; ++i) red[(w * SLOTS + i) * RING_GROUP + lane] = s[i];
-    __syncthreads();
+    lds_barrier();
     // one lag per wave and step, so every index below is wave-uniform
     for (int l = w; l < nred; l += LAG_WAVES) {
         float t = 0.f;
