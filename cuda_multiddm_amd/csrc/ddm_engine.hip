@@ -302,6 +302,7 @@ static int lag_block(ddm_engine *e, int n) {
     a.n_new = n;
     a.dmax = (int)std::min<long long>(p.max_lag, t0);
     a.lags = e->d_lags;
+    a.lags_host = p.lags.data();
     a.lag_count = (int)p.lags.size();
     a.lag_block = e->lag_block;
     Mark lag;
@@ -782,6 +783,7 @@ extern "C" int ddm_autotune(ddm_engine *e, const char *cache_path, int *lag_bloc
             a.n_new = cand;
             a.dmax = p.max_lag;
             a.lags = e->d_lags;
+            a.lags_host = p.lags.data();
             a.lag_count = (int)p.lags.size();
             a.lag_block = cand;
             float ms = 0.f;
@@ -950,6 +952,15 @@ extern "C" int ddm_k_lag_accumulate(const float *ring, int ring_slots, long long
     if (lag_block != 32 && lag_block != 64 && lag_block != 128 && lag_block != 256) return DDM_EINVAL;
     if (n_new <= 0 || n_new > lag_block) return DDM_EINVAL;
     if (max_lag < 0 || (long long)max_lag + n_new > ring_slots) return DDM_ERANGE;
+    // the launch plan is made on the host: bring the lag list over (this
+    // waits for the stream)
+    std::vector<int> host_lags(lag_count);
+    if (hipMemcpyAsync(host_lags.data(), lags, sizeof(int) * lag_count, hipMemcpyDeviceToHost,
+                       (hipStream_t)stream) != hipSuccess ||
+        hipStreamSynchronize((hipStream_t)stream) != hipSuccess)
+        return DDM_EDEVICE;
+    for (int l = 0; l < lag_count; ++l)
+        if (host_lags[l] < 0 || host_lags[l] > max_lag || (l && host_lags[l] <= host_lags[l - 1])) return DDM_EINVAL;
     ddm::LagArgs a;
     a.ring = ring;
     a.acc = acc;
@@ -959,6 +970,7 @@ extern "C" int ddm_k_lag_accumulate(const float *ring, int ring_slots, long long
     a.n_new = n_new;
     a.dmax = (int)std::min<long long>(max_lag, first_frame);
     a.lags = lags;
+    a.lags_host = host_lags.data();
     a.lag_count = lag_count;
     a.lag_block = lag_block;
     return ddm::launch_lag(a, (hipStream_t)stream) == hipSuccess ? DDM_OK : DDM_EDEVICE;

@@ -37,6 +37,7 @@ struct LagArgs {
     int n_new;                 // frames in the block, <= lag_block
     int dmax;                  // history frames paired with: min(max lag, t0)
     const int *lags;           // device, ascending
+    const int *lags_host;      // the same lags in host memory (the launch plan is made on the host)
     int lag_count;
     int lag_block;             // 32, 64, 128 or 256
 };

@@ -18,6 +18,9 @@
 //    most two contiguous runs of ring slots (one wrap) or LDS rows.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "ddm_internal.h"
 #include "ddm_kernels.h"
 
@@ -61,6 +64,25 @@ __device__ __forceinline__ void lds_barrier() {
 // Relative cost of a pair in the register window and in the history-major
 // path, for the split of the waves.
 constexpr int COST_DENSE = 4, COST_HIST = 9;
+
+// Work split of one lag launch, the same for every workgroup, computed on the
+// host from the lag list, the block size n and the history depth dmax.
+constexpr int PLAN_MAX_BLOCKS = 252;
+struct LagPlan {
+    int dense_lags;  // leading consecutive lags on the register window
+    int wd;          // window waves: chunk w / sd, segment w % sd
+    int sd;          // segments per window chunk
+    int ss;          // units per other lag; 0: one wave per lag, direct accumulation
+    int hist;        // the other lags go history-major
+    int pmin, pmax;  // history-major frame range
+    int listed;      // blk_start / blk_list hold each history wave's blocks
+    unsigned char blk_start[LAG_WAVES + 1];   // wave k: blk_list[blk_start[k] .. blk_start[k + 1])
+    unsigned char blk_list[PLAN_MAX_BLOCKS];  // block b: frames pmin + b * HIST_BLOCK ...
+};
+// Host estimate of a history block's cost in VALU-instruction units: a block
+// costs its load (a full memory latency, the same however few lags reach
+// it), plus each lag that reaches it and each pair.
+constexpr int PLAN_COST_BLOCK = 200, PLAN_COST_VISIT = 10, PLAN_COST_PAIR = 4;
 
 // The R ring slots of one ring group, read with buffer loads: the lane's
 // offset in the vector operand, the wave-uniform slot in the scalar one.  The
@@ -244,7 +266,7 @@ __device__ __forceinline__ void dense_sums(const GroupRing &g, const float2 *nl,
 // The window for kc live lags: K = D / 2, 3D / 4, 7D / 8 or D, the smallest
 // that covers kc (a wider window computes lags that are thrown away).
 template <int D>
-__device__ __forceinline__ int window_width(int kc) {
+__host__ __device__ inline int window_width(int kc) {
     if (D < 16) return D;
     return kc <= D / 2 ? D / 2 : (kc <= 3 * D / 4 ? 3 * D / 4 : (kc <= 7 * D / 8 ? 7 * D / 8 : D));
 }
@@ -330,26 +352,30 @@ __device__ __forceinline__ void hist_block(const float2 *nl, const float2 (&fp)[
 
 // Lags tau_0 < ... < tau_{ls-1} (lane l of `vtau` holds tau_l) by history
 // frame: frame p of the window (ring history for p < 0, LDS for p >= 0)
-// pairs with the new frames j = p + tau_l inside [0, n).  The wave takes
-// blocks of B consecutive p (block b = k, k + ws, ...), loads each block once
-// into registers and pairs it with the LDS rows of every lag that reaches it,
-// so a history frame is requested once per wave rather than once per lag.
-// s[l] collects lag l in a fixed block order.
+// pairs with the new frames j = p + tau_l inside [0, n).  The frames
+// [pmin, pmax] are cut into blocks of B; history wave k takes the blocks the
+// launch plan lists for it (cost-balanced on the host), or blocks k, k + ws,
+// ... when the plan has no list.  Each block is loaded once into registers
+// and paired with the LDS rows of every lag that reaches it, so a history
+// frame is requested once per wave rather than once per lag.  s[l] collects
+// lag l in a fixed block order.
 template <int B, int LSM>
-__device__ __forceinline__ void hist_sums(const GroupRing &g, const float2 *nl, int R, int slot0, int n, int dmax,
-                                          int vtau, int ls, int k, int ws, float (&s)[LSM]) {
-    const int tmin = __builtin_amdgcn_readlane(vtau, 0), tmax = __builtin_amdgcn_readlane(vtau, ls - 1);
-    const int pmin = max(-dmax, -tmax), pmax = n - 1 - tmin;
+__device__ __forceinline__ void hist_sums(const GroupRing &g, const float2 *nl, int R, int slot0, int n, int vtau,
+                                          int ls, const LagPlan &pl, int k, int ws, float (&s)[LSM]) {
     const bool live = (int)(threadIdx.x % RING_GROUP) < ls;
-    int p0 = pmin + k * B;
-    if (p0 > pmax) return;
     float2 fa[B];
-    hist_load<B>(g, nl, R, slot0, p0, pmax, fa);
-    for (;;) {
+    if (pl.listed) {
+        const int i1 = pl.blk_start[k + 1];
+        for (int i = pl.blk_start[k]; i < i1; ++i) {
+            const int p0 = pl.pmin + (int)pl.blk_list[i] * B;
+            hist_load<B>(g, nl, R, slot0, p0, pl.pmax, fa);
+            hist_block<B>(nl, fa, p0, n, vtau, live, s);
+        }
+        return;
+    }
+    for (int p0 = pl.pmin + k * B; p0 <= pl.pmax; p0 += ws * B) {
+        hist_load<B>(g, nl, R, slot0, p0, pl.pmax, fa);
         hist_block<B>(nl, fa, p0, n, vtau, live, s);
-        p0 += ws * B;
-        if (p0 > pmax) break;
-        hist_load<B>(g, nl, R, slot0, p0, pmax, fa);
     }
 }
 
@@ -431,27 +457,25 @@ __device__ __forceinline__ void hist_reduce(float2 *nb, float *accg, long long E
 // E: ring / accumulator row stride (padded), R: ring slots, slot0: slot of
 // t0, dmax = min(max lag, t0): oldest history frame the block may pair with.
 //
-// Work split over the 16 waves (identical in every workgroup):
-//  * dense part (J >= 64): the leading run of consecutive lags
-//    (lags[i] = lags[0] + i, DENSE/2 to 8 * DENSE of them, at most 64) in
-//    chunks of DENSE lags; each chunk is cut into sd segments of new frames,
-//    one wave per (chunk, segment);
-//  * the other ls lags on the remaining ws waves: each lag is cut into ss
-//    segments ("units"); unit u goes to wave u % ws as its partial slot
-//    u / ws.  With more lags than partial slots each lag is instead owned by
-//    one wave that adds its sum to the accumulator directly.
-// Dense waves are weighted by their cheaper pairs (one load per DENSE pairs).
+// Work split over the 16 waves (identical in every workgroup, from the
+// host-made plan `pl`, see make_plan):
+//  * dense part: the leading run of consecutive lags in chunks of DENSE
+//    lags; each chunk is cut into sd segments of new frames, one wave per
+//    (chunk, segment);
+//  * the other ls lags on the remaining ws waves, history-major (blocks of
+//    history frames from the plan's per-wave lists), or each lag cut into ss
+//    segments ("units"; unit u goes to wave u % ws as its partial slot
+//    u / ws), or, with more lags than partial slots, each lag owned by one
+//    wave that adds its sum to the accumulator directly.
 // Partial sums meet in LDS after the pair work and are added per lag in a
 // fixed order, so results do not depend on timing.
 template <int J>
 __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J>::WAVES_PER_EU))) void lag_kernel(
     const float2 *__restrict__ ring, float *__restrict__ acc, long long E, int R, int slot0, int n, int dmax,
-    const int *__restrict__ lags, int L) {
+    const int *__restrict__ lags, int L, const LagPlan pl) {
     using C = LagCfg<J>;
     constexpr int DENSE = C::DENSE, U = C::UNROLL;
-    constexpr int NSEG = J / DENSE_SEG;  // most segments per lag
     constexpr int SLOTS = (J / 8) < C::SLOTS_MAX ? (J / 8) : C::SLOTS_MAX;  // partials per wave that fit in LDS
-    constexpr int MAX_DENSE = (8 * DENSE) < RING_GROUP ? (8 * DENSE) : RING_GROUP;
     static_assert(LAG_WAVES * SLOTS * RING_GROUP <= J * RING_GROUP * 2, "partials exceed the staging buffer");
     __shared__ float2 nb[J * RING_GROUP];
     const int tid = threadIdx.x;
@@ -480,41 +504,12 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
             if (j < n) nb[j * RING_GROUP + lane] = v[k];
         }
     }
-    // ---- plan (wave-uniform) ----
-    int run;  // length of the leading run of consecutive lags (<= 64)
-    {
-        const int first = lags[0];
-        const bool in_run = lane < L && lags[lane < L ? lane : 0] == first + lane;
-        const unsigned long long miss = ~__ballot(in_run);
-        run = miss ? __builtin_ctzll(miss) : RING_GROUP;
-        run = __builtin_amdgcn_readfirstlane(run);
-    }
-    int dense_lags = (SLOTS >= DENSE && run >= DENSE / 2) ? min(run, MAX_DENSE) : 0;
-    if (C::HIST_LAGS && dense_lags > DENSE && dense_lags % DENSE < DENSE / 2)
-        dense_lags -= dense_lags % DENSE;  // a short last window costs more than its lags elsewhere
-    const int nchunks = (dense_lags + DENSE - 1) / DENSE;
+    // ---- plan (host-computed, wave-uniform) ----
+    const int dense_lags = pl.dense_lags;
     const int ls = L - dense_lags;
-    const bool hist = C::HIST_LAGS && ls > 0 && ls <= C::HIST_LAGS;
-    int wd = 0;  // dense waves
-    if (nchunks) {
-        // relative pair costs: the window computes whole chunks of DENSE lags
-        const int wdense = hist ? COST_DENSE * ((nchunks - 1) * DENSE + window_width<DENSE>(dense_lags - (nchunks - 1) * DENSE))
-                                : 3 * dense_lags;
-        const int wother = hist ? COST_HIST * ls : 5 * ls;
-        wd = (LAG_WAVES * wdense + (wdense + wother) / 2) / (wdense + wother);
-        wd = max(wd, nchunks);
-        wd = min(wd, ls ? LAG_WAVES - 1 : LAG_WAVES);
-        wd -= wd % nchunks;
-    }
-    const int sd = nchunks ? wd / nchunks : 1;
+    const bool hist = C::HIST_LAGS && pl.hist;
+    const int wd = pl.wd, sd = pl.sd, ss = pl.ss;
     const int ws = LAG_WAVES - wd;
-    int ss = 0;  // segments per other lag; 0: one wave per lag, direct accumulation
-    if (!hist)
-        for (int cand = NSEG; cand >= 1; cand /= 2)
-            if (ls * cand <= SLOTS * ws) {
-                ss = cand;
-                break;
-            }
     // the accumulators this wave will add to are requested beside the staging
     // loads, so they have arrived when the partial sums are added
     float *accg = acc + (long long)blockIdx.x * RING_GROUP + lane;
@@ -548,7 +543,7 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
         }
     } else if (hist) {
         const int vtau = lags[dense_lags + (lane < ls ? lane : 0)];
-        hist_sums<C::HIST_BLOCK>(g, nl, R, slot0, n, dmax, vtau, ls, w - wd, ws, h);
+        hist_sums<C::HIST_BLOCK>(g, nl, R, slot0, n, vtau, ls, pl, w - wd, ws, h);
     } else if (ss) {
         const int k = w - wd, units = ls * ss, len = (n + ss - 1) / ss;
 #pragma unroll 1
@@ -598,16 +593,100 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
     }
 }
 
+// The launch plan (same rules for every workgroup):
+//  * the leading run of consecutive lags (lags[i] = lags[0] + i) of DENSE/2
+//    to 8 * DENSE lags (at most 64) goes on the register window, cut back to
+//    whole windows when the last one would be short and history-major lags
+//    take the rest;
+//  * waves are split between window chunks and the other lags by relative
+//    pair cost; the other lags go history-major when there are at most
+//    HIST_LAGS of them, else each is cut into ss units (or owned by one wave
+//    when even one unit per lag exceeds the partial slots);
+//  * history-major blocks are handed out greedily, heaviest-loaded last
+//    block first, each to the wave with the least estimated work, so the
+//    slowest history wave (which ends the pair phase) carries little more
+//    than the mean.
+template <int J>
+static void make_plan(const int *lags, int L, int n, int dmax, LagPlan &pl) {
+    using C = LagCfg<J>;
+    constexpr int DENSE = C::DENSE, B = C::HIST_BLOCK;
+    constexpr int NSEG = J / DENSE_SEG;
+    constexpr int SLOTS = (J / 8) < C::SLOTS_MAX ? (J / 8) : C::SLOTS_MAX;
+    constexpr int MAX_DENSE = (8 * DENSE) < RING_GROUP ? (8 * DENSE) : RING_GROUP;
+    pl = LagPlan{};
+    int run = 1;
+    while (run < L && run < RING_GROUP && lags[run] == lags[0] + run) ++run;
+    int dense_lags = (SLOTS >= DENSE && run >= DENSE / 2) ? std::min(run, MAX_DENSE) : 0;
+    if (C::HIST_LAGS && dense_lags > DENSE && dense_lags % DENSE < DENSE / 2) dense_lags -= dense_lags % DENSE;
+    const int nchunks = (dense_lags + DENSE - 1) / DENSE;
+    const int ls = L - dense_lags;
+    const bool hist = C::HIST_LAGS && ls > 0 && ls <= C::HIST_LAGS;
+    int wd = 0;
+    if (nchunks) {
+        const int wdense =
+            hist ? COST_DENSE * ((nchunks - 1) * DENSE + window_width<DENSE>(dense_lags - (nchunks - 1) * DENSE))
+                 : 3 * dense_lags;
+        const int wother = hist ? COST_HIST * ls : 5 * ls;
+        wd = (LAG_WAVES * wdense + (wdense + wother) / 2) / (wdense + wother);
+        wd = std::max(wd, nchunks);
+        wd = std::min(wd, ls ? LAG_WAVES - 1 : LAG_WAVES);
+        wd -= wd % nchunks;
+    }
+    const int ws = LAG_WAVES - wd;
+    int ss = 0;
+    if (!hist)
+        for (int cand = NSEG; cand >= 1; cand /= 2)
+            if (ls * cand <= SLOTS * ws) {
+                ss = cand;
+                break;
+            }
+    pl.dense_lags = dense_lags;
+    pl.wd = wd;
+    pl.sd = nchunks ? wd / nchunks : 1;
+    pl.ss = ss;
+    pl.hist = hist;
+    if (!hist) return;
+    const int tmin = lags[dense_lags], tmax = lags[L - 1];
+    pl.pmin = std::max(-dmax, -tmax);
+    pl.pmax = n - 1 - tmin;
+    const int nb = pl.pmax >= pl.pmin ? (pl.pmax - pl.pmin) / B + 1 : 0;
+    if (nb > PLAN_MAX_BLOCKS) return;  // round-robin blocks
+    std::vector<long long> load(ws, 0);
+    std::vector<int> owner(nb);
+    for (int b = nb - 1; b >= 0; --b) {
+        const int p0 = pl.pmin + b * B;
+        long long cost = PLAN_COST_BLOCK;
+        for (int l = dense_lags; l < L; ++l) {
+            const int j0 = p0 + lags[l];
+            const int lo = std::max(0, -j0), hi = std::min(B, n - j0);
+            if (hi > lo) cost += PLAN_COST_VISIT + (long long)PLAN_COST_PAIR * (hi - lo);
+        }
+        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        owner[b] = k;
+        load[k] += cost;
+    }
+    int at = 0;
+    for (int k = 0; k < ws; ++k) {
+        pl.blk_start[k] = (unsigned char)at;
+        for (int b = 0; b < nb; ++b)
+            if (owner[b] == k) pl.blk_list[at++] = (unsigned char)b;
+    }
+    for (int k = ws; k <= LAG_WAVES; ++k) pl.blk_start[k] = (unsigned char)at;
+    pl.listed = 1;
+}
+
 template <int J>
 static void launch_lag_t(const LagArgs &a, hipStream_t st) {
+    LagPlan pl;
+    make_plan<J>(a.lags_host, a.lag_count, a.n_new, a.dmax, pl);
     const unsigned grid = (unsigned)(a.elems_per_frame / RING_GROUP);
     hipLaunchKernelGGL(lag_kernel<J>, dim3(grid), dim3(LAG_NT), 0, st, (const float2 *)a.ring, a.acc,
-                       a.elems_per_frame, a.ring_slots, a.slot0, a.n_new, a.dmax, a.lags, a.lag_count);
+                       a.elems_per_frame, a.ring_slots, a.slot0, a.n_new, a.dmax, a.lags, a.lag_count, pl);
 }
 
 hipError_t launch_lag(const LagArgs &a, hipStream_t st) {
     if (a.elems_per_frame % RING_GROUP != 0 || a.n_new < 1 || a.n_new > a.lag_block || a.dmax < 0 ||
-        a.slot0 < 0 || a.slot0 >= a.ring_slots)
+        a.slot0 < 0 || a.slot0 >= a.ring_slots || !a.lags_host || a.lag_count < 1)
         return hipErrorInvalidValue;
     switch (a.lag_block) {
     case 32: launch_lag_t<32>(a, st); break;

@@ -269,7 +269,9 @@ int ddm_k_tile_fft(const void *frames, int n_frames, long long frame_bytes,
  * src/DDM.cu:222-262): for the n_new frames first_frame .. first_frame+n_new-1
  * held in the ring and every lag l, acc[l][e] += sum |F_t[e] - F_{t-lag}[e]|^2
  * over those t with t - lag >= 0.  No 1/s^2 factor is applied.  lags is a
- * device array of lag_count ints, all <= max_lag; the ring must hold frames
+ * device array of lag_count strictly ascending ints in [0, max_lag] (read
+ * back to the host for the launch plan, so the call waits for the stream);
+ * the ring must hold frames
  * first_frame - max_lag .. first_frame + n_new - 1 (max_lag + n_new <=
  * ring_slots).  n_new <= lag_block (32, 64, 128 or 256). */
 int ddm_k_lag_accumulate(const float *ring, int ring_slots, long long elems_per_frame,
