@@ -258,3 +258,6 @@ def test_stage_entry_points_reject_bad_arguments(gpu):
         ddm.k_lag_accumulate(big, 40, 60, 0, 4, lags, 1, big, 32)
     with pytest.raises(ddm.DDMError):  # unsupported lag block
         ddm.k_lag_accumulate(big, 40, 64, 0, 4, lags, 1, big, 16)
+    for bad in ([2, 1], [1, 1], [1, 5]):  # not strictly ascending, or above max_lag
+        with pytest.raises(ddm.DDMError):
+            ddm.k_lag_accumulate(big, 40, 64, 0, 4, torch.tensor(bad, dtype=torch.int32, device=gpu), 4, big, 32)
