@@ -63,7 +63,7 @@ __device__ __forceinline__ void lds_barrier() {
 
 // Relative cost of a pair in the register window and in the history-major
 // path, for the split of the waves.
-constexpr int COST_DENSE = 4, COST_HIST = 9;
+constexpr int COST_DENSE = 4, COST_HIST = 8;
 
 // Work split of one lag launch, the same for every workgroup, computed on the
 // host from the lag list, the block size n and the history depth dmax.
@@ -327,9 +327,14 @@ __device__ __forceinline__ void hist_block(const float2 *nl, const float2 (&fp)[
         const float2 *a = nl + j0 * RING_GROUP;
         float t[4] = {0.f, 0.f, 0.f, 0.f};
         if (j0 >= 0 && j0 + B <= n) {
+            // all B rows requested before the first pair: the reads overlap
+            // (the compiler otherwise keeps a few in flight to save registers)
+            float2 va[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) va[u] = a[u * RING_GROUP];
 #pragma unroll
             for (int u = 0; u < B; ++u) {
-                const float2 v = a[u * RING_GROUP];
+                const float2 v = va[u];
                 const float dx = v.x - fp[u].x, dy = v.y - fp[u].y;
                 t[u & 3] = fmaf(dx, dx, t[u & 3]);
                 t[u & 3] = fmaf(dy, dy, t[u & 3]);
