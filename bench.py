@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MI355X_CUS, MI355X_CLOCK_HZ = 256, 2.4e9  # compute units, peak engine clock
 
 
 def pmc_traffic(workload: str, kernel: str):
@@ -331,6 +332,12 @@ def main():
     pipe_gbs = pipe_bytes / (elapsed / args.steps) / 1e9
 
     traffic, traffic_src = pmc_traffic(args.workload, f"lag_kernel<{lag_block}>")
+    # the lag engine's compute floor: every valid pair costs four FP32 VALU
+    # instructions per element (two subtractions, two fused multiply-adds),
+    # and a CU issues at most one wave64 VALU instruction per clock
+    # (4 SIMDs x 16 lanes) at the 2.4 GHz peak engine clock
+    pairs = sum(max(0, N - tau) for tau in lags) * E
+    valu_floor_ms = pairs * 4 / 64 / lag_launches / (MI355X_CUS * MI355X_CLOCK_HZ) * 1e3
     out = {
         "metric": "analysed frames/s (all scales x lags) + p99 chunk latency, at <=1e-4 err vs FP64",
         "value": round(fps, 2),
@@ -373,6 +380,10 @@ def main():
             "algorithmic_bytes_per_launch": int(lag_bytes / lag_launches),
             "avg_launch_ms": round(lag_avg_ms, 4),
             "launches": lag_launches,
+            "valu_floor": {"pairs_per_launch": int(pairs / lag_launches), "ms": round(valu_floor_ms, 4),
+                           "frac": round(valu_floor_ms / lag_avg_ms, 4),
+                           "assumes": "4 FP32 VALU instructions per pair and element, one wave64 VALU "
+                                      "instruction per CU per clock, 256 CUs at 2.4 GHz"},
         },
         "pipeline_algorithmic_GBps": round(pipe_gbs, 1),
         "precision": args.precision,
