@@ -42,6 +42,7 @@ struct Mark {
 // spectra of a batch (1 MB a frame) still fit in the last-level cache for
 // the column pass.  ddm_autotune may choose another multiple of the chunk.
 int default_fft_batch(int chunk) { return chunk * std::max(1, 128 / chunk); }
+constexpr int NSTAGE = ddm::UPLOAD_STAGES;
 
 }  // namespace
 
@@ -60,12 +61,16 @@ struct ddm_engine {
     double *d_tau = nullptr, *d_fit = nullptr;  // allocated by the first ddm_fit_results
     float *d_xt = nullptr;    // row spectra of 256 / 512 tiles between FFT passes
     long long xt_elems = 0;   // complex values d_xt holds (grown on demand)
-    uint8_t *d_stage[2] = {nullptr, nullptr};
-    uint8_t *h_stage[2] = {nullptr, nullptr};
-    bool stage_used[2] = {false, false};
+    // upload staging for host frames: NSTAGE device buffers used in turn (an
+    // upload need not wait for the frames two uploads back to be consumed
+    // while a lag launch holds the stream), stage_frames frames each
+    uint8_t *d_stage[NSTAGE] = {};
+    uint8_t *h_stage[NSTAGE] = {};
+    bool stage_used[NSTAGE] = {};
     int next_stage = 0;
+    int stage_frames = 0;
     hipStream_t st = nullptr, st_copy = nullptr;
-    hipEvent_t ev_up[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+    hipEvent_t ev_up[NSTAGE] = {}, ev_free[NSTAGE] = {};
     long long frames = 0, frames_since_reset = 0;
     long long frames_lagged = 0;  // frames the lag engine has processed (<= frames)
     std::vector<long long> pairs;
@@ -158,12 +163,14 @@ static void release(ddm_engine *e) {
     e->pool.clear();
     // device buffers come from the stream-ordered pool: freeing them does not
     // synchronise the device (other engines keep running)
-    void *dev[] = {e->d_ring, e->d_acc, e->d_lags,     e->d_az,       e->d_qoff,  e->d_qcnt, e->d_idx,
-                   e->d_norm, e->d_out, e->d_stage[0], e->d_stage[1], e->d_tau,   e->d_fit, e->d_xt};
+    void *dev[] = {e->d_ring, e->d_acc, e->d_norm, e->d_lags, e->d_az,  e->d_qoff,
+                   e->d_qcnt, e->d_idx, e->d_out,  e->d_tau,  e->d_fit, e->d_xt};
     for (void *ptr : dev)
         if (ptr && e->st) (void)hipFreeAsync(ptr, e->st);
+    for (uint8_t *ptr : e->d_stage)
+        if (ptr && e->st) (void)hipFreeAsync(ptr, e->st);
     if (e->st) (void)hipStreamSynchronize(e->st);
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < NSTAGE; ++i) {
         if (e->h_stage[i]) (void)hipHostFree(e->h_stage[i]);
         if (e->ev_up[i]) (void)hipEventDestroy(e->ev_up[i]);
         if (e->ev_free[i]) (void)hipEventDestroy(e->ev_free[i]);
@@ -232,7 +239,7 @@ extern "C" int ddm_create(const ddm_desc *desc, ddm_engine **out) {
     const size_t L = p.lags.size();
     HTRY(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
     HTRY(hipStreamCreateWithFlags(&e->st_copy, hipStreamNonBlocking));
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < NSTAGE; ++i) {
         HTRY(hipEventCreateWithFlags(&e->ev_up[i], hipEventDisableTiming));
         HTRY(hipEventCreateWithFlags(&e->ev_free[i], hipEventDisableTiming));
     }
@@ -247,7 +254,8 @@ extern "C" int ddm_create(const ddm_desc *desc, ddm_engine **out) {
     TRY(dev_alloc(e, &e->d_out, (size_t)p.result_count, "results"));
     const size_t roi_frame = (size_t)p.M * p.M * p.bpp;
     // pinned host staging is allocated on the first host submission that needs it
-    for (int i = 0; i < 2; ++i) TRY(dev_alloc(e, &e->d_stage[i], roi_frame * p.chunk, "upload staging"));
+    for (int i = 0; i < NSTAGE; ++i) TRY(dev_alloc(e, &e->d_stage[i], roi_frame * p.chunk, "upload staging"));
+    e->stage_frames = p.chunk;
     std::vector<ddm::AzScale> az;
     long long out_base = 0;
     for (const ddm::ScaleGeom &g : p.geom) {
@@ -420,6 +428,32 @@ extern "C" int ddm_submit_device(ddm_engine *e, const void *frames, int n_frames
     return DDM_OK;
 }
 
+// Staging buffers of `frames` frames each (a call that hands over many frames
+// uploads and transforms them a whole FFT batch at a time).  The old buffers
+// are released once the work that reads them is done.
+static int grow_stage(ddm_engine *e, int frames) {
+    const size_t roi_frame = (size_t)e->p.M * e->p.M * e->p.bpp;
+    for (int b = 0; b < NSTAGE; ++b) {
+        if (e->stage_used[b]) HIP_TRY(e, hipEventSynchronize(e->ev_free[b]));
+        if (e->d_stage[b]) {
+            HIP_TRY(e, hipFreeAsync(e->d_stage[b], e->st));
+            e->dev_bytes -= roi_frame * e->stage_frames;
+            e->d_stage[b] = nullptr;
+        }
+        if (e->h_stage[b]) {
+            HIP_TRY(e, hipHostFree(e->h_stage[b]));
+            e->pinned_bytes -= roi_frame * e->stage_frames;
+            e->h_stage[b] = nullptr;
+        }
+    }
+    for (int b = 0; b < NSTAGE; ++b) {
+        int rc = dev_alloc(e, &e->d_stage[b], roi_frame * frames, "upload staging");
+        if (rc) return rc;
+    }
+    e->stage_frames = frames;
+    return DDM_OK;
+}
+
 extern "C" int ddm_submit_host(ddm_engine *e, const void *frames, int n_frames) {
     if (!e) return DDM_EINVAL;
     if (n_frames < 0 || (n_frames > 0 && !frames)) return fail(e, DDM_EINVAL, "bad frame buffer");
@@ -438,8 +472,14 @@ extern "C" int ddm_submit_host(ddm_engine *e, const void *frames, int n_frames) 
     }
     const bool whole = p.x_off == 0 && p.y_off == 0 && p.width == p.M && p.height == p.M;
     const bool rows_stack = p.y_off == 0 && p.height == p.M;  // ROI rows of consecutive frames are evenly spaced
-    for (int i = 0; i < n_frames; i += p.chunk) {
-        const int n = std::min(p.chunk, n_frames - i);
+    if (n_frames > e->stage_frames && e->stage_frames < e->fft_batch) {
+        const int want = std::min(e->fft_batch, (n_frames + p.chunk - 1) / p.chunk * p.chunk);
+        int rc = grow_stage(e, want);
+        if (rc) return rc;
+    }
+    int last = -1;
+    for (int i = 0; i < n_frames;) {
+        const int n = std::min({e->stage_frames, n_frames - i, batch_room(e)});
         const int b = e->next_stage;
         if (e->stage_used[b]) HIP_TRY(e, hipEventSynchronize(e->ev_free[b]));
         if (direct && whole) {
@@ -459,8 +499,8 @@ extern "C" int ddm_submit_host(ddm_engine *e, const void *frames, int n_frames) 
             }
         } else {
             if (!e->h_stage[b]) {
-                HIP_TRY(e, hipHostMalloc((void **)&e->h_stage[b], roi_frame * p.chunk, hipHostMallocDefault));
-                e->pinned_bytes += roi_frame * p.chunk;
+                HIP_TRY(e, hipHostMalloc((void **)&e->h_stage[b], roi_frame * e->stage_frames, hipHostMallocDefault));
+                e->pinned_bytes += roi_frame * e->stage_frames;
             }
             // ROI-only copy into pinned memory (the reference uploads whole frames, src/DDM.cu:506)
             uint8_t *dst = e->h_stage[b];
@@ -473,14 +513,17 @@ extern "C" int ddm_submit_host(ddm_engine *e, const void *frames, int n_frames) 
             HIP_TRY(e, hipMemcpyAsync(e->d_stage[b], dst, roi_frame * n, hipMemcpyHostToDevice, e->st_copy));
         }
         HIP_TRY(e, hipEventRecord(e->ev_up[b], e->st_copy));
-        // the caller may reuse its buffer as soon as this call returns
-        if (direct) HIP_TRY(e, hipEventSynchronize(e->ev_up[b]));
         HIP_TRY(e, hipStreamWaitEvent(e->st, e->ev_up[b], 0));
         int rc = process_chunk(e, e->d_stage[b], n, (long long)roi_frame, p.M, 0, 0, e->ev_free[b]);
         if (rc) return rc;
         e->stage_used[b] = true;
-        e->next_stage ^= 1;
+        e->next_stage = (b + 1) % NSTAGE;
+        last = b;
+        i += n;
     }
+    // the caller may reuse its buffer as soon as this call returns (uploads
+    // run in order on the copy stream: the last one done means all are)
+    if (direct && last >= 0) HIP_TRY(e, hipEventSynchronize(e->ev_up[last]));
     return DDM_OK;
 }
 

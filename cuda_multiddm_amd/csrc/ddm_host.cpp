@@ -126,7 +126,7 @@ size_t device_footprint(const Params &p) {
     const size_t L = p.lags.size(), Q = p.lambdas.size();
     size_t bytes = (size_t)p.ring_slots * (size_t)p.elems_padded * 8;  // spectrum ring
     bytes += L * (size_t)p.elems_padded * 4;                           // accumulators
-    bytes += 2 * (size_t)p.chunk * p.M * p.M * p.bpp;                  // upload staging
+    bytes += UPLOAD_STAGES * (size_t)p.chunk * p.M * p.M * p.bpp;      // upload staging
     bytes += (size_t)p.result_count * 8 + L * 12;                      // results, lags, norms
     for (const ScaleGeom &g : p.geom) bytes += Q * ((size_t)g.tile_elems * 4 + 8);  // ring tables (bound)
     // between-pass buffer of 256 / 512 tile FFTs for one chunk (host input;

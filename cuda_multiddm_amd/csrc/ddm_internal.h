@@ -25,6 +25,9 @@ inline long long pad_elems(long long e) { return (e + RING_GROUP - 1) / RING_GRO
 // Complex values of the tile FFT's between-pass buffer (TileFftArgs::xt) a
 // batch of n frames needs for tiles of size `scale` cut from an M x M region
 // (0 for scales that do not use it).
+// Device buffers host frames are uploaded into, used in turn.
+constexpr int UPLOAD_STAGES = 3;
+
 inline long long big_fft_scratch_elems(int M, int scale, int n) {
     return (scale == 256 || scale == 512 || scale == 1024) ? (long long)n * M * M / 2 : 0;
 }
