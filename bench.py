@@ -115,7 +115,7 @@ def serving(args):
 
     def desc(s):
         sc = [1 << k for k in range(int(np.log2(s)), 4, -1)]
-        return dict(width=s, height=s, scales=sc, lags=lags, lambdas=lams, chunk=30, lag_block=32)
+        return dict(width=s, height=s, scales=sc, lags=lags, lambdas=lams, chunk=30, lag_block=args.serving_lag_block)
 
     srv = ddm.Server(max_streams=args.streams)
     waiting = list(range(args.streams))
@@ -192,7 +192,7 @@ def serving(args):
         "device_used_GB_at_end": round(st.used_bytes / 1e9, 1),
         "host_seconds": {k: round(v, 2) for k, v in host_s.items()},
         "config": {"workload": f"{args.streams} streams, frame sizes {{256,512,1024}}^2 u8, scales to 32, "
-                               f"24 log lags <= 300, 30 log lambdas 4-256, chunk 30, lag block 32, "
+                               f"24 log lags <= 300, 30 log lambdas 4-256, chunk 30, lag block {args.serving_lag_block}, "
                                f"Poisson arrivals {args.rate} chunks/s"},
     }
     srv.close()
@@ -225,6 +225,8 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0, help="serving: offered chunks per second")
     ap.add_argument("--seconds", type=float, default=20.0, help="serving: arrival window")
     ap.add_argument("--chunks-per-stream", type=int, default=12)
+    ap.add_argument("--serving-lag-block", type=int, default=32, choices=[32, 64, 128, 256],
+                    help="serving: each stream's lag block (every chunk is flushed through the lag engine)")
     args = ap.parse_args()
     if args.serving and args.gpus == 1:
         serving(args)

@@ -150,10 +150,25 @@ extern "C" int ddm_server_open(ddm_server *s, const ddm_desc *desc, int *stream_
     }
     ddm_engine *e = nullptr;
     rc = ddm_create(desc, &e);
+    if (rc == DDM_ENOMEM) {
+        // the device had less room than the budget assumed; the pool may be
+        // holding freed blocks of other sizes: return them and try once more
+        int dev = 0;
+        hipMemPool_t pool = nullptr;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess)
+            (void)hipMemPoolTrimTo(pool, 0);
+        rc = ddm_create(desc, &e);
+    }
     std::lock_guard<std::mutex> lk(s->mu);
     if (rc != DDM_OK) {
         s->used -= need;
         s->err = ddm_last_error(nullptr);
+        if (rc == DDM_ENOMEM) {
+            // a refusal like any other over-budget request: the caller may
+            // try again when streams have left
+            s->err = "device memory exhausted below the budget (" + s->err + ")";
+            return DDM_EBUSY;
+        }
         return rc;
     }
     const int id = s->next_id++;
