@@ -56,6 +56,10 @@ struct ddm_engine {
     float *d_acc = nullptr;
     int *d_lags = nullptr;
     ddm::AzScale *d_az = nullptr;
+    int *d_wl = nullptr;            // narrow-ring average waves: q | lag group << 16
+    long long az_wide_items = 0;    // averages of the large-ring scales (a workgroup each)
+    int az_s0 = 0;                  // first narrow-ring scale
+    long long az_narrow_waves = 0;
     int *d_qoff = nullptr, *d_qcnt = nullptr, *d_idx = nullptr;
     double *d_norm = nullptr, *d_out = nullptr;
     double *d_tau = nullptr, *d_fit = nullptr;  // allocated by the first ddm_fit_results
@@ -163,8 +167,8 @@ static void release(ddm_engine *e) {
     e->pool.clear();
     // device buffers come from the stream-ordered pool: freeing them does not
     // synchronise the device (other engines keep running)
-    void *dev[] = {e->d_ring, e->d_acc, e->d_norm, e->d_lags, e->d_az,  e->d_qoff,
-                   e->d_qcnt, e->d_idx, e->d_out,  e->d_tau,  e->d_fit, e->d_xt};
+    void *dev[] = {e->d_ring, e->d_acc, e->d_norm, e->d_lags, e->d_az,  e->d_qoff, e->d_qcnt,
+                   e->d_idx,  e->d_out, e->d_tau,  e->d_fit,  e->d_xt, e->d_wl};
     for (void *ptr : dev)
         if (ptr && e->st) (void)hipFreeAsync(ptr, e->st);
     for (uint8_t *ptr : e->d_stage)
@@ -268,10 +272,39 @@ extern "C" int ddm_create(const ddm_desc *desc, ddm_engine **out) {
         az.push_back(a);
         out_base += (long long)g.tiles * (long long)p.lambdas.size() * (long long)L;
     }
+    // Averages: the scales whose largest ring has AZ_WIDE_RING elements or
+    // more (the largest scales, first) take a workgroup per value; the others
+    // go to the narrow kernel, a wave per (tile, q, group of lags).
+    std::vector<int> wl;
+    {
+        const size_t Q = p.lambdas.size();
+        size_t s0 = 0;
+        for (; s0 < p.geom.size(); ++s0) {
+            int big = 0;
+            for (size_t q = 0; q < Q; ++q) big = std::max(big, e->rt.cnt[s0 * Q + q]);
+            if (big < ddm::AZ_WIDE_RING) break;
+            e->az_wide_items += (long long)p.geom[s0].tiles * (long long)Q * (long long)L;
+        }
+        e->az_s0 = (int)s0;
+        if (Q < 0x10000) {
+            for (size_t si = s0; si < p.geom.size(); ++si) {
+                az[si].wave_base = e->az_narrow_waves;
+                az[si].wl_off = (int)wl.size();
+                for (size_t q = 0; q < Q; ++q) {
+                    const int G = 64 / ddm::az_lanes(e->rt.cnt[si * Q + q]);
+                    for (int grp = 0; grp * G < (int)L; ++grp) wl.push_back((int)q | (grp << 16));
+                }
+                az[si].wl_n = (int)wl.size() - az[si].wl_off;
+                e->az_narrow_waves += (long long)p.geom[si].tiles * az[si].wl_n;
+            }
+            if (!wl.empty()) TRY(dev_alloc(e, &e->d_wl, wl.size(), "average wave list"));
+        }
+    }
     // stream-ordered after the allocations; the host arrays live until the
     // synchronisation below
     HTRY(hipMemcpyAsync(e->d_lags, p.lags.data(), L * sizeof(int), hipMemcpyHostToDevice, e->st));
     HTRY(hipMemcpyAsync(e->d_az, az.data(), az.size() * sizeof(ddm::AzScale), hipMemcpyHostToDevice, e->st));
+    if (e->d_wl) HTRY(hipMemcpyAsync(e->d_wl, wl.data(), wl.size() * sizeof(int), hipMemcpyHostToDevice, e->st));
     HTRY(hipMemcpyAsync(e->d_qoff, e->rt.off.data(), e->rt.off.size() * sizeof(int), hipMemcpyHostToDevice, e->st));
     HTRY(hipMemcpyAsync(e->d_qcnt, e->rt.cnt.data(), e->rt.cnt.size() * sizeof(int), hipMemcpyHostToDevice, e->st));
     if (!e->rt.idx.empty())
@@ -571,18 +604,10 @@ static int compute_results(ddm_engine *e) {
     a.lag_count = (int)L;
     a.out = e->d_out;
     a.result_count = p.result_count;
-    // scales come largest first; those with a ring of AZ_WIDE_RING elements
-    // or more take a workgroup per value
-    a.wide_items = 0;
-    {
-        const size_t Q = p.lambdas.size();
-        for (size_t si = 0; si < p.geom.size(); ++si) {
-            int big = 0;
-            for (size_t q = 0; q < Q; ++q) big = std::max(big, e->rt.cnt[si * Q + q]);
-            if (big < ddm::AZ_WIDE_RING) break;
-            a.wide_items += (long long)p.geom[si].tiles * (long long)Q * (long long)L;
-        }
-    }
+    a.wide_items = e->az_wide_items;
+    a.narrow_first_scale = e->az_s0;
+    a.narrow_waves = e->az_narrow_waves;
+    a.wave_list = e->d_wl;
     Mark az;
     mark_begin(e, &az, MK_AZ);
     HIP_TRY(e, ddm::launch_azimuthal(a, e->st));

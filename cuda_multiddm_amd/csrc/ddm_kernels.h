@@ -47,7 +47,15 @@ struct AzScale {
     long long elem_off;
     long long tile_elems;
     double inv_s4;             // 1 / s^4: the reference's (1/s^2)^2 per-term factor
+    // scales with small rings: waves [wave_base, wave_base + tiles * wl_n)
+    // of the narrow launch, wave k of a tile taking entry wl[wl_off + k]
+    long long wave_base = 0;
+    int wl_off = 0, wl_n = 0;
 };
+
+// Narrow-ring averages: lanes per lag for a ring of cnt elements (the
+// smallest power of two >= cnt, 8 .. 64); a wave takes 64 / lanes lags.
+__host__ __device__ inline int az_lanes(int cnt) { return cnt <= 8 ? 8 : (cnt <= 16 ? 16 : (cnt <= 32 ? 32 : 64)); }
 
 struct AzArgs {
     const float *acc;
@@ -60,8 +68,12 @@ struct AzArgs {
     double *out;               // device [result_count]
     long long result_count;
     // values [0, wide_items) (the scales with large rings, which come first)
-    // take a workgroup each, the others a wave each
+    // take a workgroup each; the others go to narrow_waves waves, each taking
+    // one (tile, q) and a group of lags (entries of wave_list: q | group << 16)
     long long wide_items = 0;
+    int narrow_first_scale = 0;
+    long long narrow_waves = 0;
+    const int *wave_list = nullptr;  // device
 };
 
 // Ring size from which a scale's averages take a whole workgroup per value.

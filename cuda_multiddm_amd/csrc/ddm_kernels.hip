@@ -409,6 +409,42 @@ __global__ __launch_bounds__(AZ_WAVES * 64) void azimuthal_kernel(const float *_
     out[item] = 2.0 * sum * sc.inv_s4 / (double)cnt * norm[l];
 }
 
+// Narrow-ring scales: a wave per (tile, q, group of lags).  A ring of cnt
+// elements gets P = az_lanes(cnt) lanes per lag and the wave 64 / P lags;
+// lane (g, p) sums elements p, p + P, ... of lag 64 / P * group + g in FP64,
+// the P lanes of each lag are added by an xor butterfly, and the lane with
+// p = 0 writes the value.  Small rings thus fill the wave instead of one
+// lag leaving most lanes idle.
+__global__ __launch_bounds__(AZ_WAVES * 64) void azimuthal_narrow_kernel(
+    const float *__restrict__ acc, long long E, const AzScale *__restrict__ scales, int S, int s0,
+    const int *__restrict__ q_off, const int *__restrict__ q_cnt, const int *__restrict__ idx,
+    const int *__restrict__ wl, const double *__restrict__ norm, int Q, int L, double *__restrict__ out,
+    long long nwaves) {
+    const long long gw = (long long)blockIdx.x * AZ_WAVES + threadIdx.x / 64;
+    if (gw >= nwaves) return;
+    const int lane = threadIdx.x % 64;
+    int si = s0;
+    while (si + 1 < S && gw >= scales[si + 1].wave_base) ++si;
+    const AzScale sc = scales[si];
+    const long long local = gw - sc.wave_base;
+    const long long tile = local / sc.wl_n;
+    const int ent = wl[sc.wl_off + (int)(local - tile * sc.wl_n)];
+    const int q = ent & 0xffff, grp = ent >> 16;
+    const int cnt = q_cnt[si * Q + q];
+    const int P = az_lanes(cnt), G = 64 / P;
+    const int g = lane / P, p = lane % P;
+    const int l = grp * G + g;
+    double sum = 0.0;
+    if (l < L) {
+        const int *ix = idx + q_off[si * Q + q];
+        const float *a = acc + (long long)l * E + sc.elem_off + tile * sc.tile_elems;
+        for (int i = p; i < cnt; i += P) sum += (double)a[ix[i]];
+    }
+    for (int o = P / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    if (p == 0 && l < L)
+        out[sc.out_base + (tile * Q + q) * L + l] = cnt ? 2.0 * sum * sc.inv_s4 / (double)cnt * norm[l] : 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
@@ -504,7 +540,17 @@ static void launch_az_range(const AzArgs &a, long long first, long long end, hip
 hipError_t launch_azimuthal(const AzArgs &a, hipStream_t st) {
     const long long split = a.wide_items < a.result_count ? a.wide_items : a.result_count;
     if (split > 0) launch_az_range<AZ_WAVES>(a, 0, split, st);
-    if (split < a.result_count) launch_az_range<1>(a, split, a.result_count, st);
+    if (split < a.result_count) {
+        if (a.wave_list && a.narrow_waves > 0) {
+            const long long blocks = (a.narrow_waves + AZ_WAVES - 1) / AZ_WAVES;
+            if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+            hipLaunchKernelGGL(azimuthal_narrow_kernel, dim3((unsigned)blocks), dim3(AZ_WAVES * 64), 0, st, a.acc,
+                               a.elems_per_frame, a.scales, a.scale_count, a.narrow_first_scale, a.q_off, a.q_cnt,
+                               a.idx, a.wave_list, a.norm, a.lambda_count, a.lag_count, a.out, a.narrow_waves);
+        } else {
+            launch_az_range<1>(a, split, a.result_count, st);
+        }
+    }
     return hipGetLastError();
 }
 
