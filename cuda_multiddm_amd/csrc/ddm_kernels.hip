@@ -364,6 +364,24 @@ __global__ __launch_bounds__(LargeCfg<S>::NTH_COLS) void tile_fft_cols(RingDst r
 // ---------------------------------------------------------------------------
 constexpr int AZ_WAVES = 4;
 
+// sum_{i = first, first + step, ... < cnt} a[ix[i]] in FP64: four independent
+// partial sums, so four element loads per lane are in flight at once; the
+// order is fixed, so the result does not depend on timing.
+__device__ __forceinline__ double ring_sum(const float *__restrict__ a, const int *__restrict__ ix, int first,
+                                           int step, int cnt) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int i = first;
+    for (; i + 3 * step < cnt; i += 4 * step) {
+        const int i0 = ix[i], i1 = ix[i + step], i2 = ix[i + 2 * step], i3 = ix[i + 3 * step];
+        s0 += (double)a[i0];
+        s1 += (double)a[i1];
+        s2 += (double)a[i2];
+        s3 += (double)a[i3];
+    }
+    for (; i < cnt; i += step) s0 += (double)a[ix[i]];
+    return (s0 + s1) + (s2 + s3);
+}
+
 template <int WPR>
 __global__ __launch_bounds__(AZ_WAVES * 64) void azimuthal_kernel(const float *__restrict__ acc, long long E,
                                                                   const AzScale *__restrict__ scales, int S,
@@ -392,8 +410,7 @@ __global__ __launch_bounds__(AZ_WAVES * 64) void azimuthal_kernel(const float *_
     }
     const int *ix = idx + q_off[si * Q + q];
     const float *a = acc + (long long)l * E + sc.elem_off + tile * sc.tile_elems;
-    double sum = 0.0;
-    for (int i = tid; i < cnt; i += NTR) sum += (double)a[ix[i]];
+    double sum = ring_sum(a, ix, tid, NTR, cnt);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) sum += __shfl_down(sum, o, 64);
     if constexpr (WPR > 1) {
@@ -438,7 +455,7 @@ __global__ __launch_bounds__(AZ_WAVES * 64) void azimuthal_narrow_kernel(
     if (l < L) {
         const int *ix = idx + q_off[si * Q + q];
         const float *a = acc + (long long)l * E + sc.elem_off + tile * sc.tile_elems;
-        for (int i = p; i < cnt; i += P) sum += (double)a[ix[i]];
+        sum = ring_sum(a, ix, p, P, cnt);
     }
     for (int o = P / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
     if (p == 0 && l < L)
