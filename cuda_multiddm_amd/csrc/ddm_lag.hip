@@ -76,6 +76,7 @@ struct LagPlan {
     int hist;        // the other lags go history-major
     int pmin, pmax;  // history-major frame range
     int listed;      // blk_start / blk_list hold each history wave's blocks
+    int all_hist;    // the lists cover all LAG_WAVES waves (window waves take blocks after their window)
     unsigned char blk_start[LAG_WAVES + 1];   // wave k: blk_list[blk_start[k] .. blk_start[k + 1])
     unsigned char blk_list[PLAN_MAX_BLOCKS];  // block b: frames pmin + b * HIST_BLOCK ...
 };
@@ -83,6 +84,11 @@ struct LagPlan {
 // costs its load (a full memory latency, the same however few lags reach
 // it), plus each lag that reaches it and each pair.
 constexpr int PLAN_COST_BLOCK = 200, PLAN_COST_VISIT = 10, PLAN_COST_PAIR = 4;
+// a window wave's own work in the same units: per step its window's pairs at
+// COST_DENSE / COST_HIST of a history pair's cost, plus the step's loads
+constexpr int PLAN_COST_STEP = 8;
+constexpr bool PLAN_ALL_WAVES = true;
+constexpr int PLAN_DENSE_PCT = 70;  // scale of a window wave's starting load (measured best)
 
 // The R ring slots of one ring group, read with buffer loads: the lane's
 // offset in the vector operand, the wave-uniform slot in the scalar one.  The
@@ -358,9 +364,10 @@ __device__ __forceinline__ void hist_block(const float2 *nl, const float2 (&fp)[
 // Lags tau_0 < ... < tau_{ls-1} (lane l of `vtau` holds tau_l) by history
 // frame: frame p of the window (ring history for p < 0, LDS for p >= 0)
 // pairs with the new frames j = p + tau_l inside [0, n).  The frames
-// [pmin, pmax] are cut into blocks of B; history wave k takes the blocks the
-// launch plan lists for it (cost-balanced on the host), or blocks k, k + ws,
-// ... when the plan has no list.  Each block is loaded once into registers
+// [pmin, pmax] are cut into blocks of B; list k of the launch plan (k the
+// wave's index among the history waves, or among all waves when the plan
+// gives window waves blocks too) names the wave's blocks, cost-balanced on
+// the host; without lists history wave k takes blocks k, k + ws, ...  Each block is loaded once into registers
 // and paired with the LDS rows of every lag that reaches it, so a history
 // frame is requested once per wave rather than once per lag.  s[l] collects
 // lag l in a fixed block order.
@@ -417,31 +424,30 @@ __device__ __forceinline__ void add_acc(float *accg, long long E, int l, int m, 
 // r, history wave k writes its sums of lags 32r + i to rows wd*SL + 32k + i,
 // and in round 0 dense wave w also writes its window sums to rows w*SL + i.
 // Each lag is then added up by one wave in a fixed order.
-template <int LSM, int SL, int DN, int SM, int PF>
+template <int LSM, int SL, int DN, int RL, int SM, int PF>
 __device__ __forceinline__ void hist_reduce(float2 *nb, float *accg, long long E, const float (&s)[SM],
-                                            const float (&h)[LSM], int w, int wd, int ws, int sd, int dense, int ls,
-                                            const float (&pre)[PF]) {
-    static_assert(SL <= SM, "window sums exceed the partial registers");
+                                            const float (&h)[LSM], int w, int wd, int hw0, int nhw, int sd,
+                                            int dense, int ls, const float (&pre)[PF]) {
+    static_assert(SL <= SM && LSM % RL == 0, "window sums exceed the partial registers");
     float *red = reinterpret_cast<float *>(nb);
     const int lane = threadIdx.x % RING_GROUP;
     const int hb = wd * SL;
 #pragma unroll
-    for (int r = 0; r < LSM / 32; ++r) {
-        if (r * 32 >= ls) break;
+    for (int r = 0; r < LSM / RL; ++r) {
+        if (r * RL >= ls) break;
         lds_barrier();
-        if (w < wd) {
-            if (r == 0) {
+        if (w < wd && r == 0) {
 #pragma unroll
-                for (int i = 0; i < SL; ++i) red[(w * SL + i) * RING_GROUP + lane] = s[i];
-            }
-        } else {
-            const int k = w - wd;
+            for (int i = 0; i < SL; ++i) red[(w * SL + i) * RING_GROUP + lane] = s[i];
+        }
+        if (w >= hw0) {
+            const int k = w - hw0;
 #pragma unroll
-            for (int i = 0; i < 32; ++i)
-                if (r * 32 + i < ls) red[(hb + k * 32 + i) * RING_GROUP + lane] = h[r * 32 + i];
+            for (int i = 0; i < RL; ++i)
+                if (r * RL + i < ls) red[(hb + k * RL + i) * RING_GROUP + lane] = h[r * RL + i];
         }
         lds_barrier();
-        const int nd = r == 0 ? dense : 0, nh = min(32, ls - r * 32);
+        const int nd = r == 0 ? dense : 0, nh = min(RL, ls - r * RL);
         for (int it = w; it < nd + nh; it += LAG_WAVES) {
             float t = 0.f;
             int l;
@@ -451,8 +457,8 @@ __device__ __forceinline__ void hist_reduce(float2 *nb, float *accg, long long E
                 l = it;
             } else {
                 const int i = it - nd;
-                for (int k = 0; k < ws; ++k) t += red[(hb + k * 32 + i) * RING_GROUP + lane];
-                l = dense + r * 32 + i;
+                for (int k = 0; k < nhw; ++k) t += red[(hb + k * RL + i) * RING_GROUP + lane];
+                l = dense + r * RL + i;
             }
             add_acc(accg, E, l, (l - w) / LAG_WAVES, t, pre);
         }
@@ -468,7 +474,8 @@ __device__ __forceinline__ void hist_reduce(float2 *nb, float *accg, long long E
 //    lags; each chunk is cut into sd segments of new frames, one wave per
 //    (chunk, segment);
 //  * the other ls lags on the remaining ws waves, history-major (blocks of
-//    history frames from the plan's per-wave lists), or each lag cut into ss
+//    history frames from the plan's per-wave lists, which may give window
+//    waves blocks too, after their window), or each lag cut into ss
 //    segments ("units"; unit u goes to wave u % ws as its partial slot
 //    u / ws), or, with more lags than partial slots, each lag owned by one
 //    wave that adds its sum to the accumulator directly.
@@ -547,8 +554,7 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
             }
         }
     } else if (hist) {
-        const int vtau = lags[dense_lags + (lane < ls ? lane : 0)];
-        hist_sums<C::HIST_BLOCK>(g, nl, R, slot0, n, vtau, ls, pl, w - wd, ws, h);
+        // history-major work below
     } else if (ss) {
         const int k = w - wd, units = ls * ss, len = (n + ss - 1) / ss;
 #pragma unroll 1
@@ -566,7 +572,18 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
     }
     if constexpr (C::HIST_LAGS > 0) {
         if (hist) {
-            hist_reduce<C::HIST_LAGS, SLOTS, DENSE>(nb, accg, E, s, h, w, wd, ws, sd, dense_lags, ls, pre);
+            // history waves, and window waves once their window is done when
+            // the plan lists blocks for them too
+            if (w >= wd || pl.all_hist) {
+                const int vtau = lags[dense_lags + (lane < ls ? lane : 0)];
+                hist_sums<C::HIST_BLOCK>(g, nl, R, slot0, n, vtau, ls, pl, pl.all_hist ? w : w - wd, ws, h);
+            }
+            if (pl.all_hist)
+                hist_reduce<C::HIST_LAGS, SLOTS, DENSE, 16>(nb, accg, E, s, h, w, wd, 0, LAG_WAVES, sd, dense_lags,
+                                                            ls, pre);
+            else
+                hist_reduce<C::HIST_LAGS, SLOTS, DENSE, 32>(nb, accg, E, s, h, w, wd, wd, ws, sd, dense_lags, ls,
+                                                            pre);
             return;
         }
     }
@@ -607,10 +624,12 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
 //    pair cost; the other lags go history-major when there are at most
 //    HIST_LAGS of them, else each is cut into ss units (or owned by one wave
 //    when even one unit per lag exceeds the partial slots);
-//  * history-major blocks are handed out greedily, heaviest-loaded last
-//    block first, each to the wave with the least estimated work, so the
-//    slowest history wave (which ends the pair phase) carries little more
-//    than the mean.
+//  * history-major blocks are handed out greedily, last block first, each to
+//    the wave with the least estimated work, so the slowest wave (which ends
+//    the pair phase) carries little more than the mean; with window waves
+//    every wave takes blocks, a window wave starting from its window's
+//    estimated work, so it turns to history blocks when its window is done
+//    instead of idling while the history waves finish.
 template <int J>
 static void make_plan(const int *lags, int L, int n, int dmax, LagPlan &pl) {
     using C = LagCfg<J>;
@@ -656,7 +675,22 @@ static void make_plan(const int *lags, int L, int n, int dmax, LagPlan &pl) {
     pl.pmax = n - 1 - tmin;
     const int nb = pl.pmax >= pl.pmin ? (pl.pmax - pl.pmin) / B + 1 : 0;
     if (nb > PLAN_MAX_BLOCKS) return;  // round-robin blocks
-    std::vector<long long> load(ws, 0);
+    // every wave takes blocks when there are window waves: each window wave
+    // starts with its window's work, in history-pair units
+    const bool all = PLAN_ALL_WAVES && wd > 0;
+    const int nw = all ? LAG_WAVES : ws;
+    std::vector<long long> load(nw, 0);
+    if (all) {
+        const int len = (n + pl.sd - 1) / pl.sd;
+        for (int w = 0; w < wd; ++w) {
+            const int c = w / pl.sd, seg = w % pl.sd;
+            const int kc = std::min(DENSE, dense_lags - c * DENSE);
+            const int steps = std::max(0, std::min(len, n - seg * len));
+            load[w] = (long long)steps *
+                      (window_width<DENSE>(kc) * PLAN_COST_PAIR * COST_DENSE / COST_HIST + PLAN_COST_STEP) *
+                      PLAN_DENSE_PCT / 100;
+        }
+    }
     std::vector<int> owner(nb);
     for (int b = nb - 1; b >= 0; --b) {
         const int p0 = pl.pmin + b * B;
@@ -671,13 +705,14 @@ static void make_plan(const int *lags, int L, int n, int dmax, LagPlan &pl) {
         load[k] += cost;
     }
     int at = 0;
-    for (int k = 0; k < ws; ++k) {
+    for (int k = 0; k < nw; ++k) {
         pl.blk_start[k] = (unsigned char)at;
         for (int b = 0; b < nb; ++b)
             if (owner[b] == k) pl.blk_list[at++] = (unsigned char)b;
     }
-    for (int k = ws; k <= LAG_WAVES; ++k) pl.blk_start[k] = (unsigned char)at;
+    for (int k = nw; k <= LAG_WAVES; ++k) pl.blk_start[k] = (unsigned char)at;
     pl.listed = 1;
+    pl.all_hist = all;
 }
 
 template <int J>
