@@ -487,6 +487,20 @@ static int grow_stage(ddm_engine *e, int frames) {
     return DDM_OK;
 }
 
+extern "C" int ddm_reserve(ddm_engine *e, int max_frames) {
+    if (!e) return DDM_EINVAL;
+    if (max_frames < 0) return fail(e, DDM_EINVAL, "bad frame count");
+    const int chunk = e->p.chunk;
+    const int batch = std::min(e->fft_batch, std::max(max_frames, 1));
+    int rc = ensure_xt(e, batch);
+    if (rc) return rc;
+    if (max_frames > e->stage_frames && e->stage_frames < e->fft_batch) {
+        rc = grow_stage(e, std::min(e->fft_batch, (max_frames + chunk - 1) / chunk * chunk));
+        if (rc) return rc;
+    }
+    return DDM_OK;
+}
+
 extern "C" int ddm_submit_host(ddm_engine *e, const void *frames, int n_frames) {
     if (!e) return DDM_EINVAL;
     if (n_frames < 0 || (n_frames > 0 && !frames)) return fail(e, DDM_EINVAL, "bad frame buffer");

@@ -38,7 +38,7 @@ EXPORTED = [
     "ddm_synth_uniform", "ddm_synth_brownian", "ddm_write_iqt_file", "ddm_flush",
     "ddm_server_create", "ddm_server_destroy", "ddm_server_open", "ddm_server_submit", "ddm_server_results",
     "ddm_server_engine", "ddm_server_drain", "ddm_server_close", "ddm_server_latencies", "ddm_server_stats",
-    "ddm_server_last_error", "ddm_fit_results", "ddm_k_fit", "ddm_set_precision", "ddm_autotune",
+    "ddm_server_last_error", "ddm_fit_results", "ddm_k_fit", "ddm_set_precision", "ddm_autotune", "ddm_reserve",
 ]
 
 
@@ -113,6 +113,7 @@ def lib():
     L.ddm_fit_results.argtypes = [vp, ctypes.c_float, dp, dp]
     L.ddm_k_fit.argtypes = [vp, vp, ctypes.c_int, ll, ctypes.c_int, vp, vp]
     L.ddm_autotune.argtypes = [vp, ctypes.c_char_p, ip, ip]
+    L.ddm_reserve.argtypes = [vp, ctypes.c_int]
     L.ddm_reset.argtypes = [vp]
     L.ddm_get_info.argtypes = [vp, ctypes.POINTER(Info)]
     L.ddm_get_params.argtypes = [vp, ip, ip, fp]
@@ -291,6 +292,11 @@ class Engine:
         path = cache_path.encode() if cache_path else None
         self._check(lib().ddm_autotune(self._h, path, ctypes.byref(j), ctypes.byref(b)))
         return j.value, b.value
+
+    def reserve(self, max_frames: int):
+        """Allocates now the transient buffers a submission of up to
+        max_frames frames per call would allocate on first use."""
+        self._check(lib().ddm_reserve(self._h, int(max_frames)))
 
     def reset(self):
         self._check(lib().ddm_reset(self._h))

@@ -195,6 +195,15 @@ int ddm_set_precision(ddm_engine *eng, int mode);
  * on the choice.  No counterpart in the reference, whose launch shapes are
  * compile-time constants (src/constants.hpp:11-13). */
 int ddm_autotune(ddm_engine *eng, const char *cache_path, int *lag_block, int *fft_batch_frames);
+
+/* Allocates now the transient device buffers that a submission of up to
+ * max_frames frames per call would otherwise allocate on first use (the FFT
+ * pass buffer of 256 .. 1024 tiles for one FFT batch, and batch-sized upload
+ * staging when max_frames exceeds a chunk), so that such a submission starts
+ * its kernels at once.  Results do not change.  No counterpart in the
+ * reference, which allocates everything up front for its fixed chunk
+ * (src/DDM.cu:493-634). */
+int ddm_reserve(ddm_engine *eng, int max_frames);
 int ddm_get_stage_stats(ddm_engine *eng, ddm_stage_stats *stats);
 /* Device time of each submitted chunk (tiler+FFT+lag), in submission order;
  * returns the number written (<= max). */

@@ -104,6 +104,35 @@ def test_device_batches_match_chunked_host_submission(gpu, chunk, lb, lags):
     assert o.max_rel_err(b, ref) <= TOL
 
 
+def test_reserve_allocates_up_front_and_changes_nothing(gpu):
+    """ddm_reserve allocates the pass buffer (256 tiles) and batch-sized
+    staging before the first frame; the results of device and bulk host
+    submissions are bit-identical to an engine without it."""
+    import torch
+    frames = ddm.synth_uniform(200, 256, 256, 8, seed=5)
+    args = (256, 256, [256, 64], [1, 2, 3, 7, 40], [3.0, 9.0, 30.0], 1.2, 30)
+    d = torch.from_numpy(frames.reshape(-1).copy()).cuda()
+    torch.cuda.synchronize()
+    outs = []
+    for reserve in (False, True):
+        for src in ("device", "host"):
+            with ddm.Engine(*args) as e:
+                before = e.info().device_bytes
+                if reserve:
+                    e.reserve(len(frames))
+                    assert e.info().device_bytes > before
+                    before = e.info().device_bytes
+                if src == "device":
+                    e.submit_device(d.data_ptr(), len(frames))
+                else:
+                    e.submit_host(frames, len(frames))
+                if reserve:
+                    assert e.info().device_bytes == before  # nothing allocated on the way
+                outs.append(e.results())
+    for o_ in outs[1:]:
+        assert np.array_equal(outs[0], o_)
+
+
 @pytest.mark.parametrize("w,h,x_off,y_off,bpp,fmt", [
     (96, 64, 32, 0, 1, ddm.PIX_U8),      # ROI rows of consecutive frames evenly spaced: one 2-D copy
     (80, 90, 8, 20, 2, ddm.PIX_U16LE),   # general ROI: one 2-D copy per frame
