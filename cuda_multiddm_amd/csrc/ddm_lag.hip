@@ -87,7 +87,6 @@ constexpr int PLAN_COST_BLOCK = 200, PLAN_COST_VISIT = 10, PLAN_COST_PAIR = 4;
 // a window wave's own work in the same units: per step its window's pairs at
 // COST_DENSE / COST_HIST of a history pair's cost, plus the step's loads
 constexpr int PLAN_COST_STEP = 8;
-constexpr bool PLAN_ALL_WAVES = true;
 constexpr int PLAN_DENSE_PCT = 70;  // scale of a window wave's starting load (measured best)
 
 // The R ring slots of one ring group, read with buffer loads: the lane's
@@ -677,7 +676,7 @@ static void make_plan(const int *lags, int L, int n, int dmax, LagPlan &pl) {
     if (nb > PLAN_MAX_BLOCKS) return;  // round-robin blocks
     // every wave takes blocks when there are window waves: each window wave
     // starts with its window's work, in history-pair units
-    const bool all = PLAN_ALL_WAVES && wd > 0;
+    const bool all = wd > 0;
     const int nw = all ? LAG_WAVES : ws;
     std::vector<long long> load(nw, 0);
     if (all) {
