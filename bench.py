@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 MI355X_CUS, MI355X_CLOCK_HZ = 256, 2.4e9  # compute units, peak engine clock
+CHECK_TOL = 1e-4  # largest relative error against the FP64 oracle the metric allows
 
 
 def pmc_traffic(workload: str, kernel: str):
@@ -445,6 +446,11 @@ def main():
                       f"time scaled by the tile count of each scale "
                       f"({', '.join(f'{s}: {per[s]:.2f}s' for s in W['scales'])})",
         }
+    if not args.no_check and max(out["max_rel_err_vs_fp64"].values()) > CHECK_TOL:
+        # a number whose results miss the FP64 check is not a measurement
+        print(json.dumps(out), file=sys.stderr)
+        sys.exit(f"[bench] FP64 check failed: max relative error "
+                 f"{max(out['max_rel_err_vs_fp64'].values()):.3e} > {CHECK_TOL:g}")
     print(json.dumps(out))
 
 

@@ -447,17 +447,17 @@ __device__ __forceinline__ void hist_reduce(float2 *nb, float *accg, long long E
         }
         lds_barrier();
         const int nd = r == 0 ? dense : 0, nh = min(RL, ls - r * RL);
-        for (int it = w; it < nd + nh; it += LAG_WAVES) {
+        // wave w adds the round's lags l = w (mod 16): those are the lags
+        // whose accumulators it read early (w, w + 16, ...)
+        const int lo = r == 0 ? 0 : dense + r * RL, hi = dense + r * RL + nh;
+        for (int l = lo + (w - lo % LAG_WAVES + LAG_WAVES) % LAG_WAVES; l < hi; l += LAG_WAVES) {
             float t = 0.f;
-            int l;
-            if (it < nd) {
-                const int c = it / DN, i = it % DN;
+            if (l < nd) {
+                const int c = l / DN, i = l % DN;
                 for (int seg = 0; seg < sd; ++seg) t += red[((c * sd + seg) * SL + i) * RING_GROUP + lane];
-                l = it;
             } else {
-                const int i = it - nd;
+                const int i = l - dense - r * RL;
                 for (int k = 0; k < nhw; ++k) t += red[(hb + k * RL + i) * RING_GROUP + lane];
-                l = dense + r * RL + i;
             }
             add_acc(accg, E, l, (l - w) / LAG_WAVES, t, pre);
         }

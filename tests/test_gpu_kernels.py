@@ -207,6 +207,13 @@ DENSE_LAG_SETS = {
     "log64": list(range(1, 34)) + [37, 42, 46, 52, 58, 64, 72, 80, 90, 100, 112, 124, 139, 155, 173, 193, 215,
                                    240, 268, 299, 334, 372, 416, 464, 517, 577, 644, 719, 802, 895, 999],
     "sparse33": [3 * k + 1 for k in range(33)],  # no run, one lag too many for the history-major path
+    # more than 16 history-major lags beside a window run that is not a
+    # multiple of 16: the partial sums meet in two rounds and the second
+    # round's lags are not aligned with the wave index (the 512^2 workload's
+    # 32 log lags up to 512, and a 9-run with 23 lags behind it)
+    "log32_512": list(range(1, 15)) + [17, 20, 25, 31, 37, 46, 56, 68, 84, 102, 125, 153, 187, 229, 280, 342,
+                                       419, 512],
+    "run9_hist23": list(range(1, 10)) + [12 + 9 * k for k in range(23)],
 }
 
 
