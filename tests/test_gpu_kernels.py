@@ -245,6 +245,38 @@ def test_lag_engine_dense_runs_bit_exact(gpu, J, t0, n_new, name):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("t0", [0, 300])
+@pytest.mark.parametrize("nhist", [1, 5, 15, 16, 17, 18, 24, 31, 32, 33])
+@pytest.mark.parametrize("run", [0, 8, 9, 14, 16, 23, 32])
+def test_lag_engine_run_by_history_grid(gpu, run, nhist, t0):
+    """256-frame blocks, a window run of `run` consecutive lags followed by
+    `nhist` spread lags: the window / history split, the per-wave history
+    block lists and the rounds in which partial sums meet (16 or 32 lags per
+    round, lag counts on either side of each round) against exact integer
+    sums, with nonzero accumulators so that a lag given another lag's
+    accumulator shows."""
+    import torch
+    J = 256
+    lags = list(range(1, run + 1)) + [run + 3 + 7 * k for k in range(nhist)]
+    n_new = J
+    R = max(lags) + n_new + 5
+    E = 2 * G
+    rng = np.random.default_rng(1000 * run + 10 * nhist + t0)
+    re = rng.integers(-8, 9, (R, E))
+    im = rng.integers(-8, 9, (R, E))
+    ring_np = re + 1j * im
+    ring = torch.from_numpy(frames_to_ring(re, im)).to(gpu)
+    acc0 = rng.integers(0, 1 << 16, (len(lags), E))
+    acc = torch.from_numpy(acc0.astype(np.float32).reshape(-1)).to(gpu)
+    d_lags = torch.tensor(lags, dtype=torch.int32, device=gpu)
+    print(f"lag grid run={run} nhist={nhist} t0={t0} R={R}", flush=True)
+    ddm.k_lag_accumulate(ring, R, E, t0, n_new, d_lags, max(lags), acc, J)
+    torch.cuda.synchronize()
+    got = acc.cpu().numpy().reshape(len(lags), E).astype(np.int64)
+    want = acc0 + _lag_expected(ring_np, R, t0, n_new, lags)
+    assert np.array_equal(got, want)
+
+
 def test_stage_entry_points_reject_bad_arguments(gpu):
     import torch
     ring = torch.zeros(16, dtype=torch.float32, device=gpu)
