@@ -255,13 +255,27 @@ def test_lag_engine_run_by_history_grid(gpu, run, nhist, t0):
     round, lag counts on either side of each round) against exact integer
     sums, with nonzero accumulators so that a lag given another lag's
     accumulator shows."""
+    _check_run_history(gpu, run, nhist, 7, t0)
+
+
+@pytest.mark.parametrize("t0", [0, 4500])
+@pytest.mark.parametrize("run,nhist", [(0, 18), (14, 18), (9, 32), (16, 31)])
+def test_lag_engine_run_by_history_wide(gpu, run, nhist, t0):
+    """As above with history lags 130 frames apart (largest 2 213 … 4 042):
+    with 31 or 32 of them the history window holds more 16-frame blocks
+    than the plan lists (261, 269 > 252), so the history waves take blocks
+    round-robin; with 18 the plan lists ~155 blocks."""
+    _check_run_history(gpu, run, nhist, 130, t0)
+
+
+def _check_run_history(gpu, run, nhist, step, t0):
     import torch
     J = 256
-    lags = list(range(1, run + 1)) + [run + 3 + 7 * k for k in range(nhist)]
+    lags = list(range(1, run + 1)) + [run + 3 + step * k for k in range(nhist)]
     n_new = J
     R = max(lags) + n_new + 5
     E = 2 * G
-    rng = np.random.default_rng(1000 * run + 10 * nhist + t0)
+    rng = np.random.default_rng(1000 * run + 10 * nhist + t0 + step)
     re = rng.integers(-8, 9, (R, E))
     im = rng.integers(-8, 9, (R, E))
     ring_np = re + 1j * im
@@ -269,7 +283,7 @@ def test_lag_engine_run_by_history_grid(gpu, run, nhist, t0):
     acc0 = rng.integers(0, 1 << 16, (len(lags), E))
     acc = torch.from_numpy(acc0.astype(np.float32).reshape(-1)).to(gpu)
     d_lags = torch.tensor(lags, dtype=torch.int32, device=gpu)
-    print(f"lag grid run={run} nhist={nhist} t0={t0} R={R}", flush=True)
+    print(f"lag grid run={run} nhist={nhist} step={step} t0={t0} R={R}", flush=True)
     ddm.k_lag_accumulate(ring, R, E, t0, n_new, d_lags, max(lags), acc, J)
     torch.cuda.synchronize()
     got = acc.cpu().numpy().reshape(len(lags), E).astype(np.int64)
