@@ -586,6 +586,10 @@ struct BigCfg {
     static constexpr int P = E1 / L;               // length-L DFTs per lane in phase 2
     static constexpr int NTH = 256;                // 4 waves
     static constexpr int SEQ = NTH / L;            // sequences (rows pairs / columns) per workgroup
+    // column pass: 16 columns per workgroup, so that its xt reads and ring
+    // writes move 128-byte row segments (8 waves for S = 1024, 131 KB of LDS)
+    static constexpr int NTH_C = 16 * L;
+    static constexpr int SEQ_C = 16;
     static_assert(E1 == 16 || E1 == 32, "phase 1 is radix 16 or 32");
 };
 
@@ -679,12 +683,13 @@ __global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg(PixSrc ps, i
 // buffer's column block and written to the ring slot; the packed column is
 // separated into the DC (kx = 0) and Nyquist (kx = H) columns on the way out.
 template <int S>
-__global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_cols_reg(RingDst rd, int tiles,
+__global__ __launch_bounds__(BigCfg<S>::NTH_C) void tile_fft_cols_reg(RingDst rd, int tiles,
                                                                    const float2 *__restrict__ xt,
                                                                    const float2 *__restrict__ twg) {
     using C = BigCfg<S>;
-    constexpr int H = C::H, L = C::L, E1 = C::E1, SEQ = C::SEQ, NB = H / SEQ, NTH = C::NTH;
+    constexpr int H = C::H, L = C::L, E1 = C::E1, SEQ = C::SEQ_C, NB = H / SEQ, NTH = C::NTH_C;
     constexpr int RP = NTH / SEQ;  // rows per pass of the staging loops (= L)
+    static_assert(RP == L && RP * E1 == S && H % SEQ == 0, "staging covers the tile block");
     // tile block [y][c ^ (y % SEQ)], SEQ columns per row; later the columns'
     // spectra, S + 1 apart so that reading one row across them is conflict free
     __shared__ float2 lds[SEQ * (S + 1)];
@@ -751,8 +756,8 @@ static void launch_big(const PixSrc &ps, const RingDst &rd, int tps, int tiles, 
         break;
     default: hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_ANY>), g1, block, 0, st, ps, tps, tiles, xt, tw);
     }
-    const dim3 g2((unsigned)((C::H / C::SEQ) * tiles * n_frames));
-    hipLaunchKernelGGL(tile_fft_cols_reg<S>, g2, block, 0, st, rd, tiles, xt, tw);
+    const dim3 g2((unsigned)((C::H / C::SEQ_C) * tiles * n_frames)), block2(C::NTH_C);
+    hipLaunchKernelGGL(tile_fft_cols_reg<S>, g2, block2, 0, st, rd, tiles, xt, tw);
 }
 
 bool launch_tile_fft_reg(const PixSrc &ps, const RingDst &rd, int scale, int tps, int tiles, int n_frames,
