@@ -77,6 +77,8 @@ typedef struct {
     int nthreads;
     int only_scale;      /* >= 0: process only this index of the descending scale list */
     int max_tiles;       /* > 0: process only the first max_tiles tiles of each scale */
+    const int *tile_list;  /* tile_list_count > 0: process only these tile indices of each scale */
+    int tile_list_count;   /* (indices past a scale's tile count are ignored) */
 } ddmo_params;
 
 static int ilog2_exact(int n) {

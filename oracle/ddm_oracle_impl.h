@@ -100,8 +100,16 @@ static void SUF(rfft2)(const SUF(fftplan) *p, const REAL *in, SUF(cplx) *out,
     /* keep the half spectrum kx <= s/2 */
     for (int y = 0; y < s; ++y)
         for (int kx = 0; kx < h; ++kx) out[(size_t)y * h + kx] = work[(size_t)y * s + kx];
-    /* columns (along y) */
-    for (int kx = 0; kx < h; ++kx) SUF(fft1d)(p, out + kx, (size_t)h);
+    /* columns (along y), eight at a time through contiguous copies in `work`
+     * (the same transforms as in place with stride h; only faster) */
+    for (int k0 = 0; k0 < h; k0 += 8) {
+        const int nb = h - k0 < 8 ? h - k0 : 8;
+        for (int y = 0; y < s; ++y)
+            for (int b = 0; b < nb; ++b) work[(size_t)b * s + y] = out[(size_t)y * h + k0 + b];
+        for (int b = 0; b < nb; ++b) SUF(fft1d)(p, work + (size_t)b * s, 1);
+        for (int y = 0; y < s; ++y)
+            for (int b = 0; b < nb; ++b) out[(size_t)y * h + k0 + b] = work[(size_t)b * s + y];
+    }
 }
 
 /* ---------------------------------------------------------------------------
@@ -248,6 +256,11 @@ int SUF(ddmo_run)(const ddmo_params *prm, const uint8_t *frames, double *iqt,
             counts[q] = ddmo_mask(s, lambdas[q], prm->tolerance, mask + (size_t)q * E);
         const double n2 = 1.0 / ((double)s * s) / ((double)s * s);
         for (int tile = 0; tile < T_run; ++tile) {
+            if (prm->tile_list_count > 0) {
+                int listed = 0;
+                for (int k = 0; k < prm->tile_list_count; ++k) listed |= prm->tile_list[k] == tile;
+                if (!listed) continue;
+            }
             SUF(tile_ctx) c;
             c.prm = prm; c.frames = frames; c.M = M; c.s = s; c.tile = tile; c.E = E;
             c.spec = spec; c.plan = &plan; c.lags = lags; c.L = L; c.acc = acc;

@@ -39,6 +39,7 @@ class _Params(ctypes.Structure):
         ("nthreads", ctypes.c_int),
         ("only_scale", ctypes.c_int),
         ("max_tiles", ctypes.c_int),
+        ("tile_list", ctypes.POINTER(ctypes.c_int)), ("tile_list_count", ctypes.c_int),
     ]
 
 
@@ -106,15 +107,16 @@ class Config:
     def result_count(self):
         return sum(self.tiles()) * len(self.lambdas) * len(self.lags)
 
-    def _params(self, nthreads=1, only_scale=-1, max_tiles=0) -> _Params:
+    def _params(self, nthreads=1, only_scale=-1, max_tiles=0, tiles=None) -> _Params:
         sc = (ctypes.c_int * len(self.scales))(*self.scales)
         lg = (ctypes.c_int * len(self.lags))(*self.lags)
         lm = (ctypes.c_float * len(self.lambdas))(*[float(x) for x in self.lambdas])
-        self._keep = [sc, lg, lm]
+        tl = (ctypes.c_int * max(1, len(tiles or [])))(*(tiles or [0]))
+        self._keep = [sc, lg, lm, tl]
         return _Params(self.width, self.height, self.bpp, self.pixfmt, self.x_off, self.y_off,
                        self.n_frames, sc, len(self.scales), lg, len(self.lags), lm,
                        len(self.lambdas), ctypes.c_float(self.tolerance).value, self.norm_mode,
-                       nthreads, only_scale, max_tiles)
+                       nthreads, only_scale, max_tiles, tl, len(tiles or []))
 
 
 def split_results(cfg: Config, flat: np.ndarray):
@@ -128,9 +130,12 @@ def split_results(cfg: Config, flat: np.ndarray):
 
 
 def run(cfg: Config, frames: np.ndarray, precision: str = "f64", nthreads: int = 0,
-        acc_scale: int = -1, only_scale: int = -1, max_tiles: int = 0):
+        acc_scale: int = -1, only_scale: int = -1, max_tiles: int = 0, tiles=None):
     """Run the whole pipeline on the CPU.
 
+    ``only_scale`` (index into the descending scale list), ``max_tiles`` (the
+    first tiles of each scale) and ``tiles`` (a list of tile indices, applied
+    to every scale) restrict the work; results of skipped tiles stay 0.
     Returns the flat I(q, tau) array [scale][tile][q][lag] (float64) and, when
     ``acc_scale >= 0``, the normalised lag sums of that scale [lag][tile][ky][kx].
     """
@@ -146,7 +151,7 @@ def run(cfg: Config, frames: np.ndarray, precision: str = "f64", nthreads: int =
         s = cfg.sorted_scales()[acc_scale]
         T = cfg.tiles()[acc_scale]
         acc = np.zeros((len(cfg.lags), T, s, s // 2 + 1), dtype=np.float64)
-    p = cfg._params(nthreads, only_scale, max_tiles)
+    p = cfg._params(nthreads, only_scale, max_tiles, tiles)
     fn = lib().ddmo_run_f64 if precision == "f64" else lib().ddmo_run_f32
     rc = fn(ctypes.byref(p), frames.ctypes.data_as(ctypes.c_void_p), _dptr(iqt), acc_scale,
             _dptr(acc) if acc is not None else None)
