@@ -36,12 +36,7 @@ struct Mark {
     int kind;
 };
 
-// Frames of device-resident input transformed per batch by default: whole
-// chunks, up to about 128 frames.  A 30-frame chunk of one scale is too
-// little work to fill the device; ~128 frames is, while the 512-tile row
-// spectra of a batch (1 MB a frame) still fit in the last-level cache for
-// the column pass.  ddm_autotune may choose another multiple of the chunk.
-int default_fft_batch(int chunk) { return chunk * std::max(1, 128 / chunk); }
+using ddm::default_fft_batch;  // ddm_autotune may choose another multiple of the chunk
 constexpr int NSTAGE = ddm::UPLOAD_STAGES;
 
 }  // namespace
@@ -78,6 +73,7 @@ struct ddm_engine {
     long long frames = 0, frames_since_reset = 0;
     long long frames_lagged = 0;  // frames the lag engine has processed (<= frames)
     std::vector<long long> pairs;
+    std::vector<double> h_norm;  // per-lag normalisation, kept alive until its upload is done
     size_t dev_bytes = 0, pinned_bytes = 0;
     std::string err;
     bool timing = false;
@@ -597,7 +593,10 @@ static int compute_results(ddm_engine *e) {
     const size_t L = p.lags.size();
     int rc = flush_lags(e);
     if (rc) return rc;
-    std::vector<double> norm(L, 0.0);
+    std::vector<double> &norm = e->h_norm;
+    // the previous upload from this vector has finished: results are read
+    // after a stream synchronisation
+    norm.assign(L, 0.0);
     for (size_t l = 0; l < L; ++l) {
         if (p.norm_mode == DDM_NORM_PAIRS)
             norm[l] = e->pairs[l] > 0 ? 1.0 / (double)e->pairs[l] : 0.0;
@@ -1034,6 +1033,7 @@ extern "C" int ddm_k_lag_accumulate(const float *ring, int ring_slots, long long
     if (lag_block != 32 && lag_block != 64 && lag_block != 128 && lag_block != 256) return DDM_EINVAL;
     if (n_new <= 0 || n_new > lag_block) return DDM_EINVAL;
     if (max_lag < 0 || (long long)max_lag + n_new > ring_slots) return DDM_ERANGE;
+    if (ring_slots >= ddm::kMaxRingSlots) return DDM_ERANGE;  // 32-bit offsets inside a ring group
     // the launch plan is made on the host: bring the lag list over (this
     // waits for the stream)
     std::vector<int> host_lags(lag_count);

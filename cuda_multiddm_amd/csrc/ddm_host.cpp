@@ -99,7 +99,9 @@ int check_desc(const ddm_desc *d, Params *out, std::string *msg) {
     // frames before them (the reference keeps 3 chunks and so requires
     // max_lag <= chunk, src/DDM.cu:462).
     long long slots = (long long)p.chunk + p.lag_block + p.max_lag;
-    if (slots > (1LL << 30)) return fail(DDM_EINVAL, "chunk + largest tau is too large");
+    // the lag engine addresses one ring group's slots with 32-bit byte
+    // offsets (slots * 64 elements * 8 bytes < 2^31)
+    if (slots >= kMaxRingSlots) return fail(DDM_EINVAL, "chunk + lag block + largest tau is too large");
     p.ring_slots = (int)slots;
 
     long long off = 0, results = 0;
@@ -129,10 +131,13 @@ size_t device_footprint(const Params &p) {
     bytes += UPLOAD_STAGES * (size_t)p.chunk * p.M * p.M * p.bpp;      // upload staging
     bytes += (size_t)p.result_count * 8 + L * 12;                      // results, lags, norms
     for (const ScaleGeom &g : p.geom) bytes += Q * ((size_t)g.tile_elems * 4 + 8);  // ring tables (bound)
-    // between-pass buffer of 256 / 512 tile FFTs for one chunk (host input;
-    // device input batches several chunks and grows it)
+    // upload staging and the between-pass buffer of 256 .. 1024 tile FFTs
+    // grow to one FFT batch of frames when a call hands over more than a
+    // chunk: counted at that size, so an admitted stream never outgrows it
+    const int batch = default_fft_batch(p.chunk);
+    bytes += UPLOAD_STAGES * (size_t)(batch - p.chunk) * p.M * p.M * p.bpp;
     size_t xt = 0;
-    for (const ScaleGeom &g : p.geom) xt = std::max(xt, (size_t)big_fft_scratch_elems(p.M, g.s, p.chunk) * 8);
+    for (const ScaleGeom &g : p.geom) xt = std::max(xt, (size_t)big_fft_scratch_elems(p.M, g.s, batch) * 8);
     return bytes + xt;
 }
 

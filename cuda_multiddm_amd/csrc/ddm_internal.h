@@ -32,6 +32,17 @@ inline long long big_fft_scratch_elems(int M, int scale, int n) {
     return (scale == 256 || scale == 512 || scale == 1024) ? (long long)n * M * M / 2 : 0;
 }
 
+// Frames of device-resident input (or of one large host call) transformed
+// per FFT batch by default: whole chunks, up to about 128 frames.  A 30-frame
+// chunk of one scale is too little work to fill the device; ~128 frames is,
+// while the 512-tile row spectra of a batch still fit in the last-level
+// cache for the column pass.
+inline int default_fft_batch(int chunk) { return chunk * (128 / chunk > 1 ? 128 / chunk : 1); }
+
+// Ring slots are addressed per ring group with 32-bit byte offsets:
+// slots * RING_GROUP * 8 bytes must stay below 2^31.
+constexpr long long kMaxRingSlots = (1LL << 31) / (RING_GROUP * 8);
+
 // Frames per lag-engine pass when the descriptor leaves it to the engine.
 constexpr int kDefaultLagBlock = 256;
 // Lag values above this are rejected (ring memory would be absurd).

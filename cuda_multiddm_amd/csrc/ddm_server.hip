@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <deque>
 #include <map>
 #include <mutex>
@@ -42,6 +43,8 @@ struct Stream {
     std::deque<Pending> pending;
     std::vector<float> latency_ms;
     long long frames = 0;
+    int busy = 0;          // calls using the engine outside the lock
+    bool closing = false;  // ddm_server_close waits for busy == 0
 };
 
 }  // namespace
@@ -50,16 +53,40 @@ struct ddm_server {
     size_t budget = 0, used = 0;
     int max_streams = 0;
     std::mutex mu;
+    std::condition_variable idle;  // a stream's busy count dropped
     std::map<int, Stream> streams;
     int next_id = 0;
     std::vector<hipEvent_t> pool;
     std::thread poller;
     std::atomic<bool> stop{false};
     long long frames_total = 0, chunks_total = 0;
-    std::string err;
 };
 
+// Message of the last failing server call on this thread (the server may be
+// used from several threads, so it keeps no shared message).
 static thread_local std::string g_server_err;
+
+static int serr(int code, const std::string &msg) {
+    g_server_err = msg;
+    return code;
+}
+
+// Marks a stream in use so that ddm_server_close waits for the caller to
+// finish with its engine.  Caller holds mu.
+static int acquire(ddm_server *s, int stream_id, ddm_engine **eng) {
+    auto it = s->streams.find(stream_id);
+    if (it == s->streams.end() || it->second.closing) return serr(DDM_ERANGE, "unknown stream");
+    it->second.busy += 1;
+    *eng = it->second.eng;
+    return DDM_OK;
+}
+
+static void release_stream(ddm_server *s, int stream_id) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->streams.find(stream_id);
+    if (it != s->streams.end()) it->second.busy -= 1;
+    s->idle.notify_all();
+}
 
 // Moves finished chunks from pending to the latency record.  Caller holds mu.
 static void harvest(ddm_server *s, Stream &st) {
@@ -130,22 +157,15 @@ extern "C" int ddm_server_open(ddm_server *s, const ddm_desc *desc, int *stream_
     ddm::Params p;
     std::string msg;
     int rc = ddm::check_desc(desc, &p, &msg);
-    if (rc != DDM_OK) {
-        s->err = msg;
-        return rc;
-    }
+    if (rc != DDM_OK) return serr(rc, msg);
     const size_t need = ddm::device_footprint(p);
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        if ((int)s->streams.size() >= s->max_streams) {
-            s->err = "stream slots are all in use";
-            return DDM_EBUSY;
-        }
-        if (s->used + need > s->budget) {
-            s->err = "stream footprint (" + std::to_string(need) + " bytes) exceeds the device budget left (" +
-                     std::to_string(s->budget - s->used) + " bytes)";
-            return DDM_EBUSY;
-        }
+        if ((int)s->streams.size() >= s->max_streams) return serr(DDM_EBUSY, "stream slots are all in use");
+        if (s->used + need > s->budget)
+            return serr(DDM_EBUSY, "stream footprint (" + std::to_string(need) +
+                                       " bytes) exceeds the device budget left (" +
+                                       std::to_string(s->budget - s->used) + " bytes)");
         s->used += need;  // reserved while the engine is created
     }
     ddm_engine *e = nullptr;
@@ -162,14 +182,11 @@ extern "C" int ddm_server_open(ddm_server *s, const ddm_desc *desc, int *stream_
     std::lock_guard<std::mutex> lk(s->mu);
     if (rc != DDM_OK) {
         s->used -= need;
-        s->err = ddm_last_error(nullptr);
-        if (rc == DDM_ENOMEM) {
-            // a refusal like any other over-budget request: the caller may
-            // try again when streams have left
-            s->err = "device memory exhausted below the budget (" + s->err + ")";
-            return DDM_EBUSY;
-        }
-        return rc;
+        const std::string why = ddm_last_error(nullptr);
+        // a refusal like any other over-budget request: the caller may try
+        // again when streams have left
+        if (rc == DDM_ENOMEM) return serr(DDM_EBUSY, "device memory exhausted below the budget (" + why + ")");
+        return serr(rc, why);
     }
     const int id = s->next_id++;
     Stream &st = s->streams[id];
@@ -184,47 +201,39 @@ extern "C" int ddm_server_submit(ddm_server *s, int stream_id, const void *frame
     ddm_engine *e = nullptr;
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        auto it = s->streams.find(stream_id);
-        if (it == s->streams.end()) {
-            s->err = "unknown stream";
-            return DDM_ERANGE;
-        }
-        e = it->second.eng;
+        int rc = acquire(s, stream_id, &e);
+        if (rc) return rc;
     }
     const Clock::time_point t0 = Clock::now();
     int rc = ddm_submit_host(e, frames, n_frames);
     if (rc == DDM_OK) rc = ddm_flush(e);
-    if (rc != DDM_OK) {
-        s->err = ddm_last_error(e);
-        return rc;
-    }
     hipEvent_t ev = nullptr;
-    {
+    if (rc == DDM_OK) {
         std::lock_guard<std::mutex> lk(s->mu);
         if (!s->pool.empty()) {
             ev = s->pool.back();
             s->pool.pop_back();
         }
+    } else {
+        g_server_err = ddm_last_error(e);
     }
-    if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return DDM_EDEVICE;
-    if (hipEventRecord(ev, ddm::engine_stream(e)) != hipSuccess) return DDM_EDEVICE;
+    if (rc == DDM_OK && !ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
+        rc = serr(DDM_EDEVICE, "event creation failed");
+    if (rc == DDM_OK && hipEventRecord(ev, ddm::engine_stream(e)) != hipSuccess)
+        rc = serr(DDM_EDEVICE, "event record failed");
     std::lock_guard<std::mutex> lk(s->mu);
-    Stream &st = s->streams[stream_id];
-    st.pending.push_back(Pending{ev, t0});
-    st.frames += n_frames;
-    s->frames_total += n_frames;
-    s->chunks_total += 1;
-    return DDM_OK;
-}
-
-static int with_stream(ddm_server *s, int stream_id, Stream **out) {
-    auto it = s->streams.find(stream_id);
-    if (it == s->streams.end()) {
-        s->err = "unknown stream";
-        return DDM_ERANGE;
+    auto it = s->streams.find(stream_id);  // held by `busy`: close waits for it
+    if (rc == DDM_OK) {
+        it->second.pending.push_back(Pending{ev, t0});
+        it->second.frames += n_frames;
+        s->frames_total += n_frames;
+        s->chunks_total += 1;
+    } else if (ev) {
+        s->pool.push_back(ev);
     }
-    *out = &it->second;
-    return DDM_OK;
+    it->second.busy -= 1;
+    s->idle.notify_all();
+    return rc;
 }
 
 extern "C" int ddm_server_results(ddm_server *s, int stream_id, double *results) {
@@ -232,37 +241,44 @@ extern "C" int ddm_server_results(ddm_server *s, int stream_id, double *results)
     ddm_engine *e = nullptr;
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        Stream *st = nullptr;
-        int rc = with_stream(s, stream_id, &st);
+        int rc = acquire(s, stream_id, &e);
         if (rc) return rc;
-        e = st->eng;
     }
     int rc = ddm_results(e, results);
-    if (rc) s->err = ddm_last_error(e);
+    if (rc) g_server_err = ddm_last_error(e);
+    release_stream(s, stream_id);
     return rc;
 }
 
 extern "C" int ddm_server_engine(ddm_server *s, int stream_id, ddm_engine **eng) {
     if (!s || !eng) return DDM_EINVAL;
     std::lock_guard<std::mutex> lk(s->mu);
-    Stream *st = nullptr;
-    int rc = with_stream(s, stream_id, &st);
-    if (rc) return rc;
-    *eng = st->eng;
+    auto it = s->streams.find(stream_id);
+    if (it == s->streams.end() || it->second.closing) return serr(DDM_ERANGE, "unknown stream");
+    *eng = it->second.eng;
     return DDM_OK;
 }
 
 extern "C" int ddm_server_drain(ddm_server *s) {
     if (!s) return DDM_EINVAL;
-    std::vector<ddm_engine *> engines;
+    std::vector<std::pair<int, ddm_engine *>> engines;
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        for (auto &kv : s->streams) engines.push_back(kv.second.eng);
+        for (auto &kv : s->streams) {
+            if (kv.second.closing) continue;
+            kv.second.busy += 1;
+            engines.emplace_back(kv.first, kv.second.eng);
+        }
     }
-    for (ddm_engine *e : engines) {
-        int rc = ddm_sync(e);
-        if (rc) return rc;
+    int rc = DDM_OK;
+    for (auto &pe : engines) {
+        if (rc == DDM_OK) {
+            rc = ddm_sync(pe.second);
+            if (rc) g_server_err = ddm_last_error(pe.second);
+        }
+        release_stream(s, pe.first);
     }
+    if (rc) return rc;
     std::lock_guard<std::mutex> lk(s->mu);
     for (auto &kv : s->streams) harvest(s, kv.second);
     return DDM_OK;
@@ -272,12 +288,11 @@ extern "C" int ddm_server_close(ddm_server *s, int stream_id) {
     if (!s) return DDM_EINVAL;
     Stream st;
     {
-        std::lock_guard<std::mutex> lk(s->mu);
+        std::unique_lock<std::mutex> lk(s->mu);
         auto it = s->streams.find(stream_id);
-        if (it == s->streams.end()) {
-            s->err = "unknown stream";
-            return DDM_ERANGE;
-        }
+        if (it == s->streams.end() || it->second.closing) return serr(DDM_ERANGE, "unknown stream");
+        it->second.closing = true;  // no new user from here on
+        s->idle.wait(lk, [&] { return it->second.busy == 0; });
         st = std::move(it->second);
         s->streams.erase(it);
     }
@@ -319,4 +334,4 @@ extern "C" int ddm_server_stats(ddm_server *s, ddm_server_info *info) {
     return DDM_OK;
 }
 
-extern "C" const char *ddm_server_last_error(const ddm_server *s) { return s ? s->err.c_str() : g_server_err.c_str(); }
+extern "C" const char *ddm_server_last_error(const ddm_server *) { return g_server_err.c_str(); }

@@ -223,7 +223,9 @@ const char *ddm_status_string(int code);
  * flushed through the lag engine and its chunk-to-result latency (host
  * submission -> chunk accumulated on the device) is recorded.  Results of a
  * stream depend only on its own frames.  The server functions may be called
- * from one host thread at a time.
+ * from several host threads at once (chunks of one stream keep their order
+ * when that stream is fed from one thread); ddm_server_close waits for calls
+ * still using the stream, and ddm_server_destroy must not overlap any call.
  * ------------------------------------------------------------------------- */
 typedef struct ddm_server ddm_server;
 
@@ -250,7 +252,8 @@ int ddm_server_close(ddm_server *srv, int stream_id);
  * all streams with stream_id < 0; returns the number written. */
 int ddm_server_latencies(ddm_server *srv, int stream_id, float *ms, int max);
 int ddm_server_stats(ddm_server *srv, ddm_server_info *info);
-/* With srv == NULL: the last ddm_server_create failure on this thread. */
+/* Message of the last failing server call (ddm_server_create included) on
+ * the calling thread; srv is not used. */
 const char *ddm_server_last_error(const ddm_server *srv);
 
 /* -------------------------------------------------------------------------
