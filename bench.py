@@ -1,16 +1,20 @@
 """Benchmark of the DDM analysis path on one MI355X.
 
-Workload (one step = one whole analysis of the video, inputs resident in
-HBM when the timer starts): 1024 frames of 512x512 uint8 synthetic
-Brownian video (2000 Gaussian spots, sigma 2 px, amplitude 80, D = 0.5
-px^2/frame, background 100, noise sigma 5), scales 512/256/128/64, 32
-log-spaced lags 1..512, 30 log-spaced lambdas 4..256 px, mask tolerance
-20 %, chunk 30 frames, plain FP32 path.
+Default workload (the largest single-GPU configuration): 1000 frames of
+2048x2048 uint16 (12-bit) synthetic Brownian video, ROI 1024^2, scales
+1024/512/256/128/64/32, 64 log-spaced lags 1..999, 40 log-spaced lambdas
+2..512 px, mask tolerance 20 %, chunk 30, plain FP32 path.  One step = one
+whole analysis of the video with the frames resident in HBM when the timer
+starts (engine creation outside the timed region).
 
-Prints one JSON line: analysed frames/s, p50/p99 chunk latency, the lag
-engine's roofline (algorithmic bytes / measured kernel time against 8 TB/s
-HBM), the max relative I(q, tau) error of checked tiles against the FP64
-oracle, and a multithreaded FP32 CPU baseline timed on this host.
+Prints one JSON line: analysed frames/s; the dominant kernel's roofline
+(algorithmic bytes / launch time from HIP events on the engine's stream,
+against 8 TB/s); chunk-to-result latency of the same video streamed chunk by
+chunk from pinned host memory; the PCIe-inclusive rate; the max / mean
+relative I(q, tau) error of tile 0 of every scale against the FP64 oracle;
+and a multithreaded FP32 CPU baseline timed on this host.
+
+`--serving` runs the many-stream serving workload instead.
 """
 from __future__ import annotations
 
@@ -18,6 +22,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -28,6 +33,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 MI355X_CUS, MI355X_CLOCK_HZ = 256, 2.4e9  # compute units, peak engine clock
 CHECK_TOL = 1e-4  # largest relative error against the FP64 oracle the metric allows
+METRIC = "analysed frames/s (all scales x lags) + p99 chunk latency, at <=1e-4 err vs FP64"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def pmc_traffic(workload: str, kernel: str):
@@ -35,34 +45,14 @@ def pmc_traffic(workload: str, kernel: str):
     PMC summary profiles/pmc_traffic_<workload>.md (FETCH_SIZE and WRITE_SIZE
     passes, scaled by a calibration copy): its line
     `traffic_bytes_per_launch <kernel> <bytes>`."""
-    here = os.path.dirname(os.path.abspath(__file__))
-    path = os.path.join(here, "profiles", f"pmc_traffic_{workload}.md")
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.md")
     if os.path.exists(path):
         with open(path) as f:
             for line in f:
                 parts = line.split()
                 if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
-                    return float(parts[2]), os.path.relpath(path, here)
+                    return float(parts[2]), os.path.relpath(path, ROOT)
     return None, None
-
-WORKLOADS = {
-    # the headline workload: 1024 frames of 512^2 uint8 Brownian video
-    "brownian512": dict(
-        frames=1024, size=512, bits=8, scales=[512, 256, 128, 64], lag_max=512, lag_count=32,
-        lam_lo=4.0, lam_hi=256.0, lam_count=30, tol=1.2, chunk=30, cpu_frames=0,
-        spots=2000, sigma=2.0, amplitude=80.0, diffusion=0.5, background=100.0, noise=5.0, seed=2026,
-        text="single-stream Brownian 1024x512^2 u8, scales 512/256/128/64, 32 log lags 1-512, "
-             "30 log lambdas 4-256 px, tol 20%, chunk 30, plain FP32"),
-    # 1000 frames of 2048^2 uint16 with 12-bit values; the analysed ROI is the
-    # largest scale (1024^2 at the origin)
-    "brownian2048_u16": dict(
-        frames=1000, size=2048, bits=12, scales=[1024, 512, 256, 128, 64, 32], lag_max=999, lag_count=64,
-        lam_lo=2.0, lam_hi=512.0, lam_count=40, tol=1.2, chunk=30, cpu_frames=60,
-        spots=8000, sigma=2.0, amplitude=80.0, diffusion=1.0, background=1000.0, noise=30.0, seed=2027,
-        text="single-stream Brownian 1000x2048^2 u16 (12-bit), ROI 1024^2, scales 1024..32, 64 log lags "
-             "1-999, 40 log lambdas 2-512 px, tol 20%, chunk 30, plain FP32"),
-}
-WORKLOAD = WORKLOADS["brownian512"]
 
 
 def distinct_frames(t0, n, lags):
@@ -91,34 +81,49 @@ def percentile(a, p):
     return float(np.percentile(np.asarray(a, dtype=np.float64), p)) if len(a) else None
 
 
-def serving(args):
-    """Serving workload: 64 streams of 256^2 / 512^2 / 1024^2 uint8 frames
-    (size drawn uniformly), scales from the frame size down to 32, 24 log
-    lags up to 300, 30 log lambdas 4..256, chunks of 30 frames arriving as a
-    Poisson process at --rate chunks/s in aggregate for --seconds.  Streams
-    are admitted by device footprint; a stream leaves after --chunks-per-stream
-    chunks and a waiting one takes its place.  Reports frames/s and
-    chunk-to-result latency percentiles."""
-    import cuda_multiddm_amd as ddm
+def cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"model": model, "hardware_concurrency": os.cpu_count(), "affinity": affinity}
 
-    rng = np.random.default_rng(7)
-    sizes = [int(s) for s in rng.choice([256, 512, 1024], size=args.streams)]
-    lags = ddm.log_lags(300, 24)
-    lams = ddm.log_lambdas(4, 256, 30)
+
+def serving(args):
+    """Serving workload: `--streams` streams of the serving mix (frame size
+    uniform over 256^2 / 512^2 / 1024^2 uint8, D uniform in 0.2..2.0, scales
+    from the frame size down to 32, 24 log lags up to 300, 30 log lambdas
+    4..256), chunks of 30 frames arriving as a Poisson process at --rate
+    chunks/s in aggregate for --seconds.  Streams are admitted by device
+    footprint; with --chunks-per-stream 0 every admitted stream stays
+    resident, else a stream leaves after that many chunks and a waiting one
+    takes its place.  Reports frames/s and chunk-to-result latency
+    percentiles."""
     import torch
+
+    import cuda_multiddm_amd as ddm
+    from cuda_multiddm_amd.workloads import serving_streams
+
+    streams = serving_streams(args.streams)
+    rng = np.random.default_rng(11)
     pool = {}
-    for s in (256, 512, 1024):
-        f = ddm.synth_brownian(30, s, s, 8, max(1, 2000 * s * s // (512 * 512)), seed=s,
-                               nthreads=min(os.cpu_count() or 1, 16))
+    for size in sorted({s.size for s in streams}):
+        proto = next(s for s in streams if s.size == size)
+        f = proto.make_frames(30)
         pinned = torch.empty(f.size, dtype=torch.uint8, pin_memory=True).numpy()  # uploads skip the staging copy
         pinned[:] = f.reshape(-1)
-        pool[s] = pinned
+        pool[size] = pinned
 
-    def desc(s):
-        sc = [1 << k for k in range(int(np.log2(s)), 4, -1)]
-        return dict(width=s, height=s, scales=sc, lags=lags, lambdas=lams, chunk=30, lag_block=args.serving_lag_block)
-
-    srv = ddm.Server(max_streams=args.streams)
+    srv = ddm.Server(max_streams=args.streams, **({"batched": True} if args.batched else {}))
     waiting = list(range(args.streams))
     active, sent = {}, {}
     admitted, refused = 0, 0
@@ -136,7 +141,7 @@ def serving(args):
         while waiting:
             k = waiting[0]
             try:
-                sid = timed("open", lambda: srv.open(**desc(sizes[k])))
+                sid = timed("open", lambda: srv.open(**streams[k].desc(args.serving_lag_block)))
             except ddm.DDMError as e:
                 if e.code != ddm.DDM_EBUSY:
                     raise
@@ -148,6 +153,7 @@ def serving(args):
             admitted += 1
 
     admit()
+    log(f"serving: {len(active)} streams resident, offered {args.rate} chunks/s")
     t_start = time.perf_counter()
     t_next = t_start
     frames = 0
@@ -161,11 +167,12 @@ def serving(args):
         if not active:
             admit()
             continue
-        k = list(active)[rng.integers(len(active))]
-        timed("submit", srv.submit, active[k], pool[sizes[k]])
+        keys = list(active)
+        k = keys[int(rng.integers(len(keys)))]
+        timed("submit", srv.submit, active[k], pool[streams[k].size])
         frames += 30
         sent[k] += 1
-        if sent[k] >= args.chunks_per_stream:
+        if args.chunks_per_stream and sent[k] >= args.chunks_per_stream:
             timed("results", srv.results, active[k])
             timed("close", srv.close_stream, active.pop(k))
             completed += 1
@@ -175,6 +182,7 @@ def serving(args):
     elapsed = time.perf_counter() - t_start
     lat = srv.latencies()
     st = srv.stats()
+    sizes = [streams[k].size for k in active]
     out = {
         "metric": "serving: analysed frames/s + chunk-to-result latency",
         "value": round(frames / elapsed, 1),
@@ -189,15 +197,121 @@ def serving(args):
         "streams_completed": completed,
         "admission_refusals": refused,
         "active_streams_at_end": st.streams,
+        "active_sizes": {str(s): sizes.count(s) for s in (256, 512, 1024)},
         "device_budget_GB": round(st.budget_bytes / 1e9, 1),
         "device_used_GB_at_end": round(st.used_bytes / 1e9, 1),
         "host_seconds": {k: round(v, 2) for k, v in host_s.items()},
         "config": {"workload": f"{args.streams} streams, frame sizes {{256,512,1024}}^2 u8, scales to 32, "
-                               f"24 log lags <= 300, 30 log lambdas 4-256, chunk 30, lag block {args.serving_lag_block}, "
-                               f"Poisson arrivals {args.rate} chunks/s"},
+                               f"24 log lags <= 300, 30 log lambdas 4-256, chunk 30, lag block "
+                               f"{args.serving_lag_block}, Poisson arrivals {args.rate} chunks/s, "
+                               + ("streams resident" if not args.chunks_per_stream else
+                                  f"streams replaced after {args.chunks_per_stream} chunks")
+                               + (", batched launches" if args.batched else ", per-stream launches")},
     }
     srv.close()
     print(json.dumps(out))
+
+
+def fp64_check(W, raw, res, out):
+    """Tile 0 of every scale over all frames against the FP64 oracle."""
+    from cuda_multiddm_amd.workloads import host_threads
+    from oracle import oracle as o
+    cfg = o.Config(W.size, W.size, W.frames, W.scales, W.lags, W.lambdas, np.float32(W.tol).item(), W.bpp,
+                   W.pixfmt)
+    per = o.split_results(cfg, res)
+    errs, mean_errs = {}, {}
+    for si, s in enumerate(W.scales):
+        ref = o.split_results(cfg, o.run(cfg, raw, "f64", nthreads=host_threads(), only_scale=si, tiles=[0]))[s][0]
+        g = per[s][0]
+        errs[s] = o.max_rel_err(g, ref)
+        nz = ref != 0
+        mean_errs[s] = float(np.mean(np.abs(g[nz] - ref[nz]) / np.abs(ref[nz]))) if nz.any() else 0.0
+        log(f"FP64 check scale {s} tile 0: max {errs[s]:.3e} mean {mean_errs[s]:.3e}")
+    out["max_rel_err_vs_fp64"] = {f"scale{s}_tile0": float(f"{v:.3e}") for s, v in errs.items()}
+    out["mean_rel_err_vs_fp64"] = {f"scale{s}_tile0": float(f"{v:.3e}") for s, v in mean_errs.items()}
+
+
+def cpu_baseline(W, raw, nf, out):
+    """The FP32 multithreaded C restatement, built for this host, on tile 0 of
+    every scale over nf frames; time scaled by each scale's tile count (the
+    work of a scale is exactly proportional to its tiles)."""
+    from cuda_multiddm_amd.workloads import host_threads
+    from oracle import oracle as o
+    kind = "-O3 -march=native"
+    try:
+        o.use_library(o.build_native(tempfile.mkdtemp(prefix="ddm_cpu_")))
+    except Exception as ex:  # no compiler: the portable build shipped with the tree
+        kind = f"-O3 portable (native build failed: {type(ex).__name__})"
+        o.use_library(os.path.join(ROOT, "oracle", "_build", "libddm_oracle.so"))
+    threads = host_threads()
+    lags = [l for l in W.lags]
+    cfg = o.Config(W.size, W.size, nf, W.scales, lags, W.lambdas, np.float32(W.tol).item(), W.bpp, W.pixfmt)
+    total, per = 0.0, {}
+    for si, s in enumerate(W.scales):
+        t0 = time.perf_counter()
+        o.run(cfg, raw[:nf * W.size * W.size * W.bpp], "f32", nthreads=threads, only_scale=si, tiles=[0])
+        dt = time.perf_counter() - t0
+        log(f"CPU baseline scale {s}: {dt:.2f} s")
+        per[s] = dt
+        total += dt * (W.scales[0] // s) ** 2
+    info = cpu_info()
+    out["cpu_baseline"] = {
+        "value": round(nf / total, 3),
+        "unit": "frames/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"FP32 multithreaded C restatement of the pipeline ({kind}, {threads} threads), tile 0 of "
+                  f"every scale over {nf} frames, time scaled by each scale's tile count "
+                  f"({', '.join(f'{s}: {per[s]:.2f}s' for s in W.scales)})",
+        "cpu_model": info["model"],
+        "hardware_concurrency": info["hardware_concurrency"],
+        "cpus_available": info["affinity"],
+    }
+
+
+def streaming_latency(W, raw, out):
+    """Chunk-to-result latency: the same video streamed chunk by chunk from
+    pinned host memory through a one-stream server (every chunk uploaded,
+    transformed and accumulated before it counts as done)."""
+    import torch
+
+    import cuda_multiddm_amd as ddm
+    h = torch.empty(raw.size, dtype=torch.uint8, pin_memory=True).numpy()
+    h[:] = raw
+    fb = W.size * W.size * W.bpp
+    with ddm.Server(max_streams=1) as srv:
+        sid = srv.open(width=W.size, height=W.size, scales=W.scales, lags=W.lags, lambdas=W.lambdas,
+                       tolerance=W.tol, chunk=W.chunk, bpp=W.bpp, pixfmt=W.pixfmt, lag_block=32)
+        t0 = time.perf_counter()
+        for i in range(0, W.frames, W.chunk):
+            n = min(W.chunk, W.frames - i)
+            srv.submit(sid, h[i * fb:(i + n) * fb].reshape(n, -1))
+        srv.drain()
+        dt = time.perf_counter() - t0
+        lat = srv.latencies(sid)
+    out["p50_chunk_ms"] = percentile(lat, 50)
+    out["p99_chunk_ms"] = percentile(lat, 99)
+    out["latency_run"] = {"frames_per_s": round(W.frames / dt, 1), "chunks": len(lat),
+                          "input": f"pinned host memory, {W.chunk}-frame chunks submitted back to back, "
+                                   f"lag block 32 (every chunk accumulated before it counts as done)"}
+    del h
+
+
+def pcie_rate(W, raw, make, out):
+    """One whole analysis with the frames in pinned host memory (ROI rows
+    cross PCIe inside the timed region)."""
+    import torch
+    h = torch.empty(raw.size, dtype=torch.uint8, pin_memory=True).numpy()
+    h[:] = raw
+    e = make()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.submit_host(h, W.frames)
+    e.results()
+    dt = time.perf_counter() - t0
+    e.close()
+    out["pcie_inclusive_frames_per_s"] = round(W.frames / dt, 1)
+    del h
 
 
 def main():
@@ -205,11 +319,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="brownian512", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="brownian2048_u16",
+                    choices=["brownian2048_u16", "brownian512", "uniform128"])
     ap.add_argument("--frames", type=int, default=0, help="frames to analyse (0 = the workload's count)")
     ap.add_argument("--lag-block", type=int, default=0, help="frames per lag-engine pass (0 = engine default)")
     ap.add_argument("--host-input", action="store_true",
-                    help="frames start in pinned host memory (PCIe-inclusive rate; not the headline value)")
+                    help="time the steps with frames in pinned host memory (PCIe-inclusive; not the headline)")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x2", "fp16x2"],
                     help="64/128 tile transforms: FP32 FFT or split-operand DFT on the matrix cores")
     ap.add_argument("--autotune", action="store_true",
@@ -217,43 +332,44 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=0, help="frames for the CPU baseline sample (0 = all)")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 tile checks")
+    ap.add_argument("--no-extras", action="store_true", help="skip the latency and PCIe-inclusive runs")
     ap.add_argument("--profile-steps", action="store_true",
                     help="only run warmup + timed steps (for rocprofv3 runs)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="with --profile-steps: finish with a device copy of this many MiB")
     ap.add_argument("--serving", action="store_true", help="run the multi-stream serving workload instead")
+    ap.add_argument("--batched", action="store_true", help="serving: batched launches across streams")
     ap.add_argument("--streams", type=int, default=64)
     ap.add_argument("--rate", type=float, default=200.0, help="serving: offered chunks per second")
     ap.add_argument("--seconds", type=float, default=20.0, help="serving: arrival window")
-    ap.add_argument("--chunks-per-stream", type=int, default=12)
+    ap.add_argument("--chunks-per-stream", type=int, default=0,
+                    help="serving: chunks after which a stream is replaced (0 = streams stay resident)")
     ap.add_argument("--serving-lag-block", type=int, default=32, choices=[32, 64, 128, 256],
                     help="serving: each stream's lag block (every chunk is flushed through the lag engine)")
     args = ap.parse_args()
-    if args.serving and args.gpus == 1:
-        serving(args)
-        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.serving:
+        serving(args)
+        return
 
     import torch
 
     import cuda_multiddm_amd as ddm
+    from cuda_multiddm_amd.workloads import WORKLOADS, host_threads
 
-    W = dict(WORKLOADS[args.workload])
+    W = WORKLOADS[args.workload]
     if args.frames:
-        W["frames"] = args.frames
-    N, S = W["frames"], W["size"]
-    M = W["scales"][0]  # analysed ROI: M x M at the frame origin
-    bpp, pixfmt = (1, ddm.PIX_U8) if W["bits"] <= 8 else (2, ddm.PIX_U16LE)
-    lags = ddm.log_lags(W["lag_max"], W["lag_count"])
-    lams = ddm.log_lambdas(W["lam_lo"], W["lam_hi"], W["lam_count"])
-    print(f"[bench] generating {N} frames {S}x{S}", file=sys.stderr, flush=True)
-    frames = ddm.synth_brownian(N, S, S, W["bits"], W["spots"], W["sigma"], W["amplitude"], W["diffusion"],
-                                W["background"], W["noise"], W["seed"], min(os.cpu_count() or 1, 16))
+        from dataclasses import replace
+        W = replace(W, frames=args.frames)
+    N, M = W.frames, W.scales[0]
+    lags, lams = W.lags, W.lambdas
+    log(f"generating {N} frames {W.size}x{W.size} ({W.bits}-bit)")
+    frames = W.make_frames(N, host_threads())
     raw = frames.reshape(-1).view(np.uint8)
-    print(f"[bench] frames ready ({raw.nbytes / 1e9:.2f} GB)", file=sys.stderr, flush=True)
+    log(f"frames ready ({raw.nbytes / 1e9:.2f} GB)")
     torch.cuda.set_device(0)
     if args.host_input:
         h_frames = torch.empty(raw.size, dtype=torch.uint8, pin_memory=True).numpy()
@@ -263,16 +379,11 @@ def main():
     torch.cuda.synchronize()
 
     prec = {"fp32": ddm.PREC_FP32, "bf16x2": ddm.PREC_BF16X2, "fp16x2": ddm.PREC_FP16X2}[args.precision]
-
-    tune_cache = None
-    if args.autotune:
-        import tempfile
-        tune_cache = os.path.join(tempfile.mkdtemp(prefix="ddm_tune_"), "schedule.txt")
+    tune_cache = os.path.join(tempfile.mkdtemp(prefix="ddm_tune_"), "schedule.txt") if args.autotune else None
     schedule = {}
 
     def make():
-        e = ddm.Engine(S, S, W["scales"], lags, lams, W["tol"], W["chunk"], bpp=bpp, pixfmt=pixfmt,
-                       lag_block=args.lag_block)
+        e = ddm.Engine(*W.engine_args(), lag_block=args.lag_block)
         e.set_precision(prec)
         if args.autotune:
             # timed on the first engine, read back from the cache by the others
@@ -296,7 +407,7 @@ def main():
     torch.cuda.synchronize()
     elapsed = 0.0
     res = None
-    print("[bench] warmup done", file=sys.stderr, flush=True)
+    log("warmup done")
     for _ in range(args.steps):
         e = make()
         torch.cuda.synchronize()
@@ -318,13 +429,14 @@ def main():
             torch.cuda.synchronize()
         print(json.dumps({"frames_per_s": fps, "ms_per_step": ms_per_step}))
         return
+    log(f"timed steps: {ms_per_step:.2f} ms per step, {fps:.0f} frames/s")
 
     # stage timing with HIP events on the engine's stream (separate run)
     with make() as e:
         e.set_timing(True)
         step(e)
         st = e.stage_stats()
-        chunk_ms = e.chunk_times()
+        batch_ms = e.chunk_times()
         info = e.info()
         lag_block = info.lag_block
     E = info.elems_per_frame
@@ -334,7 +446,7 @@ def main():
     achieved = lag_bytes / (st.lag_ms * 1e-3) / 1e9
     # whole-pipeline algorithmic bytes per frame (ROI read, spectra write,
     # lag engine reads + accumulators), for context
-    pipe_bytes = (M * M * bpp + 8 * E) * N + lag_bytes
+    pipe_bytes = (M * M * W.bpp + 8 * E) * N + lag_bytes
     pipe_gbs = pipe_bytes / (elapsed / args.steps) / 1e9
 
     traffic, traffic_src = pmc_traffic(args.workload, f"lag_kernel<{lag_block}>")
@@ -345,7 +457,7 @@ def main():
     pairs = sum(max(0, N - tau) for tau in lags) * E
     valu_floor_ms = pairs * 4 / 64 / lag_launches / (MI355X_CUS * MI355X_CLOCK_HZ) * 1e3
     out = {
-        "metric": "analysed frames/s (all scales x lags) + p99 chunk latency, at <=1e-4 err vs FP64",
+        "metric": METRIC,
         "value": round(fps, 2),
         "unit": "frames/s",
         "n_gpus": 1,
@@ -358,22 +470,21 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": W["text"].replace(f"{WORKLOADS[args.workload]['frames']}x", f"{N}x"),
+            "workload": W.text.replace(f"{WORKLOADS[args.workload].frames}x", f"{N}x"),
             "name": args.workload,
             "input": "pinned host memory (PCIe-inclusive)" if args.host_input else "device memory",
-            "frames": N, "frame_size": S, "scales": W["scales"], "lags": len(lags), "lambdas": len(lams),
-            "chunk": W["chunk"], "elems_per_frame": E, "ring_slots": info.ring_slots,
+            "frames": N, "frame_size": W.size, "scales": W.scales, "lags": len(lags), "lambdas": len(lams),
+            "chunk": W.chunk, "elems_per_frame": E, "ring_slots": info.ring_slots,
             # device-resident frames are transformed whole chunks at a time, up to
-            # ~128 frames per batch (the engine's FFT batch); host input goes chunk
-            # by chunk.  Chunk latencies below are per batch.
-            "fft_batch_frames": W["chunk"] if args.host_input else
-            schedule.get("fft_batch_frames", W["chunk"] * max(1, 128 // W["chunk"])),
+            # ~128 frames per batch (the engine's FFT batch)
+            "fft_batch_frames": schedule.get("fft_batch_frames", W.chunk * max(1, 128 // W.chunk)),
             "lag_block": lag_block, "schedule": "autotuned" if args.autotune else "default",
+            "precision": args.precision,
         },
-        "p50_chunk_ms": percentile(chunk_ms, 50),
-        "p99_chunk_ms": percentile(chunk_ms, 99),
         "stage_ms_per_step": {"fft": round(st.fft_ms, 3), "lag": round(st.lag_ms, 3),
                               "azimuthal": round(st.azimuthal_ms, 3)},
+        "batch_device_ms": {"p50": percentile(batch_ms, 50), "p99": percentile(batch_ms, 99),
+                            "what": "device time of each FFT batch (tile + FFT + the lag launches it completes)"},
         "roofline": {
             "bound": "hbm",
             "kernel": f"lag_kernel<{lag_block}>",
@@ -392,60 +503,22 @@ def main():
                                       "instruction per CU per clock, 256 CUs at 2.4 GHz"},
         },
         "pipeline_algorithmic_GBps": round(pipe_gbs, 1),
-        "precision": args.precision,
     }
     if prec != ddm.PREC_FP32:
         # matrix-core work of the split-operand DFT: 3 products in the row stage
         # and 6 in the column stage, each S^3 multiply-adds per tile
-        mf = sum(18 * s ** 3 * (M // s) ** 2 for s in W["scales"] if s in (64, 128)) * N
-        out["mfma_flops_per_step"] = mf
-
+        out["mfma_flops_per_step"] = sum(18 * s ** 3 * (M // s) ** 2 for s in W.scales if s in (64, 128)) * N
+    if not args.no_extras and not args.host_input:
+        if not args.frames or args.frames == WORKLOADS[args.workload].frames:
+            del d_frames
+        streaming_latency(W, raw, out)
+        log(f"chunk-to-result latency p50 {out['p50_chunk_ms']:.2f} ms p99 {out['p99_chunk_ms']:.2f} ms")
+        pcie_rate(W, raw, make, out)
+        log(f"PCIe-inclusive {out['pcie_inclusive_frames_per_s']:.0f} frames/s")
     if not args.no_check:
-        from oracle import oracle as o
-        cfg = o.Config(S, S, N, W["scales"], lags, lams, W["tol"], bpp, pixfmt)
-        flat = res
-        offs, off = {}, 0
-        for s in W["scales"]:
-            offs[s] = off
-            off += (W["scales"][0] // s) ** 2 * len(lams) * len(lags)
-        errs, mean_errs = {}, {}
-        for si, s in enumerate(W["scales"]):
-            if s > 256:
-                continue
-            ref = o.run(cfg, raw, "f64", nthreads=min(os.cpu_count() or 1, 16), only_scale=si, max_tiles=1)
-            n1 = len(lams) * len(lags)
-            g, r = flat[offs[s]:offs[s] + n1], ref[offs[s]:offs[s] + n1]
-            errs[s] = o.max_rel_err(g, r)
-            nz = r != 0
-            mean_errs[s] = float(np.mean(np.abs(g[nz] - r[nz]) / np.abs(r[nz]))) if nz.any() else 0.0
-            print(f"[bench] FP64 check scale {s}: {errs[s]:.3e}", file=sys.stderr, flush=True)
-        out["max_rel_err_vs_fp64"] = {f"scale{s}_tile0": float(f"{v:.3e}") for s, v in errs.items()}
-        out["mean_rel_err_vs_fp64"] = {f"scale{s}_tile0": float(f"{v:.3e}") for s, v in mean_errs.items()}
-
+        fp64_check(W, raw, res, out)
     if not args.no_cpu_baseline:
-        from oracle import oracle as o
-        threads = min(os.cpu_count() or 1, 16)
-        nf = min(args.cpu_frames or W["cpu_frames"] or N, N)
-        cfg = o.Config(S, S, nf, W["scales"], lags, lams, W["tol"], bpp, pixfmt)
-        total = 0.0
-        per = {}
-        for si, s in enumerate(W["scales"]):
-            t0 = time.perf_counter()
-            o.run(cfg, raw[:nf * S * S * bpp], "f32", nthreads=threads, only_scale=si, max_tiles=1)
-            dt = time.perf_counter() - t0
-            print(f"[bench] CPU baseline scale {s}: {dt:.2f} s", file=sys.stderr, flush=True)
-            tiles = (W["scales"][0] // s) ** 2
-            per[s] = dt
-            total += dt * tiles
-        out["cpu_baseline"] = {
-            "value": round(nf / total, 3),
-            "unit": "frames/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"FP32 multithreaded C restatement, tile 0 of each scale over {nf} frames, "
-                      f"time scaled by the tile count of each scale "
-                      f"({', '.join(f'{s}: {per[s]:.2f}s' for s in W['scales'])})",
-        }
+        cpu_baseline(W, raw, min(args.cpu_frames or N, N), out)
     worst = max(out.get("max_rel_err_vs_fp64", {}).values(), default=0.0)
     if worst > CHECK_TOL:
         # a number whose results miss the FP64 check is not a measurement

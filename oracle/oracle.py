@@ -50,6 +50,22 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
+def build_native(out_dir: str) -> str:
+    """Compile the oracle for the host it runs on (-O3 -march=native) into
+    out_dir and return the library path (the CPU baseline's build)."""
+    path = os.path.join(os.path.abspath(out_dir), "libddm_oracle_native.so")
+    subprocess.run(["make", "-s", "-B", f"OUT={path}", "EXTRA=-march=native"], cwd=_HERE, check=True,
+                   capture_output=True)
+    return path
+
+
+def use_library(path: str) -> None:
+    """Load the oracle from `path` from now on (e.g. a build_native library)."""
+    global _lib, _LIB_PATH
+    _lib = None
+    _LIB_PATH = path
+
+
 def lib():
     global _lib
     if _lib is None:

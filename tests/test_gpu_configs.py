@@ -1,0 +1,200 @@
+"""The benchmarked workloads themselves, at full size, through the HIP path
+(C ABI) against the FP64 oracle on the same seeded frames.
+
+* 1024 x 512^2 uint8 Brownian video, scales 512..64, 32 log lags to 512:
+  every tile of every scale against the oracle (<= 1e-5), device and host
+  submission bit-identical; the same workload with the 64 / 128 tile
+  transforms on the matrix cores (split BF16 and FP16 operands): per-scale
+  max / mean error against FP64 (<= 1e-4), the other scales unchanged.
+* 1000 x 2048^2 uint16 (12-bit) video, ROI 1024^2, scales 1024..32, 64 log
+  lags to 999: tile 0 and the last tile of every scale over all 1000 frames
+  against the oracle; at full size, host = device submission bit for bit,
+  and the spectra of every tile of the last frame against numpy's rfft2.
+* the serving mix (256^2 / 512^2 / 1024^2 streams, scales down to 32, 24 log
+  lags to 300): a stream of each size twice, chunks interleaved through one
+  server; each result bit-identical to the stream alone and within 1e-5 of
+  the oracle on the first and last tile of every scale.
+
+The reference publishes no outputs for these workloads (it has no tests), so
+the oracle is the check: entries that are exactly 0 in FP64 must be 0.
+"""
+import numpy as np
+import pytest
+
+import cuda_multiddm_amd as ddm
+from cuda_multiddm_amd.workloads import WORKLOADS, host_threads, serving_streams
+from oracle import oracle as o
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
+
+TOL = 1e-5        # plain FP32 path
+TOL_MIXED = 1e-4  # matrix-core tile transforms
+
+
+def _per_scale(cfg, got, ref, tiles=None):
+    """{scale: (max rel err, mean rel err)} over the tiles checked."""
+    g, r = o.split_results(cfg, got), o.split_results(cfg, ref)
+    out = {}
+    for s in cfg.sorted_scales():
+        T = g[s].shape[0]
+        sel = sorted({t % T for t in tiles}) if tiles is not None else list(range(T))
+        a, b = g[s][sel].reshape(-1), r[s][sel].reshape(-1)
+        nz = b != 0
+        mean = float(np.mean(np.abs(a[nz] - b[nz]) / np.abs(b[nz]))) if nz.any() else 0.0
+        out[s] = (o.max_rel_err(a, b), mean)
+    return out
+
+
+def _run(W, frames, device_input=True, precision=ddm.PREC_FP32):
+    import torch
+    raw = np.ascontiguousarray(frames).reshape(-1).view(np.uint8)
+    with ddm.Engine(*W.engine_args()) as e:
+        e.set_precision(precision)
+        print(f"[{W.name}] {len(frames)} frames, device_input={device_input}, precision={precision}", flush=True)
+        if device_input:
+            d = torch.from_numpy(raw).cuda()
+            torch.cuda.synchronize()  # the engine's stream does not wait for torch's
+            e.submit_device(d.data_ptr(), len(frames))
+            e.sync()
+            res = e.results()
+            del d
+        else:
+            e.submit_host(raw, len(frames))
+            res = e.results()
+    return res
+
+
+def _oracle_cfg(W, n):
+    return o.Config(W.size, W.size, n, W.scales, W.lags, W.lambdas, np.float32(W.tol).item(), W.bpp, W.pixfmt)
+
+
+# ---------------------------------------------------------------------------
+# 1024 x 512^2 uint8
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def b512():
+    W = WORKLOADS["brownian512"]
+    frames = W.make_frames()
+    cfg = _oracle_cfg(W, W.frames)
+    print(f"[{W.name}] FP64 oracle, every tile of every scale", flush=True)
+    ref = o.run(cfg, frames.reshape(-1).view(np.uint8), "f64", nthreads=host_threads())
+    return W, frames, cfg, ref
+
+
+def test_brownian512_every_tile(gpu, b512):
+    W, frames, cfg, ref = b512
+    dev = _run(W, frames, device_input=True)
+    errs = _per_scale(cfg, dev, ref)
+    for s, (mx, mean) in errs.items():
+        print(f"[{W.name}] scale {s}: {(W.scales[0] // s) ** 2} tiles, max rel err {mx:.3e}, mean {mean:.3e}")
+    assert all(mx <= TOL for mx, _ in errs.values()), errs
+    host = _run(W, frames, device_input=False)
+    assert np.array_equal(dev, host)
+
+
+@pytest.mark.parametrize("prec,name", [(ddm.PREC_BF16X2, "bf16x2"), (ddm.PREC_FP16X2, "fp16x2")])
+def test_brownian512_matrix_core_transforms(gpu, b512, prec, name):
+    W, frames, cfg, ref = b512
+    got = _run(W, frames, device_input=True, precision=prec)
+    errs = _per_scale(cfg, got, ref)
+    for s, (mx, mean) in errs.items():
+        print(f"[{W.name} {name}] scale {s}: max rel err {mx:.3e}, mean {mean:.3e}")
+    assert all(mx <= TOL_MIXED for mx, _ in errs.values()), errs
+    # only the 64 and 128 tiles change arithmetic
+    fp32 = _run(W, frames, device_input=True)
+    g, f = o.split_results(cfg, got), o.split_results(cfg, fp32)
+    for s in (512, 256):
+        assert np.array_equal(g[s], f[s]), s
+
+
+# ---------------------------------------------------------------------------
+# 1000 x 2048^2 uint16, ROI 1024^2
+# ---------------------------------------------------------------------------
+def test_brownian2048_u16_full_size(gpu):
+    W = WORKLOADS["brownian2048_u16"]
+    frames = W.make_frames()
+    dev = _run(W, frames, device_input=True)
+    host = _run(W, frames, device_input=False)
+    assert np.array_equal(dev, host)
+
+    # spectra of every tile of the last frame (still in the ring)
+    last = W.frames - 1
+    img = frames[last].astype(np.float64)
+    with ddm.Engine(*W.engine_args()) as e:
+        e.submit_host(np.ascontiguousarray(frames[W.frames - 40:]).reshape(-1).view(np.uint8), 40)
+        for si, s in enumerate(W.scales):
+            spec = e.spectra(39, si)
+            tps = W.scales[0] // s
+            worst = 0.0
+            for tile in range(tps * tps):
+                ty, tx = divmod(tile, tps)
+                ref = np.fft.rfft2(img[ty * s:(ty + 1) * s, tx * s:(tx + 1) * s])
+                worst = max(worst, float(np.max(np.abs(spec[tile] - ref)) / np.max(np.abs(ref))))
+            print(f"[{W.name}] spectra scale {s}, {tps * tps} tiles: max err / max bin {worst:.2e}", flush=True)
+            assert worst < 4e-6, (s, worst)
+
+    cfg = _oracle_cfg(W, W.frames)
+    print(f"[{W.name}] FP64 oracle, first and last tile of every scale", flush=True)
+    tiles = [0, (W.scales[0] // W.scales[-1]) ** 2 - 1]
+    ref = np.zeros_like(dev)
+    for si, s in enumerate(W.scales):
+        T = (W.scales[0] // s) ** 2
+        part = o.run(cfg, frames.reshape(-1).view(np.uint8), "f64", nthreads=host_threads(), only_scale=si,
+                     tiles=[0, T - 1])
+        ref += part
+    errs = _per_scale(cfg, dev, ref, tiles=tiles)
+    for s, (mx, mean) in errs.items():
+        print(f"[{W.name}] scale {s}: max rel err {mx:.3e}, mean {mean:.3e}")
+    assert all(mx <= TOL for mx, _ in errs.values()), errs
+
+
+# ---------------------------------------------------------------------------
+# serving mix
+# ---------------------------------------------------------------------------
+def _mix_streams():
+    pool = serving_streams(64)
+    picked = []
+    for size in (256, 512, 1024):
+        picked += [s for s in pool if s.size == size][:2]
+    return picked
+
+
+SERVING_FRAMES = 330  # 11 chunks: every lag up to 300 has pairs
+
+
+def test_serving_mix_alone_equals_mix_and_oracle(gpu):
+    streams = _mix_streams()
+    frames = [s.make_frames(SERVING_FRAMES) for s in streams]
+    chunks = [[f[i:i + 30] for i in range(0, SERVING_FRAMES, 30)] for f in frames]
+    alone = []
+    for k, s in enumerate(streams):
+        with ddm.Server() as srv:
+            sid = srv.open(**s.desc())
+            for ch in chunks[k]:
+                srv.submit(sid, ch)
+            alone.append(srv.results(sid))
+            assert len(srv.latencies(sid)) == len(chunks[k])
+    rng = np.random.default_rng(5)
+    with ddm.Server() as srv:
+        ids = [srv.open(**s.desc()) for s in streams]
+        print(f"serving mix: {len(streams)} streams resident, sizes {[s.size for s in streams]}", flush=True)
+        left = [list(c) for c in chunks]
+        while any(left):
+            k = int(rng.choice([i for i in range(len(left)) if left[i]]))
+            srv.submit(ids[k], left[k].pop(0))
+        mixed = [srv.results(i) for i in ids]
+        lat = srv.latencies()
+        assert len(lat) == sum(len(c) for c in chunks)
+    for k, s in enumerate(streams):
+        assert np.array_equal(alone[k], mixed[k]), k
+        d = s.desc()
+        cfg = o.Config(s.size, s.size, SERVING_FRAMES, d["scales"], d["lags"], d["lambdas"], np.float32(1.2).item())
+        T_last = (s.size // 32) ** 2 - 1
+        ref = np.zeros_like(mixed[k])
+        for si, sc in enumerate(d["scales"]):
+            T = (s.size // sc) ** 2
+            ref += o.run(cfg, frames[k].reshape(-1), "f64", nthreads=host_threads(), only_scale=si, tiles=[0, T - 1])
+        errs = _per_scale(cfg, mixed[k], ref, tiles=[0, T_last])
+        print(f"serving stream {k} ({s.size}^2, D {s.diffusion:.2f}): "
+              + ", ".join(f"{sc}: {mx:.2e}" for sc, (mx, _) in errs.items()))
+        assert all(mx <= TOL for mx, _ in errs.values()), (k, errs)
