@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -983,10 +984,12 @@ extern "C" const char *ddm_last_error(const ddm_engine *e) { return e ? e->err.c
 // ---------------------------------------------------------------------------
 static bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
-extern "C" int ddm_k_tile_fft(const void *frames, int n_frames, long long frame_bytes, int width, int height,
-                              int bytes_per_pixel, int pixel_format, int x_off, int y_off, int largest_scale,
-                              int scale, float *ring, int ring_slots, long long elems_per_frame,
-                              long long elem_off, long long first_frame, void *stream) {
+// Validates the arguments of the tile FFT entry points and fills `a`
+// (without a pass buffer).
+static int tile_fft_args(const void *frames, int n_frames, long long frame_bytes, int width, int height,
+                         int bytes_per_pixel, int pixel_format, int x_off, int y_off, int largest_scale, int scale,
+                         float *ring, int ring_slots, long long elems_per_frame, long long elem_off,
+                         long long first_frame, ddm::TileFftArgs *a) {
     if (!frames || !ring || n_frames <= 0 || ring_slots <= 0 || n_frames > ring_slots || first_frame < 0)
         return DDM_EINVAL;
     if (!pow2(scale) || !pow2(largest_scale) || scale > largest_scale || scale > ddm::kMaxScale) return DDM_ESCALE;
@@ -998,30 +1001,105 @@ extern "C" int ddm_k_tile_fft(const void *frames, int n_frames, long long frame_
     if (elems_per_frame % ddm::RING_GROUP != 0) return DDM_EINVAL;
     const long long tiles = (long long)(largest_scale / scale) * (largest_scale / scale);
     if (elem_off < 0 || elem_off + tiles * scale * (scale / 2 + 1) > elems_per_frame) return DDM_ERANGE;
+    a->frames = frames;
+    a->n_frames = n_frames;
+    a->frame_bytes = frame_bytes;
+    a->width = width;
+    a->bpp = bytes_per_pixel;
+    a->fmt = pixel_format;
+    a->x_off = x_off;
+    a->y_off = y_off;
+    a->largest_scale = largest_scale;
+    a->scale = scale;
+    a->ring = ring;
+    a->ring_slots = ring_slots;
+    a->first_slot = (int)(first_frame % ring_slots);
+    a->elems_per_frame = elems_per_frame;
+    a->elem_off = elem_off;
+    a->xt = nullptr;
+    return DDM_OK;
+}
+
+extern "C" int ddm_k_tile_fft(const void *frames, int n_frames, long long frame_bytes, int width, int height,
+                              int bytes_per_pixel, int pixel_format, int x_off, int y_off, int largest_scale,
+                              int scale, float *ring, int ring_slots, long long elems_per_frame,
+                              long long elem_off, long long first_frame, void *stream) {
     ddm::TileFftArgs a;
-    a.frames = frames;
-    a.n_frames = n_frames;
-    a.frame_bytes = frame_bytes;
-    a.width = width;
-    a.bpp = bytes_per_pixel;
-    a.fmt = pixel_format;
-    a.x_off = x_off;
-    a.y_off = y_off;
-    a.largest_scale = largest_scale;
-    a.scale = scale;
-    a.ring = ring;
-    a.ring_slots = ring_slots;
-    a.first_slot = (int)(first_frame % ring_slots);
-    a.elems_per_frame = elems_per_frame;
-    a.elem_off = elem_off;
-    // the between-pass buffer of 256 / 512 tiles lives for this call only
+    int rc = tile_fft_args(frames, n_frames, frame_bytes, width, height, bytes_per_pixel, pixel_format, x_off, y_off,
+                           largest_scale, scale, ring, ring_slots, elems_per_frame, elem_off, first_frame, &a);
+    if (rc) return rc;
+    // the between-pass buffer of 256 .. 1024 tiles lives for this call only
     // (stream-ordered allocation and release)
     const hipStream_t st = (hipStream_t)stream;
     const long long xt = ddm::big_fft_scratch_elems(largest_scale, scale, n_frames);
     if (xt && hipMallocAsync((void **)&a.xt, (size_t)xt * 2 * sizeof(float), st) != hipSuccess) return DDM_ENOMEM;
-    const hipError_t rc = ddm::launch_tile_fft(a, st);
+    const hipError_t err = ddm::launch_tile_fft(a, st);
     if (a.xt) (void)hipFreeAsync(a.xt, st);
-    return rc == hipSuccess ? DDM_OK : DDM_EDEVICE;
+    return err == hipSuccess ? DDM_OK : DDM_EDEVICE;
+}
+
+// ---------------------------------------------------------------------------
+// ddm_k_tile_fft without device allocation: the pass buffer of the 256 ..
+// 1024 tile transforms is this file-scope array, one copy per device, which
+// every call on that device shares (calls that overlap on one device race on
+// it).  Its device address is looked up once per device; in host code the
+// array's name is only a stand-in and is never passed to a kernel.
+// ---------------------------------------------------------------------------
+constexpr long long kScratchElems = 1LL << 23;  // complex values: 64 MiB
+static __device__ __align__(256) float2 scratch[kScratchElems];
+static std::mutex g_scratch_mu;
+static float2 *g_scratch_ptr[64] = {nullptr};
+
+static hipError_t scratch_ptr(float2 **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        float2 *p = nullptr;
+        e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+extern "C" int ddm_k_tile_fft_noalloc(const void *frames, int n_frames, long long frame_bytes, int width,
+                                      int height, int bytes_per_pixel, int pixel_format, int x_off, int y_off,
+                                      int largest_scale, int scale, float *ring, int ring_slots,
+                                      long long elems_per_frame, long long elem_off, long long first_frame,
+                                      void *stream) {
+    ddm::TileFftArgs a;
+    int rc = tile_fft_args(frames, n_frames, frame_bytes, width, height, bytes_per_pixel, pixel_format, x_off, y_off,
+                           largest_scale, scale, ring, ring_slots, elems_per_frame, elem_off, first_frame, &a);
+    if (rc) return rc;
+    const long long per_frame = ddm::big_fft_scratch_elems(largest_scale, scale, 1);
+    if (per_frame > kScratchElems) {
+        g_create_err = "ddm_k_tile_fft_noalloc: the pass buffer of one frame (" + std::to_string(per_frame * 8) +
+                       " bytes) exceeds the " + std::to_string(kScratchElems * 8) + "-byte scratch array";
+        return DDM_ERANGE;
+    }
+    const hipStream_t st = (hipStream_t)stream;
+    // the twiddle table lives in static device memory as well
+    const float2 *tw = nullptr;
+    if (ddm::twiddle_table(&tw) != hipSuccess) return DDM_EDEVICE;
+    float2 *xt = nullptr;
+    if (per_frame && scratch_ptr(&xt) != hipSuccess) return DDM_EDEVICE;
+    // frames in batches whose pass buffer fits the array; the batches run in
+    // stream order, so each reuses it after the previous one is done
+    const int batch = per_frame ? (int)std::min<long long>(n_frames, kScratchElems / per_frame) : n_frames;
+    const uint8_t *src = (const uint8_t *)frames;
+    for (int i = 0; i < n_frames; i += batch) {
+        ddm::TileFftArgs b = a;
+        b.frames = src + (long long)i * frame_bytes;
+        b.n_frames = std::min(batch, n_frames - i);
+        b.first_slot = (int)((first_frame + i) % ring_slots);
+        b.xt = (float *)xt;
+        if (ddm::launch_tile_fft(b, st) != hipSuccess) return DDM_EDEVICE;
+    }
+    return DDM_OK;
 }
 
 extern "C" int ddm_k_lag_accumulate(const float *ring, int ring_slots, long long elems_per_frame,

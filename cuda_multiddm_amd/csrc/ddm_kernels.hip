@@ -465,6 +465,9 @@ __global__ __launch_bounds__(AZ_WAVES * 64) void azimuthal_narrow_kernel(
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
+// Twiddle table (fill_twiddles) in static device memory, filled once per
+// device on first use: no launch path allocates device memory for it.
+static __device__ __align__(256) float2 g_twiddles[2 * kMaxScale];
 static std::mutex g_tw_mu;
 static float2 *g_tw[64] = {nullptr};
 
@@ -478,13 +481,12 @@ hipError_t twiddle_table(const float2 **out) {
         std::vector<float> h;
         fill_twiddles(&h);
         float2 *d = nullptr;
-        e = hipMalloc(&d, h.size() * sizeof(float));
+        // the array's device address on this device (its name in host code
+        // is only a stand-in)
+        e = hipGetSymbolAddress((void **)&d, HIP_SYMBOL(g_twiddles));
         if (e != hipSuccess) return e;
         e = hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
-            (void)hipFree(d);
-            return e;
-        }
+        if (e != hipSuccess) return e;
         g_tw[dev] = d;
     }
     *out = g_tw[dev];

@@ -34,7 +34,8 @@ EXPORTED = [
     "ddm_create", "ddm_validate", "ddm_destroy", "ddm_submit_host", "ddm_submit_device", "ddm_sync",
     "ddm_results", "ddm_reset", "ddm_get_info", "ddm_get_params", "ddm_get_accumulators",
     "ddm_get_spectra", "ddm_set_timing", "ddm_get_stage_stats", "ddm_get_chunk_times",
-    "ddm_last_error", "ddm_status_string", "ddm_k_tile_fft", "ddm_k_lag_accumulate", "ddm_ring_mask",
+    "ddm_last_error", "ddm_status_string", "ddm_k_tile_fft", "ddm_k_tile_fft_noalloc", "ddm_k_lag_accumulate",
+    "ddm_ring_mask",
     "ddm_synth_uniform", "ddm_synth_brownian", "ddm_write_iqt_file", "ddm_flush",
     "ddm_server_create", "ddm_server_destroy", "ddm_server_open", "ddm_server_submit", "ddm_server_results",
     "ddm_server_engine", "ddm_server_drain", "ddm_server_close", "ddm_server_latencies", "ddm_server_stats",
@@ -130,6 +131,7 @@ def lib():
     L.ddm_k_tile_fft.argtypes = [vp, ctypes.c_int, ll, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ll,
                                  ll, ll, vp]
+    L.ddm_k_tile_fft_noalloc.argtypes = L.ddm_k_tile_fft.argtypes
     L.ddm_k_lag_accumulate.argtypes = [vp, ctypes.c_int, ll, ll, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int,
                                        vp, ctypes.c_int, vp]
     L.ddm_ring_mask.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, vp]
@@ -447,6 +449,17 @@ def k_tile_fft(frames, n_frames, frame_bytes, width, height, bpp, pixfmt, x_off,
                               ctypes.c_void_p(stream) if stream else None)
     if rc != DDM_OK:
         raise DDMError(rc, "ddm_k_tile_fft")
+
+
+def k_tile_fft_noalloc(frames, n_frames, frame_bytes, width, height, bpp, pixfmt, x_off, y_off, largest_scale,
+                       scale, ring, ring_slots, elems_per_frame, elem_off, first_frame, stream=None):
+    """k_tile_fft without device allocation (static pass buffer, shared per GPU)."""
+    rc = lib().ddm_k_tile_fft_noalloc(ctypes.c_void_p(frames.data_ptr()), n_frames, frame_bytes, width, height, bpp,
+                                      pixfmt, x_off, y_off, largest_scale, scale, ctypes.c_void_p(ring.data_ptr()),
+                                      ring_slots, elems_per_frame, elem_off, first_frame,
+                                      ctypes.c_void_p(stream) if stream else None)
+    if rc != DDM_OK:
+        raise DDMError(rc, lib().ddm_last_error(None).decode())
 
 
 def k_lag_accumulate(ring, ring_slots, elems_per_frame, first_frame, n_new, lags, max_lag, acc, lag_block=32,

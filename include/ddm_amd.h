@@ -210,7 +210,7 @@ int ddm_get_stage_stats(ddm_engine *eng, ddm_stage_stats *stats);
 int ddm_get_chunk_times(ddm_engine *eng, float *ms, int max);
 
 /* Message of the last failure of eng; with eng == NULL, of the last
- * ddm_create on the calling thread. */
+ * ddm_create (or refused ddm_k_tile_fft_noalloc) on the calling thread. */
 const char *ddm_last_error(const ddm_engine *eng);
 const char *ddm_status_string(int code);
 
@@ -276,6 +276,22 @@ int ddm_k_tile_fft(const void *frames, int n_frames, long long frame_bytes,
                    int x_off, int y_off, int largest_scale, int scale,
                    float *ring, int ring_slots, long long elems_per_frame,
                    long long elem_off, long long first_frame, void *stream);
+
+/* ddm_k_tile_fft without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops or harnesses that forbid hipMalloc).  The
+ * pass buffer of 256 .. 1024 tiles is a static 64 MiB device array per GPU
+ * (held until the process ends) shared by every call on that GPU, so calls
+ * that overlap on one GPU race on it; frames whose pass buffers exceed it are
+ * transformed in several batches in stream order, and a call whose single
+ * frame needs more (largest_scale^2 / 2 complex values > 2^23) is refused
+ * with DDM_ERANGE (message in ddm_last_error(NULL)).  Arguments and results
+ * as ddm_k_tile_fft. */
+int ddm_k_tile_fft_noalloc(const void *frames, int n_frames, long long frame_bytes,
+                           int width, int height, int bytes_per_pixel, int pixel_format,
+                           int x_off, int y_off, int largest_scale, int scale,
+                           float *ring, int ring_slots, long long elems_per_frame,
+                           long long elem_off, long long first_frame, void *stream);
 
 /* Lag engine (processFFT / analyseChunk, src/DDM_kernel.cuh:85-101,
  * src/DDM.cu:222-262): for the n_new frames first_frame .. first_frame+n_new-1
