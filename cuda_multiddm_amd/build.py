@@ -29,7 +29,7 @@ CXX = os.environ.get("CXX_HOST", "g++")
 
 HOST_SRCS = ["ddm_host.cpp"]
 HIP_SRCS = ["ddm_kernels.hip", "ddm_lag.hip", "ddm_engine.hip", "ddm_server.hip", "ddm_fit.hip", "ddm_fft.hip", "ddm_dft_mfma.hip"]
-CLI_SRCS = ["ddm_cli.cpp", "movie_reader.cpp"]
+CLI_SRCS = ["ddm_cli.cpp", "ddm_run.cpp", "movie_reader.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

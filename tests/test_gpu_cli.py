@@ -101,3 +101,36 @@ def test_cli_fft_dump(gpu, tmp_path):
             ty, tx = divmod(t, 2)
             F = np.fft.rfft2(frames[6 + f, ty * 16:(ty + 1) * 16, tx * 16:(tx + 1) * 16].astype(float)) / 256
             assert np.allclose(vals[f * 4 + t], (np.abs(F) ** 2).ravel(), rtol=2e-5, atol=1e-9)
+
+
+def test_cli_reference_example1(gpu, tmp_path):
+    """resources/example1/run_example: -Q/-T/-S from the reference's files
+    (scales 256, 512, 1024 in ascending order), -C 60 -F 60 -v, on a
+    synthetic 1024^2 .movie (120 frames instead of 1500)."""
+    from tests.test_host import example_files, read_params
+    q, t, s = example_files("example1")
+    lams, lags, scales = read_params(q, t, s)
+    frames = ddm.synth_brownian(120, 1024, 1024, 8, 8000, seed=41, nthreads=16)
+    mv = tmp_path / "ex1.movie"
+    write_movie(mv, frames, camera=1, pixelmode=1, fps=60.0)
+    out = _run("-M", "-f", mv, "-o", tmp_path / "out", "-Q", q, "-T", t, "-S", s, "-N", 120, "-C", 60, "-v",
+               "-F", 60)
+    assert "[Time elapsed]" in out
+    cfg = o.Config(1024, 1024, 120, scales, lags, lams, 1.2)
+    _check_files(str(tmp_path / "out"), cfg, o.run(cfg, frames), fps=60.0)
+
+
+def test_cli_reference_example2(gpu, tmp_path):
+    """resources/example2/run_example (a web camera there; a 1280x720 .movie
+    here): -Q/-T/-S from the reference's files (scales 256, 512 ascending,
+    lambdas 1..16, lags 1..58), -C 75 -x 600 -y 100 -v, 150 frames."""
+    from tests.test_host import example_files, read_params
+    q, t, s = example_files("example2")
+    lams, lags, scales = read_params(q, t, s)
+    frames = ddm.synth_brownian(150, 1280, 720, 8, 6000, seed=42, nthreads=16)
+    mv = tmp_path / "ex2.movie"
+    write_movie(mv, frames, camera=1, pixelmode=1, fps=30.0)
+    _run("-M", "-f", mv, "-o", tmp_path / "out", "-Q", q, "-T", t, "-S", s, "-N", 150, "-C", 75, "-v", "-x", 600,
+         "-y", 100, "-I")
+    cfg = o.Config(1280, 720, 150, scales, lags, lams, 1.2, x_off=600, y_off=100)
+    _check_files(str(tmp_path / "out"), cfg, o.run(cfg, frames), fps=1.0)

@@ -262,3 +262,43 @@ def test_cli_movie_errors(tmp_path):
     write_movie(ok, f)
     rc, out, err = _cli("-M", "-f", ok, "-v", "-N", 1, "-o", tmp_path / "o", *_params(tmp_path, scales=(256,)))
     assert rc == 1 and "the specified out dimensions must be smaller than actual image size" in err
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+EXAMPLES = {  # the reference's example parameter files (resources/example1, example2), kept as data
+    "example1": ("lambda_vector.txt", "tau_vector.txt", "scale_vector.txt"),
+    "example2": ("lamda_vector.txt", "tau_vector.txt", "scale_vector.txt"),
+}
+
+
+def example_files(name):
+    return [os.path.join(GOLDEN, name, f) for f in EXAMPLES[name]]
+
+
+def read_params(lam_path, tau_path, scale_path):
+    """Whitespace-separated float / int / int, as src/main.cpp:246-313 reads them."""
+    lams = [float(np.float32(x)) for x in open(lam_path).read().split()]
+    lags = [int(x) for x in open(tau_path).read().split()]
+    scales = [int(x) for x in open(scale_path).read().split()]
+    return lams, lags, scales
+
+
+@pytest.mark.parametrize("name", sorted(EXAMPLES))
+def test_reference_example_parameter_files(name):
+    lams, lags, scales = read_params(*example_files(name))
+    assert scales == sorted(scales)  # ascending in the files: the engine sorts them descending
+    M = max(scales)
+    rc, msg, info = ddm.validate(width=M + 700, height=M + 120, scales=scales, lags=lags, lambdas=lams,
+                                 chunk=60, x_off=600, y_off=100)
+    assert rc == 0, msg
+    assert info.largest_scale == M and info.scale_count == len(scales)
+    assert info.lag_count == len(lags) and info.lambda_count == len(lams)
+    assert info.result_count == sum((M // s) ** 2 for s in scales) * len(lams) * len(lags)
+
+
+def test_cli_example2_offsets_outside_benchmark_frame(tmp_path):
+    """example2's run line (-x 600 -y 100) with its files on a benchmark frame
+    (the largest scale, 512^2): refused before any device access."""
+    q, t, s = example_files("example2")
+    rc, _, err = _cli("-B", "-N", 10, "-C", 75, "-x", 600, "-y", 100, "-o", tmp_path / "o", "-Q", q, "-T", t, "-S", s)
+    assert rc == 1 and "[Error] the specified out dimensions must be smaller than actual image size" in err
