@@ -327,6 +327,8 @@ def main():
                     help="time the steps with frames in pinned host memory (PCIe-inclusive; not the headline)")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x2", "fp16x2"],
                     help="64/128 tile transforms: FP32 FFT or split-operand DFT on the matrix cores")
+    ap.add_argument("--accumulation", default="compensated", choices=["compensated", "plain"],
+                    help="lag sums: compensated FP32 folded into FP64 every 10^4 pairs, or plain FP32")
     ap.add_argument("--autotune", action="store_true",
                     help="let the engine's autotuner pick the lag block and FFT batch (ddm_autotune)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -383,7 +385,8 @@ def main():
     schedule = {}
 
     def make():
-        e = ddm.Engine(*W.engine_args(), lag_block=args.lag_block)
+        e = ddm.Engine(*W.engine_args(), lag_block=args.lag_block,
+                       accumulation=ddm.ACC_PLAIN if args.accumulation == "plain" else ddm.ACC_COMPENSATED)
         e.set_precision(prec)
         if args.autotune:
             # timed on the first engine, read back from the cache by the others
@@ -479,7 +482,7 @@ def main():
             # ~128 frames per batch (the engine's FFT batch)
             "fft_batch_frames": schedule.get("fft_batch_frames", W.chunk * max(1, 128 // W.chunk)),
             "lag_block": lag_block, "schedule": "autotuned" if args.autotune else "default",
-            "precision": args.precision,
+            "precision": args.precision, "accumulation": args.accumulation,
         },
         "stage_ms_per_step": {"fft": round(st.fft_ms, 3), "lag": round(st.lag_ms, 3),
                               "azimuthal": round(st.azimuthal_ms, 3)},

@@ -50,6 +50,10 @@ struct ddm_engine {
     int fft_batch = 30;  // frames of device-resident input transformed per batch
     float *d_ring = nullptr;
     float *d_acc = nullptr;
+    float *d_comp = nullptr;     // compensation terms of d_acc (DDM_ACC_COMPENSATED)
+    double *d_acc64 = nullptr;   // FP64 accumulators d_acc + d_comp are folded into (allocated on first fold)
+    bool acc64_live = false;     // d_acc64 holds sums of the current window
+    long long frames_since_fold = 0;  // frames (pairs per lag and element, at most) in d_acc since the last fold
     int *d_lags = nullptr;
     ddm::AzScale *d_az = nullptr;
     int *d_wl = nullptr;            // narrow-ring average waves: q | lag group << 16
@@ -164,8 +168,8 @@ static void release(ddm_engine *e) {
     e->pool.clear();
     // device buffers come from the stream-ordered pool: freeing them does not
     // synchronise the device (other engines keep running)
-    void *dev[] = {e->d_ring, e->d_acc, e->d_norm, e->d_lags, e->d_az,  e->d_qoff, e->d_qcnt,
-                   e->d_idx,  e->d_out, e->d_tau,  e->d_fit,  e->d_xt, e->d_wl};
+    void *dev[] = {e->d_ring, e->d_acc, e->d_comp, e->d_acc64, e->d_norm, e->d_lags, e->d_az,  e->d_qoff,
+                   e->d_qcnt, e->d_idx, e->d_out,  e->d_tau,   e->d_fit,  e->d_xt,   e->d_wl};
     for (void *ptr : dev)
         if (ptr && e->st) (void)hipFreeAsync(ptr, e->st);
     for (uint8_t *ptr : e->d_stage)
@@ -246,6 +250,7 @@ extern "C" int ddm_create(const ddm_desc *desc, ddm_engine **out) {
     }
     TRY(dev_alloc(e, &e->d_ring, (size_t)p.ring_slots * (size_t)E * 2, "spectrum ring"));
     TRY(dev_alloc(e, &e->d_acc, L * (size_t)E, "lag accumulators"));
+    if (p.acc_mode == DDM_ACC_COMPENSATED) TRY(dev_alloc(e, &e->d_comp, L * (size_t)E, "compensation terms"));
     TRY(dev_alloc(e, &e->d_lags, L, "lags"));
     TRY(dev_alloc(e, &e->d_az, p.geom.size(), "scale table"));
     TRY(dev_alloc(e, &e->d_qoff, e->rt.off.size(), "ring offsets"));
@@ -311,6 +316,7 @@ extern "C" int ddm_create(const ddm_desc *desc, ddm_engine **out) {
     // written, and the padding elements of the last ring group feed only
     // accumulator entries that no result reads.
     HTRY(hipMemsetAsync(e->d_acc, 0, L * (size_t)E * sizeof(float), e->st));
+    if (e->d_comp) HTRY(hipMemsetAsync(e->d_comp, 0, L * (size_t)E * sizeof(float), e->st));
     const float2 *tw = nullptr;
     HTRY(ddm::twiddle_table(&tw));
     HTRY(hipStreamSynchronize(e->st));
@@ -325,6 +331,21 @@ extern "C" void ddm_destroy(ddm_engine *e) {
     if (!e) return;
     release(e);
     delete e;
+}
+
+// Moves the compensated FP32 sums into the FP64 accumulators (allocated and
+// zeroed on first use) and clears them.
+static int fold_acc(ddm_engine *e) {
+    const size_t n = e->p.lags.size() * (size_t)e->p.elems_padded;
+    if (!e->d_acc64) {
+        int rc = dev_alloc(e, &e->d_acc64, n, "FP64 accumulators");
+        if (rc) return rc;
+        HIP_TRY(e, hipMemsetAsync(e->d_acc64, 0, n * sizeof(double), e->st));
+    }
+    HIP_TRY(e, ddm::launch_acc_fold(e->d_acc, e->d_comp, e->d_acc64, (long long)n, e->st));
+    e->acc64_live = true;
+    e->frames_since_fold = 0;
+    return DDM_OK;
 }
 
 // Lag engine over the next n frames waiting in the ring.
@@ -343,12 +364,17 @@ static int lag_block(ddm_engine *e, int n) {
     a.lags_host = p.lags.data();
     a.lag_count = (int)p.lags.size();
     a.lag_block = e->lag_block;
+    a.comp = e->d_comp;
     Mark lag;
     mark_begin(e, &lag, MK_LAG);
     HIP_TRY(e, ddm::launch_lag(a, e->st));
     mark_end(e, &lag);
     e->stats.lag_launches += 1;
     e->frames_lagged += n;
+    if (e->d_comp) {
+        e->frames_since_fold += n;
+        if (e->frames_since_fold >= DDM_ACC_FOLD_PAIRS) return fold_acc(e);
+    }
     return DDM_OK;
 }
 
@@ -607,6 +633,8 @@ static int compute_results(ddm_engine *e) {
     HIP_TRY(e, hipMemcpyAsync(e->d_norm, norm.data(), L * sizeof(double), hipMemcpyHostToDevice, e->st));
     ddm::AzArgs a;
     a.acc = e->d_acc;
+    a.comp = e->d_comp;
+    a.acc64 = e->acc64_live ? e->d_acc64 : nullptr;
     a.elems_per_frame = p.elems_padded;
     a.scales = e->d_az;
     a.scale_count = (int)p.geom.size();
@@ -691,7 +719,12 @@ extern "C" int ddm_reset(ddm_engine *e) {
     const Params &p = e->p;
     int rc = flush_lags(e);  // waiting frames belong to the window being closed
     if (rc) return rc;
-    HIP_TRY(e, hipMemsetAsync(e->d_acc, 0, p.lags.size() * (size_t)p.elems_padded * sizeof(float), e->st));
+    const size_t n = p.lags.size() * (size_t)p.elems_padded;
+    HIP_TRY(e, hipMemsetAsync(e->d_acc, 0, n * sizeof(float), e->st));
+    if (e->d_comp) HIP_TRY(e, hipMemsetAsync(e->d_comp, 0, n * sizeof(float), e->st));
+    if (e->acc64_live) HIP_TRY(e, hipMemsetAsync(e->d_acc64, 0, n * sizeof(double), e->st));
+    e->acc64_live = false;
+    e->frames_since_fold = 0;
     e->frames_since_reset = 0;
     std::fill(e->pairs.begin(), e->pairs.end(), 0);
     return DDM_OK;
@@ -741,8 +774,22 @@ extern "C" int ddm_get_accumulators(ddm_engine *e, int scale_idx, float *out) {
                              hipMemcpyDeviceToHost));
     // the reference accumulates (|dF| / s^2)^2; 1/s^4 is a power of two, so the
     // scaling is exact
-    const float inv = (float)(1.0 / ((double)g.s * g.s * (double)g.s * g.s));
-    for (size_t i = 0; i < n * p.lags.size(); ++i) out[i] *= inv;
+    const double inv = 1.0 / ((double)g.s * g.s * (double)g.s * g.s);
+    if (!e->d_comp) {
+        for (size_t i = 0; i < n * p.lags.size(); ++i) out[i] *= (float)inv;
+        return DDM_OK;
+    }
+    // compensated: acc + comp (+ the FP64 part), rounded once to FP32
+    std::vector<float> c(n);
+    std::vector<double> d(e->acc64_live ? n : 0);
+    for (size_t l = 0; l < p.lags.size(); ++l) {
+        const size_t off = l * (size_t)p.elems_padded + g.elem_off;
+        HIP_TRY(e, hipMemcpy(c.data(), e->d_comp + off, n * sizeof(float), hipMemcpyDeviceToHost));
+        if (e->acc64_live) HIP_TRY(e, hipMemcpy(d.data(), e->d_acc64 + off, n * sizeof(double), hipMemcpyDeviceToHost));
+        float *o = out + l * n;
+        for (size_t i = 0; i < n; ++i)
+            o[i] = (float)((((double)o[i] + (double)c[i]) + (e->acc64_live ? d[i] : 0.0)) * inv);
+    }
     return DDM_OK;
 }
 
@@ -868,6 +915,7 @@ extern "C" int ddm_autotune(ddm_engine *e, const char *cache_path, int *lag_bloc
             a.lags_host = p.lags.data();
             a.lag_count = (int)p.lags.size();
             a.lag_block = cand;
+            a.comp = e->d_comp;
             float ms = 0.f;
             rc = time_ms(e, [&] { return ddm::launch_lag(a, e->st); }, &ms);
             if (rc) return rc;
@@ -937,6 +985,8 @@ extern "C" int ddm_autotune(ddm_engine *e, const char *cache_path, int *lag_bloc
         e->dev_bytes -= (size_t)(fb * bmax);
         // the timing runs accumulated into the lag sums
         HIP_TRY(e, hipMemsetAsync(e->d_acc, 0, p.lags.size() * (size_t)p.elems_padded * sizeof(float), e->st));
+        if (e->d_comp)
+            HIP_TRY(e, hipMemsetAsync(e->d_comp, 0, p.lags.size() * (size_t)p.elems_padded * sizeof(float), e->st));
         HIP_TRY(e, hipStreamSynchronize(e->st));
         if (cache_path) {
             if (FILE *f = std::fopen(cache_path, "a")) {

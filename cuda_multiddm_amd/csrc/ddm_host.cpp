@@ -38,6 +38,8 @@ int check_desc(const ddm_desc *d, Params *out, std::string *msg) {
     if (d->chunk_frames <= 0) return fail(DDM_EINVAL, "chunk frame count must be positive");
     if (d->norm_mode != DDM_NORM_FRAMES && d->norm_mode != DDM_NORM_PAIRS)
         return fail(DDM_EINVAL, "unknown normalisation mode");
+    if (d->accumulation != DDM_ACC_COMPENSATED && d->accumulation != DDM_ACC_PLAIN)
+        return fail(DDM_EINVAL, "unknown accumulation mode");
     if (d->lag_block != 0 && d->lag_block != 32 && d->lag_block != 64 && d->lag_block != 128 &&
         d->lag_block != 256)
         return fail(DDM_EINVAL, "lag block must be 0 (automatic), 32, 64, 128 or 256");
@@ -55,6 +57,7 @@ int check_desc(const ddm_desc *d, Params *out, std::string *msg) {
     p.chunk = d->chunk_frames;
     p.norm_mode = d->norm_mode;
     p.lag_block = d->lag_block ? d->lag_block : kDefaultLagBlock;
+    p.acc_mode = d->accumulation;
 
     // src/DDM.cu:319-321
     std::sort(p.lags.begin(), p.lags.end());
@@ -128,6 +131,8 @@ size_t device_footprint(const Params &p) {
     const size_t L = p.lags.size(), Q = p.lambdas.size();
     size_t bytes = (size_t)p.ring_slots * (size_t)p.elems_padded * 8;  // spectrum ring
     bytes += L * (size_t)p.elems_padded * 4;                           // accumulators
+    if (p.acc_mode == DDM_ACC_COMPENSATED)                              // compensation terms and the FP64
+        bytes += L * (size_t)p.elems_padded * (4 + 8);                  // accumulators they fold into
     bytes += UPLOAD_STAGES * (size_t)p.chunk * p.M * p.M * p.bpp;      // upload staging
     bytes += (size_t)p.result_count * 8 + L * 12;                      // results, lags, norms
     for (const ScaleGeom &g : p.geom) bytes += Q * ((size_t)g.tile_elems * 4 + 8);  // ring tables (bound)

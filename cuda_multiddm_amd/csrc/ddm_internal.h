@@ -68,6 +68,7 @@ struct Params {
     int chunk = 30;
     int norm_mode = DDM_NORM_FRAMES;
     int lag_block = 0;
+    int acc_mode = DDM_ACC_COMPENSATED;
     // derived
     int M = 0;
     int max_lag = 0;

@@ -40,6 +40,9 @@ struct LagArgs {
     const int *lags_host;      // the same lags in host memory (the launch plan is made on the host)
     int lag_count;
     int lag_block;             // 32, 64, 128 or 256
+    // compensated accumulation: the rounding error of every addition into
+    // acc goes to comp ([lag][elems_per_frame], same layout); nullptr: plain
+    float *comp = nullptr;
 };
 
 struct AzScale {
@@ -59,6 +62,8 @@ __host__ __device__ inline int az_lanes(int cnt) { return cnt <= 8 ? 8 : (cnt <=
 
 struct AzArgs {
     const float *acc;
+    const float *comp = nullptr;    // compensation terms of acc (same layout), or none
+    const double *acc64 = nullptr;  // FP64 accumulators folded out of acc + comp, or none (needs comp)
     long long elems_per_frame;
     const AzScale *scales;     // device
     int scale_count;
@@ -86,6 +91,8 @@ hipStream_t engine_stream(ddm_engine *e);
 hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st);
 hipError_t launch_lag(const LagArgs &a, hipStream_t st);
 hipError_t launch_azimuthal(const AzArgs &a, hipStream_t st);
+// acc64[i] += acc[i] + comp[i]; acc[i] = comp[i] = 0 for i < n (n a multiple of 4).
+hipError_t launch_acc_fold(float *acc, float *comp, double *acc64, long long n, hipStream_t st);
 // Fits A (1 - exp(-F tau)) + B to n curves of L points: out[c][4] = F, A, B, rms.
 hipError_t launch_fit(const double *curves, const double *tau, int L, long long n, int iters, double *out,
                       hipStream_t st);

@@ -364,26 +364,45 @@ __global__ __launch_bounds__(LargeCfg<S>::NTH_COLS) void tile_fft_cols(RingDst r
 // ---------------------------------------------------------------------------
 constexpr int AZ_WAVES = 4;
 
+// The lag sum of one accumulator entry in FP64: the FP32 accumulator, plus
+// (SRC >= 1) its compensation term, plus (SRC >= 2) the FP64 part folded
+// out of them earlier.
+template <int SRC>
+struct AccSrc {
+    const float *a;
+    const float *c;
+    const double *d;
+    __device__ __forceinline__ double operator[](int i) const {
+        double v = (double)a[i];
+        if constexpr (SRC >= 1) v += (double)c[i];
+        if constexpr (SRC >= 2) v += d[i];
+        return v;
+    }
+    __device__ __forceinline__ AccSrc shift(long long k) const {
+        return AccSrc{a + k, SRC >= 1 ? c + k : c, SRC >= 2 ? d + k : d};
+    }
+};
+
 // sum_{i = first, first + step, ... < cnt} a[ix[i]] in FP64: four independent
 // partial sums, so four element loads per lane are in flight at once; the
 // order is fixed, so the result does not depend on timing.
-__device__ __forceinline__ double ring_sum(const float *__restrict__ a, const int *__restrict__ ix, int first,
-                                           int step, int cnt) {
+template <class A>
+__device__ __forceinline__ double ring_sum(const A a, const int *__restrict__ ix, int first, int step, int cnt) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int i = first;
     for (; i + 3 * step < cnt; i += 4 * step) {
         const int i0 = ix[i], i1 = ix[i + step], i2 = ix[i + 2 * step], i3 = ix[i + 3 * step];
-        s0 += (double)a[i0];
-        s1 += (double)a[i1];
-        s2 += (double)a[i2];
-        s3 += (double)a[i3];
+        s0 += a[i0];
+        s1 += a[i1];
+        s2 += a[i2];
+        s3 += a[i3];
     }
-    for (; i < cnt; i += step) s0 += (double)a[ix[i]];
+    for (; i < cnt; i += step) s0 += a[ix[i]];
     return (s0 + s1) + (s2 + s3);
 }
 
-template <int WPR>
-__global__ __launch_bounds__(AZ_WAVES * 64) void azimuthal_kernel(const float *__restrict__ acc, long long E,
+template <int WPR, int SRC>
+__global__ __launch_bounds__(AZ_WAVES * 64) void azimuthal_kernel(AccSrc<SRC> acc, long long E,
                                                                   const AzScale *__restrict__ scales, int S,
                                                                   const int *__restrict__ q_off,
                                                                   const int *__restrict__ q_cnt,
@@ -409,7 +428,7 @@ __global__ __launch_bounds__(AZ_WAVES * 64) void azimuthal_kernel(const float *_
         return;
     }
     const int *ix = idx + q_off[si * Q + q];
-    const float *a = acc + (long long)l * E + sc.elem_off + tile * sc.tile_elems;
+    const AccSrc<SRC> a = acc.shift((long long)l * E + sc.elem_off + tile * sc.tile_elems);
     double sum = ring_sum(a, ix, tid, NTR, cnt);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) sum += __shfl_down(sum, o, 64);
@@ -432,8 +451,9 @@ __global__ __launch_bounds__(AZ_WAVES * 64) void azimuthal_kernel(const float *_
 // the P lanes of each lag are added by an xor butterfly, and the lane with
 // p = 0 writes the value.  Small rings thus fill the wave instead of one
 // lag leaving most lanes idle.
+template <int SRC>
 __global__ __launch_bounds__(AZ_WAVES * 64) void azimuthal_narrow_kernel(
-    const float *__restrict__ acc, long long E, const AzScale *__restrict__ scales, int S, int s0,
+    AccSrc<SRC> acc, long long E, const AzScale *__restrict__ scales, int S, int s0,
     const int *__restrict__ q_off, const int *__restrict__ q_cnt, const int *__restrict__ idx,
     const int *__restrict__ wl, const double *__restrict__ norm, int Q, int L, double *__restrict__ out,
     long long nwaves) {
@@ -454,12 +474,31 @@ __global__ __launch_bounds__(AZ_WAVES * 64) void azimuthal_narrow_kernel(
     double sum = 0.0;
     if (l < L) {
         const int *ix = idx + q_off[si * Q + q];
-        const float *a = acc + (long long)l * E + sc.elem_off + tile * sc.tile_elems;
+        const AccSrc<SRC> a = acc.shift((long long)l * E + sc.elem_off + tile * sc.tile_elems);
         sum = ring_sum(a, ix, p, P, cnt);
     }
     for (int o = P / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
     if (p == 0 && l < L)
         out[sc.out_base + (tile * Q + q) * L + l] = cnt ? 2.0 * sum * sc.inv_s4 / (double)cnt * norm[l] : 0.0;
+}
+
+// acc64 += acc + comp, then acc = comp = 0: the FP32 pair's value moves to
+// the FP64 accumulators (the two FP32 terms add exactly in FP64).
+__global__ __launch_bounds__(256) void acc_fold_kernel(float4 *__restrict__ acc, float4 *__restrict__ comp,
+                                                       double2 *__restrict__ acc64, long long n4) {
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n4; i += (long long)gridDim.x * 256) {
+        const float4 a = acc[i], c = comp[i];
+        double2 d0 = acc64[2 * i], d1 = acc64[2 * i + 1];
+        d0.x += (double)a.x + (double)c.x;
+        d0.y += (double)a.y + (double)c.y;
+        d1.x += (double)a.z + (double)c.z;
+        d1.y += (double)a.w + (double)c.w;
+        acc64[2 * i] = d0;
+        acc64[2 * i + 1] = d1;
+        acc[i] = z;
+        comp[i] = z;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -542,34 +581,55 @@ hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int WPR>
+template <int WPR, int SRC>
 static void launch_az_range(const AzArgs &a, long long first, long long end, hipStream_t st) {
     constexpr long long per_block = AZ_WAVES / WPR;
+    const AccSrc<SRC> acc{a.acc, a.comp, a.acc64};
     // grid.x is limited to 2^31 - 1 workgroups; split larger result sets
     const long long maxn = (1LL << 30) * per_block;
     for (; first < end; first += maxn) {
         const long long n = (end - first) < maxn ? (end - first) : maxn;
         const unsigned blocks = (unsigned)((n + per_block - 1) / per_block);
-        hipLaunchKernelGGL(azimuthal_kernel<WPR>, dim3(blocks), dim3(AZ_WAVES * 64), 0, st, a.acc,
+        hipLaunchKernelGGL((azimuthal_kernel<WPR, SRC>), dim3(blocks), dim3(AZ_WAVES * 64), 0, st, acc,
                            a.elems_per_frame, a.scales, a.scale_count, a.q_off, a.q_cnt, a.idx, a.norm,
                            a.lambda_count, a.lag_count, a.out, first, n);
     }
 }
 
-hipError_t launch_azimuthal(const AzArgs &a, hipStream_t st) {
+template <int SRC>
+static hipError_t launch_azimuthal_t(const AzArgs &a, hipStream_t st) {
     const long long split = a.wide_items < a.result_count ? a.wide_items : a.result_count;
-    if (split > 0) launch_az_range<AZ_WAVES>(a, 0, split, st);
+    if (split > 0) launch_az_range<AZ_WAVES, SRC>(a, 0, split, st);
     if (split < a.result_count) {
         if (a.wave_list && a.narrow_waves > 0) {
             const long long blocks = (a.narrow_waves + AZ_WAVES - 1) / AZ_WAVES;
             if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-            hipLaunchKernelGGL(azimuthal_narrow_kernel, dim3((unsigned)blocks), dim3(AZ_WAVES * 64), 0, st, a.acc,
-                               a.elems_per_frame, a.scales, a.scale_count, a.narrow_first_scale, a.q_off, a.q_cnt,
-                               a.idx, a.wave_list, a.norm, a.lambda_count, a.lag_count, a.out, a.narrow_waves);
+            const AccSrc<SRC> acc{a.acc, a.comp, a.acc64};
+            hipLaunchKernelGGL((azimuthal_narrow_kernel<SRC>), dim3((unsigned)blocks), dim3(AZ_WAVES * 64), 0, st,
+                               acc, a.elems_per_frame, a.scales, a.scale_count, a.narrow_first_scale, a.q_off,
+                               a.q_cnt, a.idx, a.wave_list, a.norm, a.lambda_count, a.lag_count, a.out,
+                               a.narrow_waves);
         } else {
-            launch_az_range<1>(a, split, a.result_count, st);
+            launch_az_range<1, SRC>(a, split, a.result_count, st);
         }
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_azimuthal(const AzArgs &a, hipStream_t st) {
+    if (a.acc64 && !a.comp) return hipErrorInvalidValue;
+    if (a.acc64) return launch_azimuthal_t<2>(a, st);
+    if (a.comp) return launch_azimuthal_t<1>(a, st);
+    return launch_azimuthal_t<0>(a, st);
+}
+
+hipError_t launch_acc_fold(float *acc, float *comp, double *acc64, long long n, hipStream_t st) {
+    if (n % 4 != 0) return hipErrorInvalidValue;
+    const long long n4 = n / 4;
+    const long long want = (n4 + 255) / 256;
+    const unsigned blocks = (unsigned)(want < 4096 ? (want > 0 ? want : 1) : 4096);
+    hipLaunchKernelGGL(acc_fold_kernel, dim3(blocks), dim3(256), 0, st, (float4 *)acc, (float4 *)comp,
+                       (double2 *)acc64, n4);
     return hipGetLastError();
 }
 

@@ -390,32 +390,75 @@ __device__ __forceinline__ void hist_sums(const GroupRing &g, const float2 *nl, 
     }
 }
 
+// A lane's accumulators: acc[l][e] and, for compensated accumulation,
+// comp[l][e], the rounding error of every addition into acc so far
+// (acc + comp is the sum carried to twice the precision of FP32).
+template <bool COMP>
+struct AccRef {
+    float *a;
+    float *c;  // unused unless COMP
+};
+
 // The accumulators of the first PF lags wave w adds (lags w, w + 16, ...),
 // requested when the workgroup starts so that they have arrived by the time
 // the partial sums are added.
-template <int PF>
-__device__ __forceinline__ void load_acc(const float *accg, long long E, int w, int nred, float (&pre)[PF]) {
+template <int PF, bool COMP>
+struct AccPre {
+    float a[PF];
+    float c[COMP ? PF : 1];
+};
+
+template <int PF, bool COMP>
+__device__ __forceinline__ void load_acc(AccRef<COMP> ar, long long E, int w, int nred, AccPre<PF, COMP> &pre) {
 #pragma unroll
     for (int m = 0; m < PF; ++m) {
         const int l = w + m * LAG_WAVES;
-        pre[m] = l < nred ? accg[(long long)l * E] : 0.f;
+        pre.a[m] = l < nred ? ar.a[(long long)l * E] : 0.f;
+    }
+}
+
+// The compensation terms of the same lags, requested when the pair work is
+// done (holding them through it would take registers the pair loops need).
+template <int PF, bool COMP>
+__device__ __forceinline__ void load_comp(AccRef<COMP> ar, long long E, int w, int nred, AccPre<PF, COMP> &pre) {
+    if constexpr (COMP) {
+#pragma unroll
+        for (int m = 0; m < PF; ++m) {
+            const int l = w + m * LAG_WAVES;
+            pre.c[m] = l < nred ? ar.c[(long long)l * E] : 0.f;
+        }
     }
 }
 
 // acc[l] += t for the m-th lag this wave adds (l = w + 16 m), from the early
-// read when there is one.
-template <int PF>
-__device__ __forceinline__ void add_acc(float *accg, long long E, int l, int m, float t, const float (&pre)[PF]) {
-    float a = 0.f;
+// read when there is one (m < 0: none).  Compensated: the sum is rounded to
+// FP32 and its exact rounding error (TwoSum, no multiplications, so no
+// contraction can change it) is added to comp.
+template <int PF, bool COMP>
+__device__ __forceinline__ void add_acc(AccRef<COMP> ar, long long E, int l, int m, float t,
+                                        const AccPre<PF, COMP> &pre) {
+    float a = 0.f, c = 0.f;
     bool have = false;
 #pragma unroll
     for (int i = 0; i < PF; ++i)
         if (i == m) {
-            a = pre[i];
+            a = pre.a[i];
+            if constexpr (COMP) c = pre.c[i];
             have = true;
         }
-    float *p = accg + (long long)l * E;
-    *p = (have ? a : *p) + t;
+    float *p = ar.a + (long long)l * E;
+    if (!have) a = *p;
+    if constexpr (COMP) {
+        float *q = ar.c + (long long)l * E;
+        if (!have) c = *q;
+        const float s = a + t;
+        const float bp = s - a;
+        const float err = (a - (s - bp)) + (t - bp);
+        *p = s;
+        *q = c + err;
+    } else {
+        *p = a + t;
+    }
 }
 
 // Partial sums of a launch with history-major lags meet in the staging buffer
@@ -423,11 +466,12 @@ __device__ __forceinline__ void add_acc(float *accg, long long E, int l, int m, 
 // r, history wave k writes its sums of lags 32r + i to rows wd*SL + 32k + i,
 // and in round 0 dense wave w also writes its window sums to rows w*SL + i.
 // Each lag is then added up by one wave in a fixed order.
-template <int LSM, int SL, int DN, int RL, int SM, int PF>
-__device__ __forceinline__ void hist_reduce(float2 *nb, float *accg, long long E, const float (&s)[SM],
+template <int LSM, int SL, int DN, int RL, int SM, int PF, bool COMP>
+__device__ __forceinline__ void hist_reduce(float2 *nb, AccRef<COMP> accg, long long E, const float (&s)[SM],
                                             const float (&h)[LSM], int w, int wd, int hw0, int nhw, int sd,
-                                            int dense, int ls, const float (&pre)[PF]) {
+                                            int dense, int ls, AccPre<PF, COMP> &pre) {
     static_assert(SL <= SM && LSM % RL == 0, "window sums exceed the partial registers");
+    load_comp(accg, E, w, dense + ls, pre);
     float *red = reinterpret_cast<float *>(nb);
     const int lane = threadIdx.x % RING_GROUP;
     const int hb = wd * SL;
@@ -480,10 +524,10 @@ __device__ __forceinline__ void hist_reduce(float2 *nb, float *accg, long long E
 //    wave that adds its sum to the accumulator directly.
 // Partial sums meet in LDS after the pair work and are added per lag in a
 // fixed order, so results do not depend on timing.
-template <int J>
+template <int J, bool COMP>
 __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J>::WAVES_PER_EU))) void lag_kernel(
-    const float2 *__restrict__ ring, float *__restrict__ acc, long long E, int R, int slot0, int n, int dmax,
-    const int *__restrict__ lags, int L, const LagPlan pl) {
+    const float2 *__restrict__ ring, float *__restrict__ acc, float *__restrict__ comp, long long E, int R,
+    int slot0, int n, int dmax, const int *__restrict__ lags, int L, const LagPlan pl) {
     using C = LagCfg<J>;
     constexpr int DENSE = C::DENSE, U = C::UNROLL;
     constexpr int SLOTS = (J / 8) < C::SLOTS_MAX ? (J / 8) : C::SLOTS_MAX;  // partials per wave that fit in LDS
@@ -523,9 +567,10 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
     const int ws = LAG_WAVES - wd;
     // the accumulators this wave will add to are requested beside the staging
     // loads, so they have arrived when the partial sums are added
-    float *accg = acc + (long long)blockIdx.x * RING_GROUP + lane;
+    const long long lane_off = (long long)blockIdx.x * RING_GROUP + lane;
+    AccRef<COMP> accg{acc + lane_off, COMP ? comp + lane_off : nullptr};
     const int nred = hist ? L : (ss ? L : dense_lags);
-    float pre[ACC_PREFETCH];
+    AccPre<ACC_PREFETCH, COMP> pre;
     load_acc(accg, E, w, nred, pre);
     lds_barrier();
     const float2 *nl = nb + lane;
@@ -567,7 +612,7 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
         }
     } else {
         for (int l = dense_lags + (w - wd); l < L; l += ws)
-            accg[(long long)l * E] += lag_range_sum<U>(g, nl, R, slot0, dmax, lags[l], 0, n);
+            add_acc(accg, E, l, -1, lag_range_sum<U>(g, nl, R, slot0, dmax, lags[l], 0, n), pre);
     }
     if constexpr (C::HIST_LAGS > 0) {
         if (hist) {
@@ -588,6 +633,7 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
     }
     if (wd == 0 && ss == 0) return;
     // ---- combine the partials: [wave][slot][lane] in the staging buffer ----
+    load_comp(accg, E, w, nred, pre);
     lds_barrier();
     float *red = reinterpret_cast<float *>(nb);
 #pragma unroll
@@ -719,8 +765,14 @@ static void launch_lag_t(const LagArgs &a, hipStream_t st) {
     LagPlan pl;
     make_plan<J>(a.lags_host, a.lag_count, a.n_new, a.dmax, pl);
     const unsigned grid = (unsigned)(a.elems_per_frame / RING_GROUP);
-    hipLaunchKernelGGL(lag_kernel<J>, dim3(grid), dim3(LAG_NT), 0, st, (const float2 *)a.ring, a.acc,
-                       a.elems_per_frame, a.ring_slots, a.slot0, a.n_new, a.dmax, a.lags, a.lag_count, pl);
+    if (a.comp)
+        hipLaunchKernelGGL((lag_kernel<J, true>), dim3(grid), dim3(LAG_NT), 0, st, (const float2 *)a.ring, a.acc,
+                           a.comp, a.elems_per_frame, a.ring_slots, a.slot0, a.n_new, a.dmax, a.lags, a.lag_count,
+                           pl);
+    else
+        hipLaunchKernelGGL((lag_kernel<J, false>), dim3(grid), dim3(LAG_NT), 0, st, (const float2 *)a.ring, a.acc,
+                           nullptr, a.elems_per_frame, a.ring_slots, a.slot0, a.n_new, a.dmax, a.lags, a.lag_count,
+                           pl);
 }
 
 hipError_t launch_lag(const LagArgs &a, hipStream_t st) {

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define DDM_ABI_VERSION 1
+#define DDM_ABI_VERSION 2
 
 /* status codes */
 #define DDM_OK 0
@@ -69,6 +69,20 @@ enum ddm_norm_mode { DDM_NORM_FRAMES = 0, DDM_NORM_PAIRS = 1 };
  * FP32 accumulation. */
 enum ddm_precision { DDM_PREC_FP32 = 0, DDM_PREC_BF16X2 = 1, DDM_PREC_FP16X2 = 2 };
 
+/* How the lag sums accumulate over a stream (ddm_desc.accumulation).
+ * DDM_ACC_COMPENSATED (default): every lag-engine launch adds its FP32
+ * partial sums to FP32 accumulators with the exact rounding error of each
+ * addition kept in FP32 compensation terms, and once the accumulators hold
+ * 10^4 frame pairs they are folded into FP64 accumulators on the device, so
+ * the error does not grow with the stream's length.  DDM_ACC_PLAIN: FP32
+ * accumulators only, as the reference adds each term in FP32
+ * (src/DDM_kernel.cuh:85-101, merge src/DDM_kernel.cuh:127-139). */
+enum ddm_accumulation { DDM_ACC_COMPENSATED = 0, DDM_ACC_PLAIN = 1 };
+
+/* Pairs of frames the FP32 accumulators take before they are folded into
+ * FP64 (DDM_ACC_COMPENSATED). */
+#define DDM_ACC_FOLD_PAIRS 10000
+
 typedef struct ddm_desc {
     int width;            /* frame width in pixels */
     int height;           /* frame height in pixels */
@@ -85,6 +99,7 @@ typedef struct ddm_desc {
     int chunk_frames;     /* frames per upload / FFT batch (-C) */
     int norm_mode;        /* enum ddm_norm_mode */
     int lag_block;        /* frames per lag-engine pass: 0 = automatic (256), else 32, 64, 128 or 256 */
+    int accumulation;     /* enum ddm_accumulation (0 = compensated with FP64 folding) */
 } ddm_desc;
 
 /* Sizes the engine derived from a descriptor (after sorting). */

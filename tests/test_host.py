@@ -302,3 +302,17 @@ def test_cli_example2_offsets_outside_benchmark_frame(tmp_path):
     q, t, s = example_files("example2")
     rc, _, err = _cli("-B", "-N", 10, "-C", 75, "-x", 600, "-y", 100, "-o", tmp_path / "o", "-Q", q, "-T", t, "-S", s)
     assert rc == 1 and "[Error] the specified out dimensions must be smaller than actual image size" in err
+
+
+def test_accumulation_mode_is_validated():
+    """ddm_desc.accumulation is checked before any device access, and the
+    compensated mode's terms (FP32 compensation + FP64 accumulators) are part
+    of the footprint a server admits by."""
+    with pytest.raises(ddm.DDMError) as ei:
+        ddm.Engine(32, 32, [32], [1], [4.0], accumulation=7)
+    assert ei.value.code == -1
+    rc, _, info_c = ddm.validate(width=64, height=64, scales=[64], lags=[1, 2], lambdas=[4.0])
+    rc2, _, info_p = ddm.validate(width=64, height=64, scales=[64], lags=[1, 2], lambdas=[4.0],
+                                  accumulation=ddm.ACC_PLAIN)
+    assert rc == 0 and rc2 == 0
+    assert info_c.device_bytes - info_p.device_bytes == 2 * info_c.elems_per_frame_padded * 12
