@@ -185,9 +185,9 @@ static void release(ddm_engine *e) {
 }
 
 template <class T>
-static int dev_alloc(ddm_engine *e, T **ptr, size_t count, const char *what) {
+static int dev_alloc(ddm_engine *e, T **ptr, size_t count, const char *what, hipStream_t st = nullptr) {
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    hipError_t rc = hipMallocAsync((void **)ptr, bytes, e->st);
+    hipError_t rc = hipMallocAsync((void **)ptr, bytes, st ? st : e->st);
     if (rc != hipSuccess) {
         *ptr = nullptr;
         e->err = std::string("device allocation of ") + what + " (" + std::to_string(bytes) +
@@ -335,14 +335,14 @@ extern "C" void ddm_destroy(ddm_engine *e) {
 
 // Moves the compensated FP32 sums into the FP64 accumulators (allocated and
 // zeroed on first use) and clears them.
-static int fold_acc(ddm_engine *e) {
+static int fold_acc(ddm_engine *e, hipStream_t st) {
     const size_t n = e->p.lags.size() * (size_t)e->p.elems_padded;
     if (!e->d_acc64) {
-        int rc = dev_alloc(e, &e->d_acc64, n, "FP64 accumulators");
+        int rc = dev_alloc(e, &e->d_acc64, n, "FP64 accumulators", st);
         if (rc) return rc;
-        HIP_TRY(e, hipMemsetAsync(e->d_acc64, 0, n * sizeof(double), e->st));
+        HIP_TRY(e, hipMemsetAsync(e->d_acc64, 0, n * sizeof(double), st));
     }
-    HIP_TRY(e, ddm::launch_acc_fold(e->d_acc, e->d_comp, e->d_acc64, (long long)n, e->st));
+    HIP_TRY(e, ddm::launch_acc_fold(e->d_acc, e->d_comp, e->d_acc64, (long long)n, st));
     e->acc64_live = true;
     e->frames_since_fold = 0;
     return DDM_OK;
@@ -373,7 +373,7 @@ static int lag_block(ddm_engine *e, int n) {
     e->frames_lagged += n;
     if (e->d_comp) {
         e->frames_since_fold += n;
-        if (e->frames_since_fold >= DDM_ACC_FOLD_PAIRS) return fold_acc(e);
+        if (e->frames_since_fold >= DDM_ACC_FOLD_PAIRS) return fold_acc(e, e->st);
     }
     return DDM_OK;
 }
@@ -524,24 +524,74 @@ extern "C" int ddm_reserve(ddm_engine *e, int max_frames) {
     return DDM_OK;
 }
 
+// Whether host frames sit in pinned memory (then the copy engine gathers the
+// ROI rows straight from them).
+static bool pinned_host(const void *frames) {
+    hipPointerAttribute_t attr;
+    const bool pinned = hipPointerGetAttributes(&attr, frames) == hipSuccess && attr.type == hipMemoryTypeHost;
+    (void)hipGetLastError();  // pageable memory reports an error here; do not leave it behind
+    return pinned;
+}
+
+// Uploads the ROI rows of n host frames (frame i at src + i * frame bytes)
+// into the next staging buffer, on the copy stream, and records its upload
+// event; waits first until the work that read that buffer last is done.
+// Returns the buffer index in *b_out.
+static int stage_upload(ddm_engine *e, const uint8_t *src, int n, bool direct, int *b_out) {
+    const Params &p = e->p;
+    const size_t row = (size_t)p.M * p.bpp;
+    const size_t roi_frame = row * p.M;
+    const size_t fb = (size_t)p.width * p.height * p.bpp;
+    const bool whole = p.x_off == 0 && p.y_off == 0 && p.width == p.M && p.height == p.M;
+    const bool rows_stack = p.y_off == 0 && p.height == p.M;  // ROI rows of consecutive frames are evenly spaced
+    const int b = e->next_stage;
+    if (e->stage_used[b]) HIP_TRY(e, hipEventSynchronize(e->ev_free[b]));
+    if (direct && whole) {
+        HIP_TRY(e, hipMemcpyAsync(e->d_stage[b], src, roi_frame * n, hipMemcpyHostToDevice, e->st_copy));
+    } else if (direct) {
+        const size_t pitch = (size_t)p.width * p.bpp;
+        const uint8_t *roi0 = src + (size_t)p.x_off * p.bpp;
+        if (rows_stack) {
+            HIP_TRY(e, hipMemcpy2DAsync(e->d_stage[b], row, roi0, pitch, row, (size_t)p.M * n, hipMemcpyHostToDevice,
+                                        e->st_copy));
+        } else {
+            for (int f = 0; f < n; ++f)
+                HIP_TRY(e, hipMemcpy2DAsync(e->d_stage[b] + (size_t)f * roi_frame, row,
+                                            roi0 + (size_t)f * fb + (size_t)p.y_off * pitch, pitch, row, p.M,
+                                            hipMemcpyHostToDevice, e->st_copy));
+        }
+    } else {
+        if (!e->h_stage[b]) {
+            HIP_TRY(e, hipHostMalloc((void **)&e->h_stage[b], roi_frame * e->stage_frames, hipHostMallocDefault));
+            e->pinned_bytes += roi_frame * e->stage_frames;
+        }
+        // ROI-only copy into pinned memory (the reference uploads whole frames, src/DDM.cu:506)
+        uint8_t *dst = e->h_stage[b];
+        for (int f = 0; f < n; ++f) {
+            const uint8_t *fr = src + (size_t)f * fb;
+            for (int y = 0; y < p.M; ++y)
+                std::memcpy(dst + (size_t)f * roi_frame + (size_t)y * row,
+                            fr + ((size_t)(p.y_off + y) * p.width + p.x_off) * p.bpp, row);
+        }
+        HIP_TRY(e, hipMemcpyAsync(e->d_stage[b], dst, roi_frame * n, hipMemcpyHostToDevice, e->st_copy));
+    }
+    HIP_TRY(e, hipEventRecord(e->ev_up[b], e->st_copy));
+    e->stage_used[b] = true;
+    e->next_stage = (b + 1) % NSTAGE;
+    *b_out = b;
+    return DDM_OK;
+}
+
 extern "C" int ddm_submit_host(ddm_engine *e, const void *frames, int n_frames) {
     if (!e) return DDM_EINVAL;
     if (n_frames < 0 || (n_frames > 0 && !frames)) return fail(e, DDM_EINVAL, "bad frame buffer");
     const Params &p = e->p;
-    const size_t row = (size_t)p.M * p.bpp;
-    const size_t roi_frame = row * p.M;
+    const size_t roi_frame = (size_t)p.M * p.bpp * p.M;
     const uint8_t *src = (const uint8_t *)frames;
     const size_t fb = (size_t)p.width * p.height * p.bpp;
     // Frames that already sit in pinned memory go to the device without the
     // staging copy: the ROI rows are gathered by the copy engine (2-D copy).
-    bool direct = false;
-    if (n_frames > 0) {
-        hipPointerAttribute_t attr;
-        direct = hipPointerGetAttributes(&attr, frames) == hipSuccess && attr.type == hipMemoryTypeHost;
-        (void)hipGetLastError();  // pageable memory reports an error here; do not leave it behind
-    }
-    const bool whole = p.x_off == 0 && p.y_off == 0 && p.width == p.M && p.height == p.M;
-    const bool rows_stack = p.y_off == 0 && p.height == p.M;  // ROI rows of consecutive frames are evenly spaced
+    const bool direct = n_frames > 0 && pinned_host(frames);
     if (n_frames > e->stage_frames && e->stage_frames < e->fft_batch) {
         const int want = std::min(e->fft_batch, (n_frames + p.chunk - 1) / p.chunk * p.chunk);
         int rc = grow_stage(e, want);
@@ -550,44 +600,12 @@ extern "C" int ddm_submit_host(ddm_engine *e, const void *frames, int n_frames) 
     int last = -1;
     for (int i = 0; i < n_frames;) {
         const int n = std::min({e->stage_frames, n_frames - i, batch_room(e)});
-        const int b = e->next_stage;
-        if (e->stage_used[b]) HIP_TRY(e, hipEventSynchronize(e->ev_free[b]));
-        if (direct && whole) {
-            HIP_TRY(e, hipMemcpyAsync(e->d_stage[b], src + (size_t)i * fb, roi_frame * n, hipMemcpyHostToDevice,
-                                      e->st_copy));
-        } else if (direct) {
-            const size_t pitch = (size_t)p.width * p.bpp;
-            const uint8_t *roi0 = src + (size_t)i * fb + (size_t)p.x_off * p.bpp;
-            if (rows_stack) {
-                HIP_TRY(e, hipMemcpy2DAsync(e->d_stage[b], row, roi0, pitch, row, (size_t)p.M * n,
-                                            hipMemcpyHostToDevice, e->st_copy));
-            } else {
-                for (int f = 0; f < n; ++f)
-                    HIP_TRY(e, hipMemcpy2DAsync(e->d_stage[b] + (size_t)f * roi_frame, row,
-                                                roi0 + (size_t)f * fb + (size_t)p.y_off * pitch, pitch, row, p.M,
-                                                hipMemcpyHostToDevice, e->st_copy));
-            }
-        } else {
-            if (!e->h_stage[b]) {
-                HIP_TRY(e, hipHostMalloc((void **)&e->h_stage[b], roi_frame * e->stage_frames, hipHostMallocDefault));
-                e->pinned_bytes += roi_frame * e->stage_frames;
-            }
-            // ROI-only copy into pinned memory (the reference uploads whole frames, src/DDM.cu:506)
-            uint8_t *dst = e->h_stage[b];
-            for (int f = 0; f < n; ++f) {
-                const uint8_t *fr = src + (size_t)(i + f) * fb;
-                for (int y = 0; y < p.M; ++y)
-                    std::memcpy(dst + (size_t)f * roi_frame + (size_t)y * row,
-                                fr + ((size_t)(p.y_off + y) * p.width + p.x_off) * p.bpp, row);
-            }
-            HIP_TRY(e, hipMemcpyAsync(e->d_stage[b], dst, roi_frame * n, hipMemcpyHostToDevice, e->st_copy));
-        }
-        HIP_TRY(e, hipEventRecord(e->ev_up[b], e->st_copy));
-        HIP_TRY(e, hipStreamWaitEvent(e->st, e->ev_up[b], 0));
-        int rc = process_chunk(e, e->d_stage[b], n, (long long)roi_frame, p.M, 0, 0, e->ev_free[b]);
+        int b = 0;
+        int rc = stage_upload(e, src + (size_t)i * fb, n, direct, &b);
         if (rc) return rc;
-        e->stage_used[b] = true;
-        e->next_stage = (b + 1) % NSTAGE;
+        HIP_TRY(e, hipStreamWaitEvent(e->st, e->ev_up[b], 0));
+        rc = process_chunk(e, e->d_stage[b], n, (long long)roi_frame, p.M, 0, 0, e->ev_free[b]);
+        if (rc) return rc;
         last = b;
         i += n;
     }
@@ -604,6 +622,48 @@ extern "C" int ddm_flush(ddm_engine *e) {
 
 namespace ddm {
 hipStream_t engine_stream(ddm_engine *e) { return e ? e->st : nullptr; }
+
+void engine_batch_stream(ddm_engine *e, BatchStream *bs) {
+    bs->p = &e->p;
+    bs->ring = e->d_ring;
+    bs->acc = e->d_acc;
+    bs->comp = e->d_comp;
+    bs->lags = e->d_lags;
+    bs->lag_block = e->lag_block;
+    bs->frames = e->frames;
+    bs->precision = e->precision;
+}
+
+int engine_batch_upload(ddm_engine *e, const void *frames, int n, int *b, hipEvent_t *ev_up, const uint8_t **stage) {
+    if (n < 1 || n > e->stage_frames || n > e->lag_block || n > batch_room(e))
+        return fail(e, DDM_EINVAL, "a batched chunk must fit a staging buffer, a lag block and the ring room");
+    const bool direct = pinned_host(frames);
+    int rc = stage_upload(e, (const uint8_t *)frames, n, direct, b);
+    if (rc) return rc;
+    // the caller may reuse its buffer as soon as this returns
+    if (direct) HIP_TRY(e, hipEventSynchronize(e->ev_up[*b]));
+    *ev_up = e->ev_up[*b];
+    *stage = e->d_stage[*b];
+    return DDM_OK;
+}
+
+int engine_batch_commit(ddm_engine *e, int n, int b, hipStream_t st) {
+    const long long t0 = e->frames;
+    HIP_TRY(e, hipEventRecord(e->ev_free[b], st));
+    for (size_t l = 0; l < e->p.lags.size(); ++l) {
+        const long long lo = std::max<long long>(t0, e->p.lags[l]);
+        if (t0 + n > lo) e->pairs[l] += t0 + n - lo;
+    }
+    e->frames += n;
+    e->frames_since_reset += n;
+    e->frames_lagged += n;
+    e->stats.lag_launches += 1;
+    if (e->d_comp) {
+        e->frames_since_fold += n;
+        if (e->frames_since_fold >= DDM_ACC_FOLD_PAIRS) return fold_acc(e, st);
+    }
+    return DDM_OK;
+}
 }  // namespace ddm
 
 extern "C" int ddm_sync(ddm_engine *e) {

@@ -24,6 +24,7 @@
 // group of 16 lanes touches 16 distinct 8-byte bank slots in each access.
 #include <hip/hip_runtime.h>
 
+#include "ddm_batch.h"
 #include "ddm_device.h"
 #include "ddm_kernels.h"
 
@@ -257,8 +258,8 @@ __device__ __forceinline__ void load_row_pair(const uint8_t *fr, const PixSrc &p
 // load pixels with constant offsets from two row pointers; PIX_ANY sends
 // every pixel through load_px.
 template <int S, int PIX>
-__global__ __launch_bounds__(REG_NTH) void tile_fft_reg(PixSrc ps, RingDst rd, int tps, int tiles, int groups,
-                                                       const float2 *__restrict__ twg) {
+__device__ __forceinline__ void tile_fft_reg_body(const PixSrc &ps, const RingDst &rd, int tps, int tiles,
+                                                  int groups, const float2 *__restrict__ twg, int bid, int nblk) {
     using C = RegCfg<S>;
     constexpr int H = C::H, L = C::L, P = C::P, Q = C::Q, WPT = C::WPT, TPW = C::TPW, TILE = C::TILE;
     constexpr int TE = S * (H + 1);  // spectrum elements per tile
@@ -270,10 +271,10 @@ __global__ __launch_bounds__(REG_NTH) void tile_fft_reg(PixSrc ps, RingDst rd, i
     const int w = tid >> 6;
     // frames vary fastest across the grid: concurrent workgroups write
     // neighbouring ring slots of the same element groups
-    const int nf = gridDim.x / groups;
-    const int f = blockIdx.x % nf;
+    const int nf = nblk / groups;
+    const int f = bid % nf;
     const int t = w / WPT, wit = w % WPT;
-    const int tile = (blockIdx.x / nf) * TPW + t;
+    const int tile = (bid / nf) * TPW + t;
     const bool valid = tile < tiles;
     const int q = lane % L, sq = lane / L;  // lane within the sequence, sequence within the wave
     float2 *reg = lds + w * REG_WAVE_SLOTS;  // this wave's region
@@ -386,7 +387,7 @@ __global__ __launch_bounds__(REG_NTH) void tile_fft_reg(PixSrc ps, RingDst rd, i
         }
     }
     __syncthreads();
-    const int first = (blockIdx.x / nf) * TPW;  // the workgroup's first tile
+    const int first = (bid / nf) * TPW;  // the workgroup's first tile
     const int ntile = min(TPW, tiles - first);
     int slot = rd.first_slot + f;
     slot -= (slot >= rd.ring_slots) ? rd.ring_slots : 0;
@@ -395,6 +396,12 @@ __global__ __launch_bounds__(REG_NTH) void tile_fft_reg(PixSrc ps, RingDst rd, i
         const long long e = rd.elem_off + (long long)first * TE + i;  // the workgroup's tiles are consecutive
         rd.ring[(e >> RING_GROUP_LOG2) * gstride + (long long)slot * RING_GROUP + (e & (RING_GROUP - 1))] = lds[i];
     }
+}
+
+template <int S, int PIX>
+__global__ __launch_bounds__(REG_NTH) void tile_fft_reg(PixSrc ps, RingDst rd, int tps, int tiles, int groups,
+                                                       const float2 *__restrict__ twg) {
+    tile_fft_reg_body<S, PIX>(ps, rd, tps, tiles, groups, twg, blockIdx.x, gridDim.x);
 }
 
 template <int S>
@@ -480,14 +487,15 @@ __device__ __forceinline__ void load_row(const uint8_t *fr, const PixSrc &ps, in
 }
 
 template <int S, int PIX, bool A4>
-__global__ __launch_bounds__(SMALL_NTH) void tile_fft_lane(PixSrc ps, RingDst rd, int tps, int tiles, int groups) {
+__device__ __forceinline__ void tile_fft_lane_body(const PixSrc &ps, const RingDst &rd, int tps, int tiles,
+                                                   int groups, int bid, int nblk) {
     constexpr int R = S / 2, TPG = SMALL_NTH / R;  // lanes per tile, tiles per workgroup
     constexpr int LS = R + 1;                      // LDS row stride (complex values)
     __shared__ float2 lds[TPG * S * LS];
     const int tid = threadIdx.x, lt = tid / R, r = tid % R;
-    const int nf = gridDim.x / groups;  // frames vary fastest
-    const int f = blockIdx.x % nf;
-    const int tile = (blockIdx.x / nf) * TPG + lt;
+    const int nf = nblk / groups;  // frames vary fastest
+    const int f = bid % nf;
+    const int tile = (bid / nf) * TPG + lt;
     const bool live = tile < tiles;
     float2 *blk = lds + lt * S * LS;
     if (live) {
@@ -537,7 +545,7 @@ __global__ __launch_bounds__(SMALL_NTH) void tile_fft_lane(PixSrc ps, RingDst rd
     }
     __syncthreads();
     constexpr int TE = S * LS;  // spectrum elements per tile
-    const int first = (blockIdx.x / nf) * TPG;
+    const int first = (bid / nf) * TPG;
     const int ntile = min(TPG, tiles - first);
     int slot = rd.first_slot + f;
     slot -= (slot >= rd.ring_slots) ? rd.ring_slots : 0;
@@ -546,6 +554,11 @@ __global__ __launch_bounds__(SMALL_NTH) void tile_fft_lane(PixSrc ps, RingDst rd
         const long long e = rd.elem_off + (long long)first * TE + i;  // the workgroup's tiles are consecutive
         rd.ring[(e >> RING_GROUP_LOG2) * gstride + (long long)slot * RING_GROUP + (e & (RING_GROUP - 1))] = lds[i];
     }
+}
+
+template <int S, int PIX, bool A4>
+__global__ __launch_bounds__(SMALL_NTH) void tile_fft_lane(PixSrc ps, RingDst rd, int tps, int tiles, int groups) {
+    tile_fft_lane_body<S, PIX, A4>(ps, rd, tps, tiles, groups, blockIdx.x, gridDim.x);
 }
 
 template <int S>
@@ -639,17 +652,16 @@ __device__ __forceinline__ void big_fft(float2 (&v)[BigCfg<S>::E1], float2 *seqb
 // so every 16 lanes of a row piece write 128 aligned bytes and the column
 // pass reads each block as one contiguous run.
 template <int S, int PIX>
-__global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg(PixSrc ps, int tps, int tiles,
-                                                                   float2 *__restrict__ xt,
-                                                                   const float2 *__restrict__ twg) {
+__device__ __forceinline__ void tile_fft_rows_reg_body(const PixSrc &ps, int tps, int tiles, float2 *__restrict__ xt,
+                                                       const float2 *__restrict__ twg, int bid, int nblk) {
     using C = BigCfg<S>;
     constexpr int H = C::H, L = C::L, E1 = C::E1, SEQ = C::SEQ, GROUPS = H / SEQ;
     __shared__ float2 lds[SEQ * S];
     const int tid = threadIdx.x, q = tid & (L - 1), sq = tid / L;
-    const int nf = gridDim.x / (GROUPS * tiles);  // frames vary fastest
-    const int f = blockIdx.x % nf;
-    const int g = (blockIdx.x / nf) % GROUPS;
-    const int tile = blockIdx.x / (nf * GROUPS);
+    const int nf = nblk / (GROUPS * tiles);  // frames vary fastest
+    const int f = bid % nf;
+    const int g = (bid / nf) % GROUPS;
+    const int tile = bid / (nf * GROUPS);
     const int r = g * SEQ + sq;  // row pair within the tile
     const uint8_t *fr = ps.frames + (long long)f * ps.frame_bytes;
     const int x0 = ps.x_off + (tile % tps) * S + q;
@@ -679,13 +691,19 @@ __global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg(PixSrc ps, i
     }
 }
 
+template <int S, int PIX>
+__global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg(PixSrc ps, int tps, int tiles,
+                                                                   float2 *__restrict__ xt,
+                                                                   const float2 *__restrict__ twg) {
+    tile_fft_rows_reg_body<S, PIX>(ps, tps, tiles, xt, twg, blockIdx.x, gridDim.x);
+}
+
 // Columns: SEQ consecutive packed columns per workgroup, read from the pass
 // buffer's column block and written to the ring slot; the packed column is
 // separated into the DC (kx = 0) and Nyquist (kx = H) columns on the way out.
 template <int S>
-__global__ __launch_bounds__(BigCfg<S>::NTH_C) void tile_fft_cols_reg(RingDst rd, int tiles,
-                                                                   const float2 *__restrict__ xt,
-                                                                   const float2 *__restrict__ twg) {
+__device__ __forceinline__ void tile_fft_cols_reg_body(const RingDst &rd, int tiles, const float2 *__restrict__ xt,
+                                                       const float2 *__restrict__ twg, int bid, int nblk) {
     using C = BigCfg<S>;
     constexpr int H = C::H, L = C::L, E1 = C::E1, SEQ = C::SEQ_C, NB = H / SEQ, NTH = C::NTH_C;
     constexpr int RP = NTH / SEQ;  // rows per pass of the staging loops (= L)
@@ -694,10 +712,10 @@ __global__ __launch_bounds__(BigCfg<S>::NTH_C) void tile_fft_cols_reg(RingDst rd
     // spectra, S + 1 apart so that reading one row across them is conflict free
     __shared__ float2 lds[SEQ * (S + 1)];
     const int tid = threadIdx.x;
-    const int nf = gridDim.x / (NB * tiles);  // frames vary fastest
-    const int f = blockIdx.x % nf;
-    const int cb = (blockIdx.x / nf) % NB;
-    const int tile = blockIdx.x / (nf * NB);
+    const int nf = nblk / (NB * tiles);  // frames vary fastest
+    const int f = bid % nf;
+    const int cb = (bid / nf) % NB;
+    const int tile = bid / (nf * NB);
     const int c0 = cb * SEQ;
     const TileOut t = tile_out(rd, f, tile, (long long)S * (H + 1));
     const int col = tid % SEQ, row = tid / SEQ;  // staging: SEQ consecutive columns of RP rows per pass
@@ -743,6 +761,13 @@ __global__ __launch_bounds__(BigCfg<S>::NTH_C) void tile_fft_cols_reg(RingDst rd
 }
 
 template <int S>
+__global__ __launch_bounds__(BigCfg<S>::NTH_C) void tile_fft_cols_reg(RingDst rd, int tiles,
+                                                                   const float2 *__restrict__ xt,
+                                                                   const float2 *__restrict__ twg) {
+    tile_fft_cols_reg_body<S>(rd, tiles, xt, twg, blockIdx.x, gridDim.x);
+}
+
+template <int S>
 static void launch_big(const PixSrc &ps, const RingDst &rd, int tps, int tiles, int n_frames, const float2 *tw,
                        float2 *xt, hipStream_t st) {
     using C = BigCfg<S>;
@@ -758,6 +783,144 @@ static void launch_big(const PixSrc &ps, const RingDst &rd, int tps, int tiles, 
     }
     const dim3 g2((unsigned)((C::H / C::SEQ_C) * tiles * n_frames)), block2(C::NTH_C);
     hipLaunchKernelGGL(tile_fft_cols_reg<S>, g2, block2, 0, st, rd, tiles, xt, tw);
+}
+
+// ---------------------------------------------------------------------------
+// Batched forms: one launch per kernel kind over a job table of many
+// streams' chunks, a workgroup per queue item (the batch's blocks are the
+// concatenation of the jobs' own grids).  Each block runs the body of its
+// job's own launch with the job's block index, so a chunk's spectra are the
+// same bit for bit whether it is transformed alone or in a batch.  (A
+// persistent loop over the items was measured out: the loop kept the
+// bodies' loop-invariant values live across it, 15-90 % more registers.)
+// ---------------------------------------------------------------------------
+template <int S, int PIX>
+__global__ __launch_bounds__(REG_NTH) void tile_fft_reg_batch(const BatchHdr *__restrict__ hdr,
+                                                             const FftJob *__restrict__ jobs,
+                                                             const float2 *__restrict__ twg) {
+    const long long b = blockIdx.x;
+    const FftJob &j = jobs[batch_job(jobs, hdr->njobs, b)];
+    tile_fft_reg_body<S, PIX>(j.ps, j.rd, j.tps, j.tiles, j.groups, twg, (int)(b - j.first), j.nblk);
+}
+
+template <int S, int PIX>
+__global__ __launch_bounds__(SMALL_NTH) void tile_fft_lane_batch(const BatchHdr *__restrict__ hdr,
+                                                                const FftJob *__restrict__ jobs) {
+    const long long b = blockIdx.x;
+    const FftJob &j = jobs[batch_job(jobs, hdr->njobs, b)];
+    tile_fft_lane_body<S, PIX, true>(j.ps, j.rd, j.tps, j.tiles, j.groups, (int)(b - j.first), j.nblk);
+}
+
+template <int S, int PIX>
+__global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg_batch(const BatchHdr *__restrict__ hdr,
+                                                                         const FftJob *__restrict__ jobs,
+                                                                         const float2 *__restrict__ twg) {
+    const long long b = blockIdx.x;
+    const FftJob &j = jobs[batch_job(jobs, hdr->njobs, b)];
+    tile_fft_rows_reg_body<S, PIX>(j.ps, j.tps, j.tiles, j.xt, twg, (int)(b - j.first), j.nblk);
+}
+
+template <int S>
+__global__ __launch_bounds__(BigCfg<S>::NTH_C) void tile_fft_cols_reg_batch(const BatchHdr *__restrict__ hdr,
+                                                                           const FftJob *__restrict__ jobs,
+                                                                           const float2 *__restrict__ twg) {
+    const long long b = blockIdx.x;
+    const FftJob &j = jobs[batch_job(jobs, hdr->njobs, b)];
+    tile_fft_cols_reg_body<S>(j.rd, j.tiles, j.xt, twg, (int)(b - j.first), j.nblk);
+}
+
+bool fft_batch_jobs(const PixSrc &ps, const RingDst &rd, int scale, int tps, int tiles, int n_frames, float2 *xt,
+                    std::vector<std::pair<FftKey, FftJob>> *out) {
+    const int pix = pixel_mode(ps);
+    if (pix == PIX_ANY) return false;
+    FftJob j{};
+    j.ps = ps;
+    j.rd = rd;
+    j.xt = xt;
+    j.tps = tps;
+    j.tiles = tiles;
+    switch (scale) {
+    case 8:
+    case 16:
+    case 32: {
+        // the batched lane kernel reads 4-byte aligned rows
+        const bool a4 = ((ps.row_bytes | ps.frame_bytes | (long long)(uintptr_t)ps.frames |
+                          (long long)ps.x_off * ps.bpp) & 3) == 0;
+        if (!a4) return false;
+        const int tpg = SMALL_NTH / (scale / 2);
+        j.groups = (tiles + tpg - 1) / tpg;
+        j.nblk = j.groups * n_frames;
+        out->push_back({FftKey{FFT_LANE, scale, pix}, j});
+        return true;
+    }
+    case 64:
+    case 128: {
+        const int tpw = scale == 64 ? RegCfg<64>::TPW : RegCfg<128>::TPW;
+        j.groups = (tiles + tpw - 1) / tpw;
+        j.nblk = j.groups * n_frames;
+        out->push_back({FftKey{FFT_REG, scale, pix}, j});
+        return true;
+    }
+    case 256:
+    case 512:
+    case 1024: {
+        if (!xt) return false;
+        const int h = scale / 2, seq = scale >= 1024 ? BigCfg<1024>::SEQ : BigCfg<256>::SEQ;
+        j.groups = 0;
+        j.nblk = (h / seq) * tiles * n_frames;
+        out->push_back({FftKey{FFT_ROWS, scale, pix}, j});
+        j.nblk = (h / BigCfg<256>::SEQ_C) * tiles * n_frames;
+        out->push_back({FftKey{FFT_COLS, scale, 0}, j});
+        return true;
+    }
+    default: return false;
+    }
+}
+
+template <int S>
+static void launch_fft_batch_s(const FftKey &k, const BatchHdr *hdr, const FftJob *jobs, long long blocks,
+                               const float2 *tw, hipStream_t st) {
+    const dim3 grid((unsigned)blocks);
+    if (k.kind == FFT_COLS) {
+        if constexpr (S >= 256)
+            hipLaunchKernelGGL(tile_fft_cols_reg_batch<S>, grid, dim3(BigCfg<S>::NTH_C), 0, st, hdr, jobs, tw);
+        return;
+    }
+    if constexpr (S <= 32) {
+        if (k.pix == PIX_U8_1)
+            hipLaunchKernelGGL((tile_fft_lane_batch<S, PIX_U8_1>), grid, dim3(SMALL_NTH), 0, st, hdr, jobs);
+        else
+            hipLaunchKernelGGL((tile_fft_lane_batch<S, PIX_U16LE_2>), grid, dim3(SMALL_NTH), 0, st, hdr, jobs);
+    } else if constexpr (S <= 128) {
+        if (k.pix == PIX_U8_1)
+            hipLaunchKernelGGL((tile_fft_reg_batch<S, PIX_U8_1>), grid, dim3(REG_NTH), 0, st, hdr, jobs, tw);
+        else
+            hipLaunchKernelGGL((tile_fft_reg_batch<S, PIX_U16LE_2>), grid, dim3(REG_NTH), 0, st, hdr, jobs, tw);
+    } else {
+        if (k.pix == PIX_U8_1)
+            hipLaunchKernelGGL((tile_fft_rows_reg_batch<S, PIX_U8_1>), grid, dim3(BigCfg<S>::NTH), 0, st, hdr, jobs,
+                               tw);
+        else
+            hipLaunchKernelGGL((tile_fft_rows_reg_batch<S, PIX_U16LE_2>), grid, dim3(BigCfg<S>::NTH), 0, st, hdr,
+                               jobs, tw);
+    }
+}
+
+hipError_t launch_fft_batch(const FftKey &k, const BatchHdr *hdr, const FftJob *jobs, long long blocks,
+                            const float2 *tw, hipStream_t st) {
+    if (blocks < 1 || blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    switch (k.scale) {
+    case 8: launch_fft_batch_s<8>(k, hdr, jobs, blocks, tw, st); break;
+    case 16: launch_fft_batch_s<16>(k, hdr, jobs, blocks, tw, st); break;
+    case 32: launch_fft_batch_s<32>(k, hdr, jobs, blocks, tw, st); break;
+    case 64: launch_fft_batch_s<64>(k, hdr, jobs, blocks, tw, st); break;
+    case 128: launch_fft_batch_s<128>(k, hdr, jobs, blocks, tw, st); break;
+    case 256: launch_fft_batch_s<256>(k, hdr, jobs, blocks, tw, st); break;
+    case 512: launch_fft_batch_s<512>(k, hdr, jobs, blocks, tw, st); break;
+    case 1024: launch_fft_batch_s<1024>(k, hdr, jobs, blocks, tw, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 bool launch_tile_fft_reg(const PixSrc &ps, const RingDst &rd, int scale, int tps, int tiles, int n_frames,

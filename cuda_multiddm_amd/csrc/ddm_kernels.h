@@ -88,6 +88,30 @@ hipError_t twiddle_table(const float2 **out);
 
 // The HIP stream an engine launches its kernels on (ddm_engine.hip).
 hipStream_t engine_stream(ddm_engine *e);
+
+struct Params;
+// What the batched server (ddm_server.hip) uses of an engine: its buffers,
+// geometry and the index of its next frame.
+struct BatchStream {
+    const Params *p;
+    float *ring;
+    float *acc;
+    float *comp;       // nullptr: plain accumulation
+    const int *lags;   // device
+    int lag_block;
+    long long frames;  // index of the next frame
+    int precision;
+};
+void engine_batch_stream(ddm_engine *e, BatchStream *bs);
+// Uploads n host frames (n <= chunk, <= lag block) into the engine's next
+// staging buffer on its copy stream; *b is the buffer, *ev_up its upload
+// event, *stage its device address.  Pinned frames are waited for.
+int engine_batch_upload(ddm_engine *e, const void *frames, int n, int *b, hipEvent_t *ev_up, const uint8_t **stage);
+// The n frames of staging buffer b were transformed into the ring and
+// through the lag engine by work queued on st: the buffer is free once st
+// gets past this point, counts advance, and the accumulators are folded into
+// FP64 (on st) when due.
+int engine_batch_commit(ddm_engine *e, int n, int b, hipStream_t st);
 hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st);
 hipError_t launch_lag(const LagArgs &a, hipStream_t st);
 hipError_t launch_azimuthal(const AzArgs &a, hipStream_t st);

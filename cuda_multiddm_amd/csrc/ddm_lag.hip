@@ -21,13 +21,12 @@
 #include <algorithm>
 #include <vector>
 
+#include "ddm_batch.h"
 #include "ddm_internal.h"
 #include "ddm_kernels.h"
 
 namespace ddm {
 
-constexpr int LAG_NT = 1024;                    // threads per workgroup: 16 waves
-constexpr int LAG_WAVES = LAG_NT / RING_GROUP;  // waves sharing one ring group
 constexpr int DENSE_SEG = 32;                   // frames per segment of a lag's units
 // Word 3 of a raw (untyped, stride 0) buffer resource on gfx9-family parts:
 // a valid 32-bit data format, so dword loads return memory.
@@ -65,21 +64,6 @@ __device__ __forceinline__ void lds_barrier() {
 // path, for the split of the waves.
 constexpr int COST_DENSE = 4, COST_HIST = 8;
 
-// Work split of one lag launch, the same for every workgroup, computed on the
-// host from the lag list, the block size n and the history depth dmax.
-constexpr int PLAN_MAX_BLOCKS = 252;
-struct LagPlan {
-    int dense_lags;  // leading consecutive lags on the register window
-    int wd;          // window waves: chunk w / sd, segment w % sd
-    int sd;          // segments per window chunk
-    int ss;          // units per other lag; 0: one wave per lag, direct accumulation
-    int hist;        // the other lags go history-major
-    int pmin, pmax;  // history-major frame range
-    int listed;      // blk_start / blk_list hold each history wave's blocks
-    int all_hist;    // the lists cover all LAG_WAVES waves (window waves take blocks after their window)
-    unsigned char blk_start[LAG_WAVES + 1];   // wave k: blk_list[blk_start[k] .. blk_start[k + 1])
-    unsigned char blk_list[PLAN_MAX_BLOCKS];  // block b: frames pmin + b * HIST_BLOCK ...
-};
 // Host estimate of a history block's cost in VALU-instruction units: a block
 // costs its load (a full memory latency, the same however few lags reach
 // it), plus each lag that reaches it and each pair.
@@ -370,9 +354,9 @@ __device__ __forceinline__ void hist_block(const float2 *nl, const float2 (&fp)[
 // and paired with the LDS rows of every lag that reaches it, so a history
 // frame is requested once per wave rather than once per lag.  s[l] collects
 // lag l in a fixed block order.
-template <int B, int LSM>
+template <int B, int LSM, class PL>
 __device__ __forceinline__ void hist_sums(const GroupRing &g, const float2 *nl, int R, int slot0, int n, int vtau,
-                                          int ls, const LagPlan &pl, int k, int ws, float (&s)[LSM]) {
+                                          int ls, const PL &pl, int k, int ws, float (&s)[LSM]) {
     const bool live = (int)(threadIdx.x % RING_GROUP) < ls;
     float2 fa[B];
     if (pl.listed) {
@@ -524,10 +508,12 @@ __device__ __forceinline__ void hist_reduce(float2 *nb, AccRef<COMP> accg, long 
 //    wave that adds its sum to the accumulator directly.
 // Partial sums meet in LDS after the pair work and are added per lag in a
 // fixed order, so results do not depend on timing.
-template <int J, bool COMP>
-__global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J>::WAVES_PER_EU))) void lag_kernel(
-    const float2 *__restrict__ ring, float *__restrict__ acc, float *__restrict__ comp, long long E, int R,
-    int slot0, int n, int dmax, const int *__restrict__ lags, int L, const LagPlan pl) {
+// TAG makes each kernel's instantiation, and so its LDS stage, its own (an
+// LDS array shared by two kernels is addressed less directly in both).
+template <int J, bool COMP, int TAG, class PL>
+__device__ __forceinline__ void lag_body(const float2 *__restrict__ ring, float *__restrict__ acc,
+                                         float *__restrict__ comp, long long E, int R, int slot0, int n, int dmax,
+                                         const int *__restrict__ lags, int L, const PL &pl, int group) {
     using C = LagCfg<J>;
     constexpr int DENSE = C::DENSE, U = C::UNROLL;
     constexpr int SLOTS = (J / 8) < C::SLOTS_MAX ? (J / 8) : C::SLOTS_MAX;  // partials per wave that fit in LDS
@@ -537,7 +523,7 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
     const int lane = tid % RING_GROUP;
     const int w = __builtin_amdgcn_readfirstlane(tid / RING_GROUP);
     GroupRing g;
-    g.rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<float2 *>(ring + (long long)blockIdx.x * R * RING_GROUP), 0,
+    g.rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<float2 *>(ring + (long long)group * R * RING_GROUP), 0,
                                              R * (int)(RING_GROUP * sizeof(float2)), BUFFER_RSRC_WORD3);
     g.voff = lane * (int)sizeof(float2);
     {
@@ -567,7 +553,7 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
     const int ws = LAG_WAVES - wd;
     // the accumulators this wave will add to are requested beside the staging
     // loads, so they have arrived when the partial sums are added
-    const long long lane_off = (long long)blockIdx.x * RING_GROUP + lane;
+    const long long lane_off = (long long)group * RING_GROUP + lane;
     AccRef<COMP> accg{acc + lane_off, COMP ? comp + lane_off : nullptr};
     const int nred = hist ? L : (ss ? L : dense_lags);
     AccPre<ACC_PREFETCH, COMP> pre;
@@ -658,6 +644,32 @@ __global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J
         }
         add_acc(accg, E, l, (l - w) / LAG_WAVES, t, pre);
     }
+}
+
+template <int J, bool COMP>
+__global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J>::WAVES_PER_EU))) void lag_kernel(
+    const float2 *__restrict__ ring, float *__restrict__ acc, float *__restrict__ comp, long long E, int R,
+    int slot0, int n, int dmax, const int *__restrict__ lags, int L, const LagPlan pl) {
+    lag_body<J, COMP, 0>(ring, acc, comp, E, R, slot0, n, dmax, lags, L, pl, blockIdx.x);
+}
+
+// One pass over many streams: a workgroup per (stream, ring group) item of
+// the batch (job = stream, group within its spectrum).
+template <int J, bool COMP>
+__global__ __launch_bounds__(LAG_NT) __attribute__((amdgpu_waves_per_eu(LagCfg<J>::WAVES_PER_EU))) void
+lag_kernel_batch(const BatchHdr *__restrict__ hdr, const LagJob *__restrict__ jobs,
+                 const LagPlan *__restrict__ plans) {
+    // the tables are read-only for the whole launch: read them through the
+    // constant address space, so that their loads stay scalar beside the
+    // accumulator stores (which the compiler cannot otherwise prove apart)
+    typedef const __attribute__((address_space(4))) LagJob CJob;
+    typedef const __attribute__((address_space(4))) LagPlan CPlan;
+    CJob *cj = (CJob *)jobs;
+    const long long b = blockIdx.x;
+    CJob &jb = cj[batch_job(cj, hdr->njobs, b)];
+    CPlan &pl = ((CPlan *)plans)[jb.plan];
+    lag_body<J, COMP, 1>(jb.ring, jb.acc, jb.comp, jb.E, jb.R, jb.slot0, jb.n, jb.dmax, jb.lags, jb.L, pl,
+                         (int)(b - jb.first));
 }
 
 // The launch plan (same rules for every workgroup):
@@ -773,6 +785,41 @@ static void launch_lag_t(const LagArgs &a, hipStream_t st) {
         hipLaunchKernelGGL((lag_kernel<J, false>), dim3(grid), dim3(LAG_NT), 0, st, (const float2 *)a.ring, a.acc,
                            nullptr, a.elems_per_frame, a.ring_slots, a.slot0, a.n_new, a.dmax, a.lags, a.lag_count,
                            pl);
+}
+
+void make_lag_plan(int J, const int *lags, int L, int n, int dmax, LagPlan *pl) {
+    switch (J) {
+    case 32: make_plan<32>(lags, L, n, dmax, *pl); break;
+    case 64: make_plan<64>(lags, L, n, dmax, *pl); break;
+    case 128: make_plan<128>(lags, L, n, dmax, *pl); break;
+    default: make_plan<256>(lags, L, n, dmax, *pl); break;
+    }
+}
+
+template <int J>
+static hipError_t launch_lag_batch_t(bool comp, const BatchHdr *hdr, const LagJob *jobs, const LagPlan *plans,
+                                     long long groups, hipStream_t st) {
+    const dim3 grid((unsigned)groups);
+    if (comp)
+        hipLaunchKernelGGL((lag_kernel_batch<J, true>), grid, dim3(LAG_NT), 0, st, hdr, jobs, plans);
+    else
+        hipLaunchKernelGGL((lag_kernel_batch<J, false>), grid, dim3(LAG_NT), 0, st, hdr, jobs, plans);
+    return hipGetLastError();
+}
+
+// Blocks of 32 .. 128 frames only: serving flushes every chunk through the
+// lag engine, so its blocks are small; instantiating the 256-frame batch
+// kernel as well made the compiler spill more in the 256-frame single
+// kernel of the whole-video path.
+hipError_t launch_lag_batch(int J, bool comp, const BatchHdr *hdr, const LagJob *jobs, const LagPlan *plans,
+                            long long groups, hipStream_t st) {
+    if (groups < 1 || groups > 0x7fffffffLL) return hipErrorInvalidValue;
+    switch (J) {
+    case 32: return launch_lag_batch_t<32>(comp, hdr, jobs, plans, groups, st);
+    case 64: return launch_lag_batch_t<64>(comp, hdr, jobs, plans, groups, st);
+    case 128: return launch_lag_batch_t<128>(comp, hdr, jobs, plans, groups, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_lag(const LagArgs &a, hipStream_t st) {

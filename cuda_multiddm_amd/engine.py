@@ -29,6 +29,7 @@ PIX_U8, PIX_U16LE, PIX_U16BE = 0, 1, 2
 NORM_FRAMES, NORM_PAIRS = 0, 1
 PREC_FP32, PREC_BF16X2, PREC_FP16X2 = 0, 1, 2
 ACC_COMPENSATED, ACC_PLAIN = 0, 1
+SERVER_BATCHED, SERVER_NO_GRAPH = 1, 2
 
 # Every symbol include/ddm_amd.h declares.
 EXPORTED = [
@@ -40,7 +41,7 @@ EXPORTED = [
     "ddm_synth_uniform", "ddm_synth_brownian", "ddm_write_iqt_file", "ddm_flush",
     "ddm_server_create", "ddm_server_destroy", "ddm_server_open", "ddm_server_submit", "ddm_server_results",
     "ddm_server_engine", "ddm_server_drain", "ddm_server_close", "ddm_server_latencies", "ddm_server_stats",
-    "ddm_server_last_error", "ddm_fit_results", "ddm_k_fit", "ddm_set_precision", "ddm_autotune", "ddm_reserve",
+    "ddm_server_last_error", "ddm_server_create2", "ddm_fit_results", "ddm_k_fit", "ddm_set_precision", "ddm_autotune", "ddm_reserve",
 ]
 
 
@@ -85,7 +86,8 @@ class ServerInfo(ctypes.Structure):
     _fields_ = [
         ("budget_bytes", ctypes.c_size_t), ("used_bytes", ctypes.c_size_t), ("streams", ctypes.c_int),
         ("max_streams", ctypes.c_int), ("frames", ctypes.c_longlong), ("chunks", ctypes.c_longlong),
-        ("chunks_in_flight", ctypes.c_longlong),
+        ("chunks_in_flight", ctypes.c_longlong), ("ticks", ctypes.c_longlong), ("graph_ticks", ctypes.c_longlong),
+        ("tick_items", ctypes.c_longlong),
     ]
 
 
@@ -144,6 +146,7 @@ def lib():
     L.ddm_write_iqt_file.argtypes = [ctypes.c_char_p, dp, fp, ctypes.c_int, ip, ctypes.c_int, ctypes.c_float]
     L.ddm_flush.argtypes = [vp]
     L.ddm_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(vp)]
+    L.ddm_server_create2.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
     L.ddm_server_destroy.argtypes = [vp]
     L.ddm_server_destroy.restype = None
     L.ddm_server_open.argtypes = [vp, ctypes.POINTER(Desc), ctypes.POINTER(ctypes.c_int)]
@@ -341,9 +344,12 @@ class Server:
     footprint, chunks of different streams overlapping on the GPU, and
     chunk-to-result latencies."""
 
-    def __init__(self, device_budget: int = 0, max_streams: int = 256):
+    def __init__(self, device_budget: int = 0, max_streams: int = 256, batched: bool = False, graph: bool = True):
+        """batched: chunks of all streams run as ticks of batched launches
+        (DDM_SERVER_BATCHED), as HIP graphs unless graph is False."""
         h = ctypes.c_void_p()
-        rc = lib().ddm_server_create(device_budget, max_streams, ctypes.byref(h))
+        flags = (SERVER_BATCHED if batched else 0) | (0 if graph else SERVER_NO_GRAPH)
+        rc = lib().ddm_server_create2(device_budget, max_streams, flags, ctypes.byref(h))
         if rc != DDM_OK:
             raise DDMError(rc, lib().ddm_server_last_error(None).decode())
         self._h = h

@@ -248,10 +248,30 @@ typedef struct ddm_server_info {
     size_t budget_bytes, used_bytes;
     int streams, max_streams;
     long long frames, chunks, chunks_in_flight;
+    long long ticks;        /* batched: ticks run */
+    long long graph_ticks;  /* batched: ticks run as a HIP graph */
+    long long tick_items;   /* batched: queue items (workgroups) of all ticks */
 } ddm_server_info;
 
-/* device_budget 0 = 90 % of the device memory free at creation. */
+/* Flags of ddm_server_create2.
+ * DDM_SERVER_BATCHED: submissions only upload and queue their chunk; a
+ *   scheduler thread runs the waiting chunks of all streams (the oldest of
+ *   each) as one tick: every tile-transform kind (per tile size) and every
+ *   lag-block size is one launch over the tick's work queue of (stream,
+ *   scale, tile block, frame) and (stream, ring group) items, the tick's
+ *   launches captured as a HIP graph.  Chunks arriving while a tick runs go
+ *   into the next (at most two ticks in flight); each stream keeps at most
+ *   three chunks waiting (further submissions block).  A stream's chunks are
+ *   submitted in pieces of at most min(chunk, lag block) frames.  Results
+ *   are bit-identical to the per-stream mode.
+ * DDM_SERVER_NO_GRAPH: batched ticks are launched directly, not as graphs. */
+#define DDM_SERVER_BATCHED 1
+#define DDM_SERVER_NO_GRAPH 2
+
+/* device_budget 0 = 90 % of the device memory free at creation (a batched
+ * server takes 1 GiB of it for the tick pass buffer). */
 int ddm_server_create(size_t device_budget, int max_streams, ddm_server **out);
+int ddm_server_create2(size_t device_budget, int max_streams, int flags, ddm_server **out);
 void ddm_server_destroy(ddm_server *srv);
 /* DDM_EBUSY when no slot is free or the footprint does not fit yet. */
 int ddm_server_open(ddm_server *srv, const ddm_desc *desc, int *stream_id);
