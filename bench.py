@@ -123,7 +123,7 @@ def serving(args):
         pinned[:] = f.reshape(-1)
         pool[size] = pinned
 
-    srv = ddm.Server(max_streams=args.streams, **({"batched": True} if args.batched else {}))
+    srv = ddm.Server(max_streams=args.streams, batched=args.batched, group=args.group, hold_ms=args.hold_ms)
     waiting = list(range(args.streams))
     active, sent = {}, {}
     admitted, refused = 0, 0
@@ -198,6 +198,7 @@ def serving(args):
         "admission_refusals": refused,
         "active_streams_at_end": st.streams,
         "active_sizes": {str(s): sizes.count(s) for s in (256, 512, 1024)},
+        "ticks": int(st.ticks), "graph_ticks": int(st.graph_ticks),
         "device_budget_GB": round(st.budget_bytes / 1e9, 1),
         "device_used_GB_at_end": round(st.used_bytes / 1e9, 1),
         "host_seconds": {k: round(v, 2) for k, v in host_s.items()},
@@ -206,7 +207,8 @@ def serving(args):
                                f"{args.serving_lag_block}, Poisson arrivals {args.rate} chunks/s, "
                                + ("streams resident" if not args.chunks_per_stream else
                                   f"streams replaced after {args.chunks_per_stream} chunks")
-                               + (", batched launches" if args.batched else ", per-stream launches")},
+                               + (", batched launches" if args.batched else ", per-stream launches")
+                               + (f", chunks grouped per lag pass (hold {args.hold_ms} ms)" if args.group else "")},
     }
     srv.close()
     print(json.dumps(out))
@@ -341,6 +343,11 @@ def main():
                     help="with --profile-steps: finish with a device copy of this many MiB")
     ap.add_argument("--serving", action="store_true", help="run the multi-stream serving workload instead")
     ap.add_argument("--batched", action="store_true", help="serving: batched launches across streams")
+    ap.add_argument("--group", action="store_true",
+                    help="serving, batched: a tick runs all of a stream's waiting chunks (up to its lag block) as "
+                         "one lag pass")
+    ap.add_argument("--hold-ms", type=float, default=0.0,
+                    help="serving, grouped: how long a stream's chunk may wait for more of its chunks")
     ap.add_argument("--streams", type=int, default=64)
     ap.add_argument("--rate", type=float, default=200.0, help="serving: offered chunks per second")
     ap.add_argument("--seconds", type=float, default=20.0, help="serving: arrival window")

@@ -38,7 +38,7 @@ struct Mark {
 };
 
 using ddm::default_fft_batch;  // ddm_autotune may choose another multiple of the chunk
-constexpr int NSTAGE = ddm::UPLOAD_STAGES;
+constexpr int NSTAGE = ddm::MAX_UPLOAD_STAGES;  // capacity; an engine uses nstage of them
 
 }  // namespace
 
@@ -72,6 +72,7 @@ struct ddm_engine {
     uint8_t *h_stage[NSTAGE] = {};
     bool stage_used[NSTAGE] = {};
     int next_stage = 0;
+    int nstage = ddm::UPLOAD_STAGES;  // staging buffers in use
     int stage_frames = 0;
     hipStream_t st = nullptr, st_copy = nullptr;
     hipEvent_t ev_up[NSTAGE] = {}, ev_free[NSTAGE] = {};
@@ -260,7 +261,7 @@ extern "C" int ddm_create(const ddm_desc *desc, ddm_engine **out) {
     TRY(dev_alloc(e, &e->d_out, (size_t)p.result_count, "results"));
     const size_t roi_frame = (size_t)p.M * p.M * p.bpp;
     // pinned host staging is allocated on the first host submission that needs it
-    for (int i = 0; i < NSTAGE; ++i) TRY(dev_alloc(e, &e->d_stage[i], roi_frame * p.chunk, "upload staging"));
+    for (int i = 0; i < e->nstage; ++i) TRY(dev_alloc(e, &e->d_stage[i], roi_frame * p.chunk, "upload staging"));
     e->stage_frames = p.chunk;
     std::vector<ddm::AzScale> az;
     long long out_base = 0;
@@ -489,7 +490,7 @@ extern "C" int ddm_submit_device(ddm_engine *e, const void *frames, int n_frames
 // are released once the work that reads them is done.
 static int grow_stage(ddm_engine *e, int frames) {
     const size_t roi_frame = (size_t)e->p.M * e->p.M * e->p.bpp;
-    for (int b = 0; b < NSTAGE; ++b) {
+    for (int b = 0; b < e->nstage; ++b) {
         if (e->stage_used[b]) HIP_TRY(e, hipEventSynchronize(e->ev_free[b]));
         if (e->d_stage[b]) {
             HIP_TRY(e, hipFreeAsync(e->d_stage[b], e->st));
@@ -502,7 +503,7 @@ static int grow_stage(ddm_engine *e, int frames) {
             e->h_stage[b] = nullptr;
         }
     }
-    for (int b = 0; b < NSTAGE; ++b) {
+    for (int b = 0; b < e->nstage; ++b) {
         int rc = dev_alloc(e, &e->d_stage[b], roi_frame * frames, "upload staging");
         if (rc) return rc;
     }
@@ -577,7 +578,7 @@ static int stage_upload(ddm_engine *e, const uint8_t *src, int n, bool direct, i
     }
     HIP_TRY(e, hipEventRecord(e->ev_up[b], e->st_copy));
     e->stage_used[b] = true;
-    e->next_stage = (b + 1) % NSTAGE;
+    e->next_stage = (b + 1) % e->nstage;
     *b_out = b;
     return DDM_OK;
 }
@@ -644,6 +645,18 @@ int engine_batch_upload(ddm_engine *e, const void *frames, int n, int *b, hipEve
     if (direct) HIP_TRY(e, hipEventSynchronize(e->ev_up[*b]));
     *ev_up = e->ev_up[*b];
     *stage = e->d_stage[*b];
+    return DDM_OK;
+}
+
+int engine_batch_stages(ddm_engine *e, int k) {
+    if (e->frames != 0 || k < e->nstage || k > NSTAGE) return fail(e, DDM_ESTATE, "staging buffers are fixed once frames arrived");
+    const size_t bytes = (size_t)e->p.M * e->p.M * e->p.bpp * e->stage_frames;
+    for (int b = e->nstage; b < k; ++b) {
+        int rc = dev_alloc(e, &e->d_stage[b], bytes, "upload staging");
+        if (rc) return rc;
+    }
+    e->nstage = k;
+    HIP_TRY(e, hipStreamSynchronize(e->st));
     return DDM_OK;
 }
 

@@ -27,6 +27,8 @@ inline long long pad_elems(long long e) { return (e + RING_GROUP - 1) / RING_GRO
 // (0 for scales that do not use it).
 // Device buffers host frames are uploaded into, used in turn.
 constexpr int UPLOAD_STAGES = 3;
+// At most this many (a batched server that groups chunks gives its streams more).
+constexpr int MAX_UPLOAD_STAGES = 8;
 
 inline long long big_fft_scratch_elems(int M, int scale, int n) {
     return (scale == 256 || scale == 512 || scale == 1024) ? (long long)n * M * M / 2 : 0;

@@ -107,6 +107,9 @@ void engine_batch_stream(ddm_engine *e, BatchStream *bs);
 // staging buffer on its copy stream; *b is the buffer, *ev_up its upload
 // event, *stage its device address.  Pinned frames are waited for.
 int engine_batch_upload(ddm_engine *e, const void *frames, int n, int *b, hipEvent_t *ev_up, const uint8_t **stage);
+// Before the first frame: k staging buffers instead of UPLOAD_STAGES (at
+// most MAX_UPLOAD_STAGES), so that more chunks can wait for a batched tick.
+int engine_batch_stages(ddm_engine *e, int k);
 // The n frames of staging buffer b were transformed into the ring and
 // through the lag engine by work queued on st: the buffer is free once st
 // gets past this point, counts advance, and the accumulators are folded into

@@ -11,7 +11,9 @@
 //   different streams overlap on the device;
 // * batched (DDM_SERVER_BATCHED): ddm_server_submit only uploads the chunk
 //   and queues it; a scheduler thread gathers the waiting chunks of all
-//   streams (one per stream, the oldest) into a tick, writes the tick's work
+//   streams (the oldest of each, or with DDM_SERVER_GROUP all of a stream's
+//   waiting chunks up to a lag block, which then share one lag pass) into a
+//   tick, writes the tick's work
 //   queue -- one job per (stream, scale, kernel kind) for the tile
 //   transforms and one per stream for the lag engine, each job covering its
 //   tile blocks and frames or its ring groups -- and runs every kernel kind
@@ -75,6 +77,10 @@ struct Stream {
     bool closing = false;  // ddm_server_close waits for busy == 0
     std::deque<Chunk> queue;  // batched: uploaded chunks waiting for a tick
     int outstanding = 0;      // batched: chunks uploaded and not yet committed by a tick
+    int lag_block = 0;        // batched: frames one lag pass of this stream may take
+    int piece = 0;            // batched: frames of one queued chunk, at most
+    int nstage = 0;           // batched: staging buffers (chunks uploaded and not yet committed, at most)
+    int urgent = 0;           // batched: calls waiting for this stream's queue to drain (no holding)
     long long xt_need = 0;    // batched: pass-buffer values one chunk of this stream takes
     std::shared_ptr<std::mutex> submit_mu = std::make_shared<std::mutex>();  // one submission at a time
 };
@@ -121,6 +127,9 @@ struct ddm_server {
     // batched mode
     bool batched = false;
     bool use_graph = true;
+    bool group = false;  // DDM_SERVER_GROUP
+    long long hold_us = 0;  // grouping: how long a stream's chunk may wait for more of its chunks
+    int draining = 0;       // ddm_server_drain calls waiting (no holding)
     std::thread sched;
     bool sched_stop = false;
     hipStream_t tick_st = nullptr;
@@ -201,10 +210,13 @@ static hipEvent_t take_event(ddm_server *s) {
 // ---------------------------------------------------------------------------
 // Batched ticks
 // ---------------------------------------------------------------------------
+// The chunks of one stream a tick takes: the oldest waiting one, or (with
+// DDM_SERVER_GROUP) all its waiting chunks up to a lag block of frames,
+// which then share one lag pass.
 struct Taken {
     int id;
     ddm_engine *eng;
-    Chunk c;
+    std::vector<Chunk> cs;
 };
 
 static bool ensure_table(ddm_server *s, TickBuf &tb, size_t bytes) {
@@ -283,27 +295,31 @@ static int run_tick(ddm_server *s, const std::vector<Taken> &taken) {
         ddm::BatchStream bs;
         ddm::engine_batch_stream(t.eng, &bs);
         const ddm::Params &p = *bs.p;
-        const int n = t.c.n;
         const int R = p.ring_slots;
-        const int slot = (int)(bs.frames % R);
+        const int slot = (int)(bs.frames % R);  // of the tick's first frame of this stream
         const size_t roi_frame = (size_t)p.M * p.M * p.bpp;
-        ddm::PixSrc ps{t.c.stage, (long long)roi_frame, (long long)p.M * p.bpp, p.bpp, p.fmt, 0, 0};
+        int n = 0;  // frames of the stream in this tick
+        for (const Chunk &c : t.cs) {
+        const int cslot = (int)((bs.frames + n) % R);
+        n += c.n;
+        ddm::PixSrc ps{c.stage, (long long)roi_frame, (long long)p.M * p.bpp, p.bpp, p.fmt, 0, 0};
         float2 *xt = nullptr;
         long long xt_need = 0;
-        for (const ddm::ScaleGeom &gm : p.geom) xt_need = std::max(xt_need, ddm::big_fft_scratch_elems(p.M, gm.s, n));
+        for (const ddm::ScaleGeom &gm : p.geom)
+            xt_need = std::max(xt_need, ddm::big_fft_scratch_elems(p.M, gm.s, c.n));
         if (xt_need) {
             xt = s->d_xt + xt_off;
             xt_off += xt_need;
         }
         for (const ddm::ScaleGeom &gm : p.geom) {
-            ddm::RingDst rd{(float2 *)bs.ring, gm.elem_off, R, slot};
+            ddm::RingDst rd{(float2 *)bs.ring, gm.elem_off, R, cslot};
             tmp.clear();
-            if (bs.precision == DDM_PREC_FP32 && ddm::fft_batch_jobs(ps, rd, gm.s, gm.tps, gm.tiles, n, xt, &tmp)) {
+            if (bs.precision == DDM_PREC_FP32 && ddm::fft_batch_jobs(ps, rd, gm.s, gm.tps, gm.tiles, c.n, xt, &tmp)) {
                 for (auto &kj : tmp) fft[kj.first].push_back(kj.second);
             } else {
                 ddm::TileFftArgs a;
-                a.frames = t.c.stage;
-                a.n_frames = n;
+                a.frames = c.stage;
+                a.n_frames = c.n;
                 a.frame_bytes = (long long)roi_frame;
                 a.width = p.M;
                 a.bpp = p.bpp;
@@ -313,13 +329,14 @@ static int run_tick(ddm_server *s, const std::vector<Taken> &taken) {
                 a.scale = gm.s;
                 a.ring = bs.ring;
                 a.ring_slots = R;
-                a.first_slot = slot;
+                a.first_slot = cslot;
                 a.elems_per_frame = p.elems_padded;
                 a.elem_off = gm.elem_off;
                 a.precision = bs.precision;
                 a.xt = (float *)xt;
                 direct_fft.push_back(a);
             }
+        }
         }
         const int dmax = (int)std::min<long long>(p.max_lag, bs.frames);
         if (bs.lag_block <= 128) {
@@ -414,7 +431,8 @@ static int run_tick(ddm_server *s, const std::vector<Taken> &taken) {
     if (!plans.empty()) std::memcpy(tb.h + plans_off, plans.data(), plans.size() * sizeof(ddm::LagPlan));
     // ---- run: uploads, table, kernels ----
     for (const Taken &t : taken)
-        if (hipStreamWaitEvent(s->tick_st, t.c.ev_up, 0) != hipSuccess) return fail("waiting for an upload");
+        for (const Chunk &c : t.cs)
+            if (hipStreamWaitEvent(s->tick_st, c.ev_up, 0) != hipSuccess) return fail("waiting for an upload");
     if (hipMemcpyAsync(tb.d, tb.h, off, hipMemcpyHostToDevice, s->tick_st) != hipSuccess)
         return fail("tick table upload failed");
     const bool graph = s->use_graph && direct_fft.empty() && direct_lag.empty();
@@ -468,12 +486,13 @@ static int run_tick(ddm_server *s, const std::vector<Taken> &taken) {
     }
     if (hipEventRecord(tb.done, s->tick_st) != hipSuccess) return fail("tick event");
     tb.used = true;
-    for (const Taken &t : taken) {
-        const int rc = ddm::engine_batch_commit(t.eng, t.c.n, t.c.b, s->tick_st);
-        if (rc) return fail(ddm_last_error(t.eng));
-    }
+    for (const Taken &t : taken)
+        for (const Chunk &c : t.cs) {
+            const int rc = ddm::engine_batch_commit(t.eng, c.n, c.b, s->tick_st);
+            if (rc) return fail(ddm_last_error(t.eng));
+            s->chunks_batched += 1;
+        }
     s->ticks += 1;
-    s->chunks_batched += (long long)taken.size();
     for (const FftLaunch &f : fl) s->items += f.blocks;
     for (const LagLaunch &l : ll) s->items += l.groups;
     return DDM_OK;
@@ -485,29 +504,69 @@ static bool any_queued(const ddm_server *s) {
     return false;
 }
 
+// Whether a stream's waiting chunks go into the next tick: always, unless
+// grouping holds them (for at most hold_us after the oldest arrived) while a
+// fuller lag pass can still form and nobody waits for the stream.
+static bool stream_ready(const ddm_server *s, const Stream &st, Clock::time_point now) {
+    if (st.queue.empty()) return false;
+    if (!s->group || s->hold_us <= 0 || s->sched_stop || s->draining || st.urgent) return true;
+    int frames = 0;
+    for (const Chunk &c : st.queue) frames += c.n;
+    if (frames + st.piece > st.lag_block) return true;        // the pass is full
+    if ((int)st.queue.size() + 1 >= st.nstage) return true;   // the next submission would block
+    return now - st.queue.front().t_submit >= std::chrono::microseconds(s->hold_us);
+}
+
 static void sched_loop(ddm_server *s) {
     std::unique_lock<std::mutex> lk(s->mu);
     int start = 0;  // streams take turns at being first when the pass buffer is short
     for (;;) {
-        s->work.wait(lk, [&] { return s->sched_stop || any_queued(s); });
-        if (!any_queued(s)) break;  // stop requested and nothing left
-        // the oldest chunk of every stream with one waiting, within the
-        // tick's pass buffer
+        // wait for a stream whose chunks are due
+        bool ready = false;
+        for (;;) {
+            const Clock::time_point now = Clock::now();
+            Clock::time_point due = Clock::time_point::max();
+            for (auto &kv : s->streams) {
+                if (stream_ready(s, kv.second, now)) {
+                    ready = true;
+                    break;
+                }
+                if (!kv.second.queue.empty())
+                    due = std::min(due, kv.second.queue.front().t_submit + std::chrono::microseconds(s->hold_us));
+            }
+            if (ready) break;
+            if (s->sched_stop && !any_queued(s)) return;
+            if (due == Clock::time_point::max())
+                s->work.wait(lk);
+            else
+                s->work.wait_until(lk, due);
+        }
+        // the chunks of every due stream (the oldest, or with grouping all
+        // of them up to a lag block), within the tick's pass buffer
         std::vector<Taken> taken;
         std::vector<std::pair<int, Stream *>> order;
+        const Clock::time_point now = Clock::now();
         for (auto &kv : s->streams)
-            if (!kv.second.queue.empty()) order.emplace_back(kv.first, &kv.second);
+            if (stream_ready(s, kv.second, now)) order.emplace_back(kv.first, &kv.second);
         std::rotate(order.begin(), order.begin() + (start % order.size()), order.end());
         ++start;
         long long xt = 0;
         for (auto &o : order) {
             Stream &st = *o.second;
-            const long long need = st.xt_need;
-            if (!taken.empty() && xt + need > s->xt_cap) continue;
-            xt += need;
-            taken.push_back(Taken{o.first, st.eng, st.queue.front()});
-            st.queue.pop_front();
+            Taken t{o.first, st.eng, {}};
+            int frames = 0;
+            while (!st.queue.empty()) {
+                const Chunk &c = st.queue.front();
+                if (!t.cs.empty() && (!s->group || frames + c.n > st.lag_block)) break;
+                if (!(taken.empty() && t.cs.empty()) && xt + st.xt_need > s->xt_cap) break;
+                xt += st.xt_need;
+                frames += c.n;
+                t.cs.push_back(c);
+                st.queue.pop_front();
+            }
+            if (t.cs.empty()) continue;
             st.busy += 1;
+            taken.push_back(std::move(t));
         }
         lk.unlock();
         int rc = DDM_OK;
@@ -521,12 +580,14 @@ static void sched_loop(ddm_server *s) {
             auto it = s->streams.find(t.id);
             if (it == s->streams.end()) continue;
             Stream &st = it->second;
-            if (rc == DDM_OK) {
-                hipEvent_t ev = take_event(s);
-                if (ev && hipEventRecord(ev, s->tick_st) == hipSuccess)
-                    st.pending.push_back(Pending{ev, t.c.t_submit});
+            for (const Chunk &c : t.cs) {
+                if (rc == DDM_OK) {
+                    hipEvent_t ev = take_event(s);
+                    if (ev && hipEventRecord(ev, s->tick_st) == hipSuccess)
+                        st.pending.push_back(Pending{ev, c.t_submit});
+                }
+                st.outstanding -= 1;
             }
-            st.outstanding -= 1;
             st.busy -= 1;
         }
         s->idle.notify_all();
@@ -548,7 +609,7 @@ static Stream *wait_queue_empty(ddm_server *s, std::unique_lock<std::mutex> &lk,
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" int ddm_server_create2(size_t device_budget, int max_streams, int flags, ddm_server **out) {
-    if (!out || max_streams <= 0 || (flags & ~(DDM_SERVER_BATCHED | DDM_SERVER_NO_GRAPH)))
+    if (!out || max_streams <= 0 || (flags & ~(DDM_SERVER_BATCHED | DDM_SERVER_NO_GRAPH | DDM_SERVER_GROUP)))
         return serr(DDM_EINVAL, "bad server arguments");
     *out = nullptr;
     int ndev = 0;
@@ -572,6 +633,7 @@ extern "C" int ddm_server_create2(size_t device_budget, int max_streams, int fla
     if (flags & DDM_SERVER_BATCHED) {
         s->batched = true;
         s->use_graph = !(flags & DDM_SERVER_NO_GRAPH);
+        s->group = (flags & DDM_SERVER_GROUP) != 0;
         bool ok = hipStreamCreateWithFlags(&s->tick_st, hipStreamNonBlocking) == hipSuccess &&
                   hipStreamCreateWithFlags(&s->cap_st, hipStreamNonBlocking) == hipSuccess;
         for (TickBuf &tb : s->tb) ok = ok && hipEventCreateWithFlags(&tb.done, hipEventDisableTiming) == hipSuccess;
@@ -594,6 +656,16 @@ extern "C" int ddm_server_create2(size_t device_budget, int max_streams, int fla
     }
     s->poller = std::thread(poll_loop, s);
     *out = s;
+    return DDM_OK;
+}
+
+extern "C" int ddm_server_set_hold(ddm_server *s, double hold_ms) {
+    if (!s || !(hold_ms >= 0.0) || hold_ms > 10000.0) return serr(DDM_EINVAL, "bad hold time");
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->hold_us = (long long)(hold_ms * 1000.0);
+    }
+    s->work.notify_all();
     return DDM_OK;
 }
 
@@ -640,7 +712,14 @@ extern "C" int ddm_server_open(ddm_server *s, const ddm_desc *desc, int *stream_
     std::string msg;
     int rc = ddm::check_desc(desc, &p, &msg);
     if (rc != DDM_OK) return serr(rc, msg);
-    const size_t need = ddm::device_footprint(p);
+    // grouping: enough staging buffers for a full lag pass of chunks to wait
+    // while the next one uploads
+    const int piece = std::min(p.chunk, p.lag_block);
+    const int stages = (s->batched && s->group)
+                           ? std::min(ddm::MAX_UPLOAD_STAGES, std::max(ddm::UPLOAD_STAGES, p.lag_block / piece + 2))
+                           : ddm::UPLOAD_STAGES;
+    const size_t need = ddm::device_footprint(p) +
+                        (size_t)(stages - ddm::UPLOAD_STAGES) * p.chunk * p.M * p.M * p.bpp;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if ((int)s->streams.size() >= s->max_streams) return serr(DDM_EBUSY, "stream slots are all in use");
@@ -661,10 +740,19 @@ extern "C" int ddm_server_open(ddm_server *s, const ddm_desc *desc, int *stream_
             (void)hipMemPoolTrimTo(pool, 0);
         rc = ddm_create(desc, &e);
     }
+    std::string why;
+    if (rc == DDM_OK && stages > ddm::UPLOAD_STAGES) {
+        rc = ddm::engine_batch_stages(e, stages);
+        if (rc != DDM_OK) {
+            why = ddm_last_error(e);
+            ddm_destroy(e);
+        }
+    } else if (rc != DDM_OK) {
+        why = ddm_last_error(nullptr);
+    }
     std::lock_guard<std::mutex> lk(s->mu);
     if (rc != DDM_OK) {
         s->used -= need;
-        const std::string why = ddm_last_error(nullptr);
         // a refusal like any other over-budget request: the caller may try
         // again when streams have left
         if (rc == DDM_ENOMEM) return serr(DDM_EBUSY, "device memory exhausted below the budget (" + why + ")");
@@ -674,7 +762,9 @@ extern "C" int ddm_server_open(ddm_server *s, const ddm_desc *desc, int *stream_
     Stream &st = s->streams[id];
     st.eng = e;
     st.footprint = need;
-    const int piece = std::min(p.chunk, p.lag_block);
+    st.lag_block = p.lag_block;
+    st.piece = piece;
+    st.nstage = stages;
     for (const ddm::ScaleGeom &g : p.geom)
         st.xt_need = std::max(st.xt_need, ddm::big_fft_scratch_elems(p.M, g.s, piece));
     *stream_id = id;
@@ -700,11 +790,14 @@ static int submit_batched(ddm_server *s, int stream_id, ddm_engine *e, const voi
         c.n = std::min(piece, n_frames - i);
         c.t_submit = t0;
         // back-pressure: the staging buffer this chunk takes held the
-        // stream's chunk UPLOAD_STAGES back, which must have been committed
+        // stream's chunk nstage back, which must have been committed
         // by its tick (the upload then waits for that tick to finish)
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->idle.wait(lk, [&] { return s->streams.find(stream_id)->second.outstanding < ddm::UPLOAD_STAGES; });
+            s->idle.wait(lk, [&] {
+                const Stream &st = s->streams.find(stream_id)->second;
+                return st.outstanding < st.nstage;
+            });
         }
         const int rc = ddm::engine_batch_upload(e, (const uint8_t *)frames + (size_t)i * fb, c.n, &c.b, &c.ev_up,
                                                 &c.stage);
@@ -770,7 +863,13 @@ extern "C" int ddm_server_results(ddm_server *s, int stream_id, double *results)
         std::unique_lock<std::mutex> lk(s->mu);
         int rc = acquire(s, stream_id, &e);
         if (rc) return rc;
-        if (s->batched && !wait_queue_empty(s, lk, stream_id)) return serr(DDM_ERANGE, "unknown stream");
+        if (s->batched) {
+            auto it = s->streams.find(stream_id);
+            it->second.urgent += 1;  // no holding its chunks back now
+            s->work.notify_all();
+            Stream *st = wait_queue_empty(s, lk, stream_id);
+            if (st) st->urgent -= 1;
+        }
     }
     int rc;
     if (s->batched) {
@@ -811,12 +910,15 @@ extern "C" int ddm_server_drain(ddm_server *s) {
     if (!s) return DDM_EINVAL;
     if (s->batched) {
         std::unique_lock<std::mutex> lk(s->mu);
+        s->draining += 1;  // no holding chunks back now
+        s->work.notify_all();
         s->idle.wait(lk, [&] {
             if (any_queued(s)) return false;
             for (auto &kv : s->streams)
                 if (kv.second.busy) return false;
             return true;
         });
+        s->draining -= 1;
         lk.unlock();
         if (hipStreamSynchronize(s->tick_st) != hipSuccess) return serr(DDM_EDEVICE, "tick stream failed");
         if (!s->tick_err.empty()) return serr(DDM_EDEVICE, s->tick_err);
@@ -852,6 +954,8 @@ extern "C" int ddm_server_close(ddm_server *s, int stream_id) {
         auto it = s->streams.find(stream_id);
         if (it == s->streams.end() || it->second.closing) return serr(DDM_ERANGE, "unknown stream");
         it->second.closing = true;  // no new user from here on
+        it->second.urgent += 1;     // and no holding its chunks back
+        s->work.notify_all();
         s->idle.wait(lk, [&] { return it->second.busy == 0 && it->second.queue.empty(); });
         st = std::move(it->second);
         s->streams.erase(it);

@@ -29,7 +29,7 @@ PIX_U8, PIX_U16LE, PIX_U16BE = 0, 1, 2
 NORM_FRAMES, NORM_PAIRS = 0, 1
 PREC_FP32, PREC_BF16X2, PREC_FP16X2 = 0, 1, 2
 ACC_COMPENSATED, ACC_PLAIN = 0, 1
-SERVER_BATCHED, SERVER_NO_GRAPH = 1, 2
+SERVER_BATCHED, SERVER_NO_GRAPH, SERVER_GROUP = 1, 2, 4
 
 # Every symbol include/ddm_amd.h declares.
 EXPORTED = [
@@ -41,7 +41,7 @@ EXPORTED = [
     "ddm_synth_uniform", "ddm_synth_brownian", "ddm_write_iqt_file", "ddm_flush",
     "ddm_server_create", "ddm_server_destroy", "ddm_server_open", "ddm_server_submit", "ddm_server_results",
     "ddm_server_engine", "ddm_server_drain", "ddm_server_close", "ddm_server_latencies", "ddm_server_stats",
-    "ddm_server_last_error", "ddm_server_create2", "ddm_fit_results", "ddm_k_fit", "ddm_set_precision", "ddm_autotune", "ddm_reserve",
+    "ddm_server_last_error", "ddm_server_create2", "ddm_server_set_hold", "ddm_fit_results", "ddm_k_fit", "ddm_set_precision", "ddm_autotune", "ddm_reserve",
 ]
 
 
@@ -147,6 +147,7 @@ def lib():
     L.ddm_flush.argtypes = [vp]
     L.ddm_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(vp)]
     L.ddm_server_create2.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+    L.ddm_server_set_hold.argtypes = [vp, ctypes.c_double]
     L.ddm_server_destroy.argtypes = [vp]
     L.ddm_server_destroy.restype = None
     L.ddm_server_open.argtypes = [vp, ctypes.POINTER(Desc), ctypes.POINTER(ctypes.c_int)]
@@ -344,17 +345,22 @@ class Server:
     footprint, chunks of different streams overlapping on the GPU, and
     chunk-to-result latencies."""
 
-    def __init__(self, device_budget: int = 0, max_streams: int = 256, batched: bool = False, graph: bool = True):
+    def __init__(self, device_budget: int = 0, max_streams: int = 256, batched: bool = False, graph: bool = True,
+                 group: bool = False, hold_ms: float = 0.0):
         """batched: chunks of all streams run as ticks of batched launches
-        (DDM_SERVER_BATCHED), as HIP graphs unless graph is False."""
+        (DDM_SERVER_BATCHED), as HIP graphs unless graph is False; group: a
+        tick takes all of a stream's waiting chunks up to its lag block into
+        one lag pass (DDM_SERVER_GROUP)."""
         h = ctypes.c_void_p()
-        flags = (SERVER_BATCHED if batched else 0) | (0 if graph else SERVER_NO_GRAPH)
+        flags = (SERVER_BATCHED if batched else 0) | (0 if graph else SERVER_NO_GRAPH) | (SERVER_GROUP if group else 0)
         rc = lib().ddm_server_create2(device_budget, max_streams, flags, ctypes.byref(h))
         if rc != DDM_OK:
             raise DDMError(rc, lib().ddm_server_last_error(None).decode())
         self._h = h
         self._keep = {}
         self._shape = {}
+        if hold_ms:
+            self._check(lib().ddm_server_set_hold(self._h, hold_ms))
 
     def _check(self, rc):
         if rc < 0:

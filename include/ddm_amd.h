@@ -264,9 +264,23 @@ typedef struct ddm_server_info {
  *   three chunks waiting (further submissions block).  A stream's chunks are
  *   submitted in pieces of at most min(chunk, lag block) frames.  Results
  *   are bit-identical to the per-stream mode.
- * DDM_SERVER_NO_GRAPH: batched ticks are launched directly, not as graphs. */
+ * DDM_SERVER_NO_GRAPH: batched ticks are launched directly, not as graphs.
+ * DDM_SERVER_GROUP: a batched tick takes all of a stream's waiting chunks up
+ *   to its lag block and runs them as one lag pass (a chunk that arrives
+ *   alone still runs alone): under load the stream's history is read once
+ *   for several chunks.  Results then depend on how chunks happened to be
+ *   grouped, within the FP32 path's error bound, instead of being
+ *   bit-identical to the per-stream mode. */
 #define DDM_SERVER_BATCHED 1
 #define DDM_SERVER_NO_GRAPH 2
+#define DDM_SERVER_GROUP 4
+
+/* With DDM_SERVER_GROUP: a stream's waiting chunks may wait up to hold_ms
+ * (from the arrival of the oldest) for more of the stream's chunks, so that a
+ * lag pass takes several; they go at once when a pass is full, when the
+ * stream's staging buffers are all in use, or when results, close or drain
+ * wait for the stream.  0 (the default) never holds a chunk back. */
+int ddm_server_set_hold(ddm_server *srv, double hold_ms);
 
 /* device_budget 0 = 90 % of the device memory free at creation (a batched
  * server takes 1 GiB of it for the tick pass buffer). */

@@ -148,3 +148,46 @@ def test_batched_results_mid_stream_and_close_while_others_run(gpu):
         assert o.max_rel_err(got, o.run(cfg, f[:n])) <= 1e-5
     ref, _ = _run(["c"], batched=False)
     assert np.array_equal(full_c, ref["c"])
+
+
+def test_grouped_chunks_share_a_lag_pass(gpu):
+    """DDM_SERVER_GROUP: a tick takes all of a stream's waiting chunks up to
+    its lag block into one lag pass.  Chunks submitted back to back pile up,
+    so passes of several chunks happen; results stay within the FP32 bound of
+    the oracle, and alone vs mix agree to within that bound."""
+    pool = serving_streams(64)
+    picked = [next(s for s in pool if s.size == size) for size in (256, 512)]
+    N = 330
+    frames = [s.make_frames(N) for s in picked]
+    chunks = [[f[i:i + 30] for i in range(0, N, 30)] for f in frames]
+    res = {}
+    for mode in ("alone", "mix"):
+        out = []
+        with ddm.Server(batched=True, group=True) as srv:
+            if mode == "alone":
+                for k, s in enumerate(picked):
+                    sid = srv.open(**s.desc(lag_block=128))
+                    for ch in chunks[k]:
+                        srv.submit(sid, ch)
+                    out.append(srv.results(sid))
+            else:
+                ids = [srv.open(**s.desc(lag_block=128)) for s in picked]
+                for i in range(len(chunks[0])):
+                    for k in range(len(picked)):
+                        srv.submit(ids[k], chunks[k][i])
+                out = [srv.results(i) for i in ids]
+            st = srv.stats()
+            print(f"{mode}: {st.chunks} chunks in {st.ticks} ticks")
+        res[mode] = out
+    for k, s in enumerate(picked):
+        d = s.desc()
+        cfg = o.Config(s.size, s.size, N, d["scales"], d["lags"], d["lambdas"], np.float32(1.2).item())
+        ref = o.run(cfg, frames[k].reshape(-1), "f64", nthreads=host_threads(), tiles=[0])
+        g = {m: o.split_results(cfg, res[m][k]) for m in res}
+        r = o.split_results(cfg, ref)
+        for sc in d["scales"]:
+            for m in res:
+                assert o.max_rel_err(g[m][sc][0], r[sc][0]) <= 1e-5, (k, sc, m)
+        a, b = res["alone"][k], res["mix"][k]
+        nz = a != 0
+        assert np.all(b[~nz] == 0) and np.max(np.abs(a[nz] - b[nz]) / np.abs(a[nz])) <= 1e-6
