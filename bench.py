@@ -64,15 +64,17 @@ def distinct_frames(t0, n, lags):
     return len(s)
 
 
-def lag_engine_bytes(E, lags, n_frames, block):
+def lag_engine_bytes(E, lags, n_frames, block, compensated=True):
     """Algorithmic HBM bytes of every lag-engine launch of one analysis (the
     engine runs one launch per block of `block` frames, the last one partial):
     each distinct ring frame a launch pairs with read once (8 B per complex
-    element) plus the FP32 accumulators of every lag read and written once."""
+    element) plus the FP32 accumulators of every lag read and written once,
+    and with compensated accumulation their FP32 compensation terms too."""
+    acc = 16 if compensated else 8
     total, launches = 0, 0
     for t0 in range(0, n_frames, block):
         m = min(block, n_frames - t0)
-        total += 8 * E * distinct_frames(t0, m, lags) + 8 * E * len(lags)
+        total += 8 * E * distinct_frames(t0, m, lags) + acc * E * len(lags)
         launches += 1
     return total, launches
 
@@ -450,7 +452,7 @@ def main():
         info = e.info()
         lag_block = info.lag_block
     E = info.elems_per_frame
-    lag_bytes, lag_launches = lag_engine_bytes(E, lags, N, lag_block)
+    lag_bytes, lag_launches = lag_engine_bytes(E, lags, N, lag_block, args.accumulation == "compensated")
     assert lag_launches == st.lag_launches, (lag_launches, st.lag_launches)
     lag_avg_ms = st.lag_ms / st.lag_launches
     achieved = lag_bytes / (st.lag_ms * 1e-3) / 1e9
@@ -459,7 +461,8 @@ def main():
     pipe_bytes = (M * M * W.bpp + 8 * E) * N + lag_bytes
     pipe_gbs = pipe_bytes / (elapsed / args.steps) / 1e9
 
-    traffic, traffic_src = pmc_traffic(args.workload, f"lag_kernel<{lag_block}>")
+    kernel = f"lag_kernel<{lag_block}, {'true' if args.accumulation == 'compensated' else 'false'}>"
+    traffic, traffic_src = pmc_traffic(args.workload, kernel.replace(" ", ""))
     # the lag engine's compute floor: every valid pair costs four FP32 VALU
     # instructions per element (two subtractions, two fused multiply-adds),
     # and a CU issues at most one wave64 VALU instruction per clock
@@ -497,7 +500,7 @@ def main():
                             "what": "device time of each FFT batch (tile + FFT + the lag launches it completes)"},
         "roofline": {
             "bound": "hbm",
-            "kernel": f"lag_kernel<{lag_block}>",
+            "kernel": kernel,
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
