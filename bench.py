@@ -40,6 +40,9 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+SPECTRUM = {"masked": 0, "full": 1}  # ddm_desc.spectrum
+
+
 def pmc_traffic(workload: str, kernel: str):
     """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3
     PMC summary profiles/pmc_traffic_<workload>.md (FETCH_SIZE and WRITE_SIZE
@@ -143,7 +146,8 @@ def serving(args):
         while waiting:
             k = waiting[0]
             try:
-                sid = timed("open", lambda: srv.open(**streams[k].desc(args.serving_lag_block)))
+                sid = timed("open", lambda: srv.open(spectrum=SPECTRUM[args.spectrum],
+                                                     **streams[k].desc(args.serving_lag_block)))
             except ddm.DDMError as e:
                 if e.code != ddm.DDM_EBUSY:
                     raise
@@ -210,7 +214,8 @@ def serving(args):
                                + ("streams resident" if not args.chunks_per_stream else
                                   f"streams replaced after {args.chunks_per_stream} chunks")
                                + (", batched launches" if args.batched else ", per-stream launches")
-                               + (f", chunks grouped per lag pass (hold {args.hold_ms} ms)" if args.group else "")},
+                               + (f", chunks grouped per lag pass (hold {args.hold_ms} ms)" if args.group else "")
+                               + f", {args.spectrum} spectrum"},
     }
     srv.close()
     print(json.dumps(out))
@@ -333,6 +338,8 @@ def main():
                     help="64/128 tile transforms: FP32 FFT or split-operand DFT on the matrix cores")
     ap.add_argument("--accumulation", default="compensated", choices=["compensated", "plain"],
                     help="lag sums: compensated FP32 folded into FP64 every 10^4 pairs, or plain FP32")
+    ap.add_argument("--spectrum", default="masked", choices=["masked", "full"],
+                    help="spectrum elements kept: the band holding every q-ring member, or all of them")
     ap.add_argument("--autotune", action="store_true",
                     help="let the engine's autotuner pick the lag block and FFT batch (ddm_autotune)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -395,7 +402,8 @@ def main():
 
     def make():
         e = ddm.Engine(*W.engine_args(), lag_block=args.lag_block,
-                       accumulation=ddm.ACC_PLAIN if args.accumulation == "plain" else ddm.ACC_COMPENSATED)
+                       accumulation=ddm.ACC_PLAIN if args.accumulation == "plain" else ddm.ACC_COMPENSATED,
+                       spectrum=SPECTRUM[args.spectrum])
         e.set_precision(prec)
         if args.autotune:
             # timed on the first engine, read back from the cache by the others
@@ -492,7 +500,7 @@ def main():
             # ~128 frames per batch (the engine's FFT batch)
             "fft_batch_frames": schedule.get("fft_batch_frames", W.chunk * max(1, 128 // W.chunk)),
             "lag_block": lag_block, "schedule": "autotuned" if args.autotune else "default",
-            "precision": args.precision, "accumulation": args.accumulation,
+            "precision": args.precision, "accumulation": args.accumulation, "spectrum": args.spectrum,
         },
         "stage_ms_per_step": {"fft": round(st.fft_ms, 3), "lag": round(st.lag_ms, 3),
                               "azimuthal": round(st.azimuthal_ms, 3)},

@@ -45,22 +45,74 @@ struct RingDst {
     long long elem_off;  // of this scale
     int ring_slots;
     int first_slot;      // slot of frame 0 of this launch
+    TileLayout lay;      // the part of each tile's half spectrum the ring holds
 };
 
-// Writes element `local` of tile `tile` of batch frame f.
+// floor(n / d) for 0 <= n < 2^22 and d >= 1 with rcp = 1.0f / d: the float
+// product is within one of the quotient, and one step corrects it.
+__device__ __forceinline__ int div_small(int n, int d, float rcp) {
+    int q = (int)((float)n * rcp);
+    const int r = n - q * d;
+    q += (r >= d) ? 1 : 0;
+    q -= (r < 0) ? 1 : 0;
+    return q;
+}
+
+// Writes the elements of tile `tile` of batch frame f into the ring,
+// addressed by (ky, kx) of the tile's half spectrum; elements the layout
+// leaves out are dropped.
 struct TileOut {
     float2 *ring;
     long long base;  // element index of the tile's first element
-    int slot, R;
-    __device__ __forceinline__ float2 &operator[](long long local) const {
-        return ring[ring_index(base + local, slot, R)];
+    int slot, R, S;
+    TileLayout lay;
+    __device__ __forceinline__ void put(int ky, int kx, float2 v) const {
+        const long long at = lay.pos(S, ky, kx);
+        if (at >= 0) ring[ring_index(base + at, slot, R)] = v;
+    }
+    // element (ky, kx) of a tile in the full row-order layout
+    __device__ __forceinline__ float2 &full(int ky, int kx) const {
+        return ring[ring_index(base + (long long)ky * (S / 2 + 1) + kx, slot, R)];
     }
 };
 
-__device__ __forceinline__ TileOut tile_out(const RingDst &rd, int f, int tile, long long tile_elems) {
-    int slot = rd.first_slot + f;
-    slot -= (slot >= rd.ring_slots) ? rd.ring_slots : 0;  // f < ring_slots
-    return TileOut{rd.ring, rd.elem_off + (long long)tile * tile_elems, slot, rd.ring_slots};
+__device__ __forceinline__ int ring_slot(const RingDst &rd, int f) {
+    const int slot = rd.first_slot + f;
+    return slot - ((slot >= rd.ring_slots) ? rd.ring_slots : 0);  // f < ring_slots
+}
+
+__device__ __forceinline__ TileOut tile_out(const RingDst &rd, int f, int tile, int S) {
+    return TileOut{rd.ring, rd.elem_off + (long long)tile * rd.lay.elems(), ring_slot(rd, f), rd.ring_slots, S,
+                   rd.lay};
+}
+
+// The spectra of ntile consecutive tiles (first, first + 1, ...) of batch
+// frame f, staged in LDS at lds[t * S * (S/2 + 1) + ky * (S/2 + 1) + kx],
+// stored to the ring in the row-order layout rd.lay (full or a band) by nth
+// threads: consecutive threads write consecutive ring elements.
+template <int S>
+__device__ __forceinline__ void store_tiles(const RingDst &rd, const float2 *lds, int f, int first, int ntile,
+                                            int tid, int nth) {
+    constexpr int H = S / 2, TE = S * (H + 1);
+    const int slot = ring_slot(rd, f);
+    const long long gstride = (long long)rd.ring_slots * RING_GROUP;
+    if (rd.lay.is_full(S)) {
+        for (int i = tid; i < ntile * TE; i += nth) {
+            const long long e = rd.elem_off + (long long)first * TE + i;
+            rd.ring[(e >> RING_GROUP_LOG2) * gstride + (long long)slot * RING_GROUP + (e & (RING_GROUP - 1))] = lds[i];
+        }
+        return;
+    }
+    const int cols = rd.lay.cols, nr = rd.lay.rows(), CE = cols * nr;
+    const float rce = 1.0f / (float)CE, rcols = 1.0f / (float)cols;
+    for (int i = tid; i < ntile * CE; i += nth) {
+        const int t = div_small(i, CE, rce), ci = i - t * CE;
+        const int r = div_small(ci, cols, rcols), kx = ci - r * cols;
+        const int ky = r < rd.lay.row_lo ? r : r + (S - nr);
+        const long long e = rd.elem_off + (long long)first * CE + i;
+        rd.ring[(e >> RING_GROUP_LOG2) * gstride + (long long)slot * RING_GROUP + (e & (RING_GROUP - 1))] =
+            lds[t * TE + ky * (H + 1) + kx];
+    }
 }
 
 // Register-resident tile FFT (ddm_fft.hip) for the scales it covers;

@@ -248,7 +248,7 @@ __global__ __launch_bounds__(MF_NTH) void tile_dft_mfma(PixSrc ps, RingDst rd, i
     }
     __syncthreads();
     if (!valid) return;
-    const TileOut out = tile_out(rd, f, tile, (long long)S * (H + 1));
+    const TileOut out = tile_out(rd, f, tile, S);
 #pragma unroll
     for (int a = 0; a < RT; ++a)
 #pragma unroll
@@ -256,15 +256,14 @@ __global__ __launch_bounds__(MF_NTH) void tile_dft_mfma(PixSrc ps, RingDst rd, i
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int ky = r0 + 16 * a + lk + r, k = 16 * b + li;
-                if (k != 0) out[(long long)ky * (H + 1) + k] =
-                    make_float2(acc[a][b][r] * BACK, acc[a][b + CT / 2][r] * BACK);
+                if (k != 0) out.put(ky, k, make_float2(acc[a][b][r] * BACK, acc[a][b + CT / 2][r] * BACK));
             }
     // separate the packed column: DC (kx = 0) and Nyquist (kx = H)
     for (int ky = lane; ky < S; ky += 64) {
         if (S == 128 && w != 0) break;  // one wave per tile writes it
         const float2 wv = zc[t][ky], pv = zc[t][(S - ky) & (S - 1)];
-        out[(long long)ky * (H + 1)] = make_float2(0.5f * (wv.x + pv.x), 0.5f * (wv.y - pv.y));
-        out[(long long)ky * (H + 1) + H] = make_float2(0.5f * (wv.y + pv.y), -0.5f * (wv.x - pv.x));
+        out.put(ky, 0, make_float2(0.5f * (wv.x + pv.x), 0.5f * (wv.y - pv.y)));
+        out.put(ky, H, make_float2(0.5f * (wv.y + pv.y), -0.5f * (wv.x - pv.x)));
     }
 }
 

@@ -435,6 +435,7 @@ static int process_chunk(ddm_engine *e, const uint8_t *src, int n, long long fra
         a.first_slot = first_slot;
         a.elems_per_frame = p.elems_padded;
         a.elem_off = g.elem_off;
+        a.lay = g.lay;
         a.precision = e->precision;
         a.xt = e->d_xt;
         HIP_TRY(e, ddm::launch_tile_fft(a, e->st));
@@ -832,6 +833,24 @@ extern "C" int ddm_get_params(const ddm_engine *e, int *scales, int *lags, float
     return DDM_OK;
 }
 
+// Element values of every tile of a scale, `comps` floats each, stored in
+// the scale's tile layout (src), into out in the full row order
+// [tile][ky][kx <= s/2]; elements the layout leaves out read NaN.
+static void expand_tiles(const ddm::ScaleGeom &g, const float *src, int comps, float *out) {
+    const int s = g.s, H = s / 2;
+    const size_t full = (size_t)s * (H + 1);
+    for (int t = 0; t < g.tiles; ++t) {
+        const float *ts = src + (size_t)t * g.tile_elems * comps;
+        float *to = out + (size_t)t * full * comps;
+        for (int ky = 0; ky < s; ++ky)
+            for (int kx = 0; kx <= H; ++kx) {
+                const long long at = g.lay.pos(s, ky, kx);
+                float *o = to + ((size_t)ky * (H + 1) + kx) * comps;
+                for (int c = 0; c < comps; ++c) o[c] = at < 0 ? NAN : ts[(size_t)at * comps + c];
+            }
+    }
+}
+
 extern "C" int ddm_get_accumulators(ddm_engine *e, int scale_idx, float *out) {
     if (!e || !out) return DDM_EINVAL;
     const Params &p = e->p;
@@ -841,27 +860,27 @@ extern "C" int ddm_get_accumulators(ddm_engine *e, int scale_idx, float *out) {
     rc = ddm_sync(e);
     if (rc) return rc;
     const ddm::ScaleGeom &g = p.geom[scale_idx];
-    const size_t n = (size_t)g.tiles * g.tile_elems;
-    for (size_t l = 0; l < p.lags.size(); ++l)
-        HIP_TRY(e, hipMemcpy(out + l * n, e->d_acc + l * (size_t)p.elems_padded + g.elem_off, n * sizeof(float),
-                             hipMemcpyDeviceToHost));
+    const size_t n = (size_t)g.tiles * g.tile_elems;             // stored
+    const size_t nfull = (size_t)g.tiles * g.s * (g.s / 2 + 1);  // returned
     // the reference accumulates (|dF| / s^2)^2; 1/s^4 is a power of two, so the
     // scaling is exact
     const double inv = 1.0 / ((double)g.s * g.s * (double)g.s * g.s);
-    if (!e->d_comp) {
-        for (size_t i = 0; i < n * p.lags.size(); ++i) out[i] *= (float)inv;
-        return DDM_OK;
-    }
-    // compensated: acc + comp (+ the FP64 part), rounded once to FP32
-    std::vector<float> c(n);
+    std::vector<float> a(n), c(e->d_comp ? n : 0);
     std::vector<double> d(e->acc64_live ? n : 0);
     for (size_t l = 0; l < p.lags.size(); ++l) {
         const size_t off = l * (size_t)p.elems_padded + g.elem_off;
-        HIP_TRY(e, hipMemcpy(c.data(), e->d_comp + off, n * sizeof(float), hipMemcpyDeviceToHost));
-        if (e->acc64_live) HIP_TRY(e, hipMemcpy(d.data(), e->d_acc64 + off, n * sizeof(double), hipMemcpyDeviceToHost));
-        float *o = out + l * n;
-        for (size_t i = 0; i < n; ++i)
-            o[i] = (float)((((double)o[i] + (double)c[i]) + (e->acc64_live ? d[i] : 0.0)) * inv);
+        HIP_TRY(e, hipMemcpy(a.data(), e->d_acc + off, n * sizeof(float), hipMemcpyDeviceToHost));
+        if (e->d_comp) {
+            // compensated: acc + comp (+ the FP64 part), rounded once to FP32
+            HIP_TRY(e, hipMemcpy(c.data(), e->d_comp + off, n * sizeof(float), hipMemcpyDeviceToHost));
+            if (e->acc64_live)
+                HIP_TRY(e, hipMemcpy(d.data(), e->d_acc64 + off, n * sizeof(double), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; ++i)
+                a[i] = (float)((((double)a[i] + (double)c[i]) + (e->acc64_live ? d[i] : 0.0)) * inv);
+        } else {
+            for (size_t i = 0; i < n; ++i) a[i] *= (float)inv;
+        }
+        expand_tiles(g, a.data(), 1, out + l * nfull);
     }
     return DDM_OK;
 }
@@ -883,7 +902,7 @@ extern "C" int ddm_get_spectra(ddm_engine *e, long long frame, int scale_idx, fl
     HIP_TRY(e, hipMemcpy2D(rows.data(), G * 2 * sizeof(float),
                            e->d_ring + ((g0 * p.ring_slots + slot) * G) * 2, (size_t)p.ring_slots * G * 2 * sizeof(float),
                            G * 2 * sizeof(float), g1 - g0, hipMemcpyDeviceToHost));
-    std::memcpy(out, rows.data() + ((size_t)g.elem_off - g0 * G) * 2, n * 2 * sizeof(float));
+    expand_tiles(g, rows.data() + ((size_t)g.elem_off - g0 * G) * 2, 2, out);
     return DDM_OK;
 }
 
@@ -1037,6 +1056,7 @@ extern "C" int ddm_autotune(ddm_engine *e, const char *cache_path, int *lag_bloc
                     a.first_slot = 0;
                     a.elems_per_frame = p.elems_padded;
                     a.elem_off = g.elem_off;
+                    a.lay = g.lay;
                     a.precision = e->precision;
                     a.xt = e->d_xt;
                     const hipError_t err = ddm::launch_tile_fft(a, e->st);

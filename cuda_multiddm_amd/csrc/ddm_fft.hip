@@ -388,14 +388,7 @@ __device__ __forceinline__ void tile_fft_reg_body(const PixSrc &ps, const RingDs
     }
     __syncthreads();
     const int first = (bid / nf) * TPW;  // the workgroup's first tile
-    const int ntile = min(TPW, tiles - first);
-    int slot = rd.first_slot + f;
-    slot -= (slot >= rd.ring_slots) ? rd.ring_slots : 0;
-    const long long gstride = (long long)rd.ring_slots * RING_GROUP;
-    for (int i = tid; i < ntile * TE; i += REG_NTH) {
-        const long long e = rd.elem_off + (long long)first * TE + i;  // the workgroup's tiles are consecutive
-        rd.ring[(e >> RING_GROUP_LOG2) * gstride + (long long)slot * RING_GROUP + (e & (RING_GROUP - 1))] = lds[i];
-    }
+    store_tiles<S>(rd, lds, f, first, min(TPW, tiles - first), tid, REG_NTH);
 }
 
 template <int S, int PIX>
@@ -544,16 +537,8 @@ __device__ __forceinline__ void tile_fft_lane_body(const PixSrc &ps, const RingD
         }
     }
     __syncthreads();
-    constexpr int TE = S * LS;  // spectrum elements per tile
-    const int first = (bid / nf) * TPG;
-    const int ntile = min(TPG, tiles - first);
-    int slot = rd.first_slot + f;
-    slot -= (slot >= rd.ring_slots) ? rd.ring_slots : 0;
-    const long long gstride = (long long)rd.ring_slots * RING_GROUP;
-    for (int i = tid; i < ntile * TE; i += SMALL_NTH) {
-        const long long e = rd.elem_off + (long long)first * TE + i;  // the workgroup's tiles are consecutive
-        rd.ring[(e >> RING_GROUP_LOG2) * gstride + (long long)slot * RING_GROUP + (e & (RING_GROUP - 1))] = lds[i];
-    }
+    const int first = (bid / nf) * TPG;  // the workgroup's tiles are consecutive
+    store_tiles<S>(rd, lds, f, first, min(TPG, tiles - first), tid, SMALL_NTH);
 }
 
 template <int S, int PIX, bool A4>
@@ -650,10 +635,12 @@ __device__ __forceinline__ void big_fft(float2 (&v)[BigCfg<S>::E1], float2 *seqb
 // column-block buffer xt: frame f, tile t, column block b = kx / 16, row ky,
 // column kx % 16 at xt[(((f * tiles + t) * H / 16 + b) * S + ky) * 16 + kx % 16],
 // so every 16 lanes of a row piece write 128 aligned bytes and the column
-// pass reads each block as one contiguous run.
+// pass reads each block as one contiguous run.  Only columns kx < xcols (a
+// multiple of 16: the blocks the column pass reads) are written.
 template <int S, int PIX>
 __device__ __forceinline__ void tile_fft_rows_reg_body(const PixSrc &ps, int tps, int tiles, float2 *__restrict__ xt,
-                                                       const float2 *__restrict__ twg, int bid, int nblk) {
+                                                       const float2 *__restrict__ twg, int xcols, int bid,
+                                                       int nblk) {
     using C = BigCfg<S>;
     constexpr int H = C::H, L = C::L, E1 = C::E1, SEQ = C::SEQ, GROUPS = H / SEQ;
     __shared__ float2 lds[SEQ * S];
@@ -686,21 +673,25 @@ __device__ __forceinline__ void tile_fft_rows_reg_body(const PixSrc &ps, int tps
             a = make_float2(zk.x, zh.x);
             b = make_float2(zk.y, zh.y);
         }
-        xa[(long long)m * (L / 16) * S * 16] = a;
-        xa[(long long)m * (L / 16) * S * 16 + 16] = b;
+        if (k < xcols) {
+            xa[(long long)m * (L / 16) * S * 16] = a;
+            xa[(long long)m * (L / 16) * S * 16 + 16] = b;
+        }
     }
 }
 
 template <int S, int PIX>
 __global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg(PixSrc ps, int tps, int tiles,
                                                                    float2 *__restrict__ xt,
-                                                                   const float2 *__restrict__ twg) {
-    tile_fft_rows_reg_body<S, PIX>(ps, tps, tiles, xt, twg, blockIdx.x, gridDim.x);
+                                                                   const float2 *__restrict__ twg, int xcols) {
+    tile_fft_rows_reg_body<S, PIX>(ps, tps, tiles, xt, twg, xcols, blockIdx.x, gridDim.x);
 }
 
 // Columns: SEQ consecutive packed columns per workgroup, read from the pass
 // buffer's column block and written to the ring slot; the packed column is
 // separated into the DC (kx = 0) and Nyquist (kx = H) columns on the way out.
+// In a column-order layout the workgroup's columns are one contiguous run of
+// ring elements; blocks of columns the layout leaves out end at once.
 template <int S>
 __device__ __forceinline__ void tile_fft_cols_reg_body(const RingDst &rd, int tiles, const float2 *__restrict__ xt,
                                                        const float2 *__restrict__ twg, int bid, int nblk) {
@@ -717,7 +708,8 @@ __device__ __forceinline__ void tile_fft_cols_reg_body(const RingDst &rd, int ti
     const int cb = (bid / nf) % NB;
     const int tile = bid / (nf * NB);
     const int c0 = cb * SEQ;
-    const TileOut t = tile_out(rd, f, tile, (long long)S * (H + 1));
+    if (c0 >= rd.lay.cols) return;  // the same for the whole workgroup
+    const TileOut t = tile_out(rd, f, tile, S);
     const int col = tid % SEQ, row = tid / SEQ;  // staging: SEQ consecutive columns of RP rows per pass
     {
         float2 v[E1];
@@ -744,19 +736,48 @@ __device__ __forceinline__ void tile_fft_cols_reg_body(const RingDst &rd, int ti
     float2 *seqbuf = lds + sq * (S + 1);
     big_fft<S>(v, seqbuf, q, twg);
     __syncthreads();
-    // column c's spectrum is at lds[c * (S + 1) + ky]; write rows of SEQ columns
+    // column c's spectrum is at lds[c * (S + 1) + ky]; the packed column's
+    // DC and Nyquist of row ky come from V[ky] and V[S - ky]
+    if (t.lay.col_major) {
+        // the layout's rows of columns c0 .. c0 + nc - 1, in ring element
+        // order: consecutive threads write consecutive elements
+        const int nr = t.lay.rows(), nc = min(SEQ, t.lay.cols - c0);
+        const float rnr = 1.0f / (float)nr;
+        const long long e0 = t.base + (long long)c0 * nr;
+        for (int i = tid; i < nc * nr; i += NTH) {
+            const int c = div_small(i, nr, rnr), r = i - c * nr;
+            const int ky = r < t.lay.row_lo ? r : r + (S - nr);
+            float2 val;
+            if (c0 + c == 0) {
+                const float2 w = lds[ky], pv = lds[(S - ky) & (S - 1)];
+                val = make_float2(0.5f * (w.x + pv.x), 0.5f * (w.y - pv.y));
+            } else {
+                val = lds[c * (S + 1) + ky];
+            }
+            t.ring[ring_index(e0 + i, t.slot, t.R)] = val;
+        }
+        if (c0 == 0 && t.lay.cols > H) {
+            for (int r = tid; r < nr; r += NTH) {
+                const int ky = r < t.lay.row_lo ? r : r + (S - nr);
+                const float2 w = lds[ky], pv = lds[(S - ky) & (S - 1)];
+                t.ring[ring_index(t.base + (long long)H * nr + r, t.slot, t.R)] =
+                    make_float2(0.5f * (w.y + pv.y), -0.5f * (w.x - pv.x));
+            }
+        }
+        return;
+    }
+    // row-order layout: rows of SEQ columns
     if (c0 == 0) {
-        // packed column: DC and Nyquist of row ky from V[ky] and V[S - ky]
         for (int ky = tid; ky < S; ky += NTH) {
             const float2 w = lds[ky], pv = lds[(S - ky) & (S - 1)];
-            t[(long long)ky * (H + 1)] = make_float2(0.5f * (w.x + pv.x), 0.5f * (w.y - pv.y));
-            t[(long long)ky * (H + 1) + H] = make_float2(0.5f * (w.y + pv.y), -0.5f * (w.x - pv.x));
+            t.put(ky, 0, make_float2(0.5f * (w.x + pv.x), 0.5f * (w.y - pv.y)));
+            t.put(ky, H, make_float2(0.5f * (w.y + pv.y), -0.5f * (w.x - pv.x)));
         }
     }
 #pragma unroll
     for (int k = 0; k < E1; ++k) {
         const int ky = row + RP * k;
-        if (c0 + col != 0) t[(long long)ky * (H + 1) + c0 + col] = lds[col * (S + 1) + ky];
+        if (c0 + col != 0) t.put(ky, c0 + col, lds[col * (S + 1) + ky]);
     }
 }
 
@@ -772,14 +793,15 @@ static void launch_big(const PixSrc &ps, const RingDst &rd, int tps, int tiles, 
                        float2 *xt, hipStream_t st) {
     using C = BigCfg<S>;
     const dim3 g1((unsigned)((C::H / C::SEQ) * tiles * n_frames)), block(C::NTH);
+    const int xcols = rd.lay.pass_cols(S);
     switch (pixel_mode(ps)) {
     case PIX_U8_1:
-        hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_U8_1>), g1, block, 0, st, ps, tps, tiles, xt, tw);
+        hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_U8_1>), g1, block, 0, st, ps, tps, tiles, xt, tw, xcols);
         break;
     case PIX_U16LE_2:
-        hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_U16LE_2>), g1, block, 0, st, ps, tps, tiles, xt, tw);
+        hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_U16LE_2>), g1, block, 0, st, ps, tps, tiles, xt, tw, xcols);
         break;
-    default: hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_ANY>), g1, block, 0, st, ps, tps, tiles, xt, tw);
+    default: hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_ANY>), g1, block, 0, st, ps, tps, tiles, xt, tw, xcols);
     }
     const dim3 g2((unsigned)((C::H / C::SEQ_C) * tiles * n_frames)), block2(C::NTH_C);
     hipLaunchKernelGGL(tile_fft_cols_reg<S>, g2, block2, 0, st, rd, tiles, xt, tw);
@@ -817,7 +839,8 @@ __global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg_batch(const 
                                                                          const float2 *__restrict__ twg) {
     const long long b = blockIdx.x;
     const FftJob &j = jobs[batch_job(jobs, hdr->njobs, b)];
-    tile_fft_rows_reg_body<S, PIX>(j.ps, j.tps, j.tiles, j.xt, twg, (int)(b - j.first), j.nblk);
+    tile_fft_rows_reg_body<S, PIX>(j.ps, j.tps, j.tiles, j.xt, twg, j.rd.lay.pass_cols(S), (int)(b - j.first),
+                                   j.nblk);
 }
 
 template <int S>

@@ -19,6 +19,38 @@ namespace ddm {
 
 static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
+static std::shared_ptr<const std::vector<int>> ring_members(int s, float lambda, float tol);
+
+// The part of each s x s tile's half spectrum an engine's ring holds.  The
+// 256 .. 1024 transforms end in a column pass, so their tiles are stored
+// column after column (each column-pass workgroup then writes one contiguous
+// run); the others row after row.  With DDM_SPECTRUM_MASKED the rows and
+// columns shrink to the band holding every member of the scale's q rings
+// when that saves at least a tenth of the elements (8 .. 1024 tiles: the
+// 2 / 4 / 2048 transforms write the full spectrum).
+static TileLayout kept_layout(int s, const Params &p) {
+    TileLayout full = TileLayout::full(s);
+    const bool two_pass = s >= 256 && s <= 1024;
+    if (two_pass) full.col_major = 1;
+    if (p.spectrum != DDM_SPECTRUM_MASKED || s < 8 || s > 1024) return full;
+    const int H = s / 2;
+    int lo = 0, hi = 0, cols = 0;  // rows ky < lo and ky >= s - hi, columns kx < cols
+    for (float lam : p.lambdas) {
+        auto m = ring_members(s, lam, p.tol);
+        if (!m) return full;
+        for (int li : *m) {
+            const int ky = li / (H + 1), kx = li % (H + 1);
+            cols = std::max(cols, kx + 1);
+            if (ky < H)
+                lo = std::max(lo, ky + 1);
+            else
+                hi = std::max(hi, s - ky);
+        }
+    }
+    const TileLayout band{std::max(cols, 1), std::max(lo, 1), hi, two_pass ? 1 : 0};
+    return band.elems() * 10 > full.elems() * 9 ? full : band;
+}
+
 int check_desc(const ddm_desc *d, Params *out, std::string *msg) {
     auto fail = [&](int code, const char *text) {
         if (msg) *msg = text;
@@ -40,6 +72,8 @@ int check_desc(const ddm_desc *d, Params *out, std::string *msg) {
         return fail(DDM_EINVAL, "unknown normalisation mode");
     if (d->accumulation != DDM_ACC_COMPENSATED && d->accumulation != DDM_ACC_PLAIN)
         return fail(DDM_EINVAL, "unknown accumulation mode");
+    if (d->spectrum != DDM_SPECTRUM_MASKED && d->spectrum != DDM_SPECTRUM_FULL)
+        return fail(DDM_EINVAL, "unknown spectrum mode");
     if (d->lag_block != 0 && d->lag_block != 32 && d->lag_block != 64 && d->lag_block != 128 &&
         d->lag_block != 256)
         return fail(DDM_EINVAL, "lag block must be 0 (automatic), 32, 64, 128 or 256");
@@ -58,6 +92,7 @@ int check_desc(const ddm_desc *d, Params *out, std::string *msg) {
     p.norm_mode = d->norm_mode;
     p.lag_block = d->lag_block ? d->lag_block : kDefaultLagBlock;
     p.acc_mode = d->accumulation;
+    p.spectrum = d->spectrum;
 
     // src/DDM.cu:319-321
     std::sort(p.lags.begin(), p.lags.end());
@@ -113,7 +148,8 @@ int check_desc(const ddm_desc *d, Params *out, std::string *msg) {
         g.s = s;
         g.tps = p.M / s;
         g.tiles = g.tps * g.tps;
-        g.tile_elems = (long long)s * (s / 2 + 1);
+        g.lay = kept_layout(s, p);
+        g.tile_elems = g.lay.elems();
         g.elem_off = off;
         off += g.tile_elems * g.tiles;
         results += (long long)g.tiles * (long long)p.lambdas.size() * (long long)p.lags.size();
@@ -177,13 +213,20 @@ int build_ring_tables(const Params &p, RingTables *out) {
     out->off.assign(p.geom.size() * Q, 0);
     out->cnt.assign(p.geom.size() * Q, 0);
     for (size_t si = 0; si < p.geom.size(); ++si) {
-        const int s = p.geom[si].s;
+        const int s = p.geom[si].s, H = s / 2;
+        const TileLayout &lay = p.geom[si].lay;
         for (int q = 0; q < Q; ++q) {
             auto members = ring_members(s, p.lambdas[q], p.tol);
             if (!members) return DDM_EINVAL;
             out->off[si * Q + q] = (int)out->idx.size();
             out->cnt[si * Q + q] = (int)members->size();
-            out->idx.insert(out->idx.end(), members->begin(), members->end());
+            // each member's place in the tile's stored layout, in member
+            // order (the averages sum in list order, whatever the layout)
+            for (int li : *members) {
+                const long long at = lay.pos(s, li / (H + 1), li % (H + 1));
+                if (at < 0) return DDM_EINVAL;
+                out->idx.push_back((int)at);
+            }
         }
     }
     return DDM_OK;

@@ -22,17 +22,46 @@ constexpr int RING_GROUP_LOG2 = 6;
 constexpr int RING_GROUP = 1 << RING_GROUP_LOG2;
 inline long long pad_elems(long long e) { return (e + RING_GROUP - 1) / RING_GROUP * RING_GROUP; }
 
-// Complex values of the tile FFT's between-pass buffer (TileFftArgs::xt) a
-// batch of n frames needs for tiles of size `scale` cut from an M x M region
-// (0 for scales that do not use it).
 // Device buffers host frames are uploaded into, used in turn.
 constexpr int UPLOAD_STAGES = 3;
 // At most this many (a batched server that groups chunks gives its streams more).
 constexpr int MAX_UPLOAD_STAGES = 8;
 
+// Complex values of the tile FFT's between-pass buffer (TileFftArgs::xt) a
+// batch of n frames needs for tiles of size `scale` cut from an M x M region
+// (0 for scales that do not use it).
 inline long long big_fft_scratch_elems(int M, int scale, int n) {
     return (scale == 256 || scale == 512 || scale == 1024) ? (long long)n * M * M / 2 : 0;
 }
+
+// The part of an s x s tile's half spectrum [ky][kx <= s/2] a spectrum ring
+// holds, and its order: rows ky < row_lo and ky >= s - row_hi (the low
+// |ky| band, both signs), columns kx < cols, stored row after row
+// ([row][kx]) or, with col_major, column after column ([kx][row]).  Only
+// elements some q ring contains reach the averages, so an engine keeps the
+// smallest such band holding every ring member; the full spectrum in row
+// order is full(s).
+struct TileLayout {
+    int cols = 0, row_lo = 0, row_hi = 0, col_major = 0;
+    static constexpr TileLayout full(int s) { return TileLayout{s / 2 + 1, s, 0, 0}; }
+    constexpr int rows() const { return row_lo + row_hi; }
+    constexpr long long elems() const { return (long long)cols * (row_lo + row_hi); }
+    constexpr bool is_full(int s) const { return !col_major && cols == s / 2 + 1 && row_lo == s; }
+    // position of (ky, kx) inside the tile, or -1 when the layout leaves it out
+    constexpr long long pos(int s, int ky, int kx) const {
+        const int nr = row_lo + row_hi;
+        if (kx >= cols || (ky >= row_lo && ky < s - row_hi)) return -1;
+        const int r = ky < row_lo ? ky : ky - (s - nr);
+        return col_major ? (long long)kx * nr + r : (long long)r * cols + kx;
+    }
+    // row-pass columns the column pass reads (256 .. 1024 tiles): whole
+    // blocks of 16 up to cols, at most s/2 (the packed DC/Nyquist column
+    // is column 0)
+    constexpr int pass_cols(int s) const {
+        const int c = (cols + 15) / 16 * 16;
+        return c < s / 2 ? c : s / 2;
+    }
+};
 
 // Frames of device-resident input (or of one large host call) transformed
 // per FFT batch by default: whole chunks, up to about 128 frames.  A 30-frame
@@ -54,8 +83,9 @@ struct ScaleGeom {
     int s = 0;            // tile size
     int tps = 0;          // tiles per side, M / s
     int tiles = 0;        // tps * tps
-    long long tile_elems = 0;  // s * (s/2 + 1)
+    long long tile_elems = 0;  // lay.elems(): s * (s/2 + 1) for the full spectrum
     long long elem_off = 0;    // offset of this scale inside one frame's spectrum
+    TileLayout lay;            // the part of each tile's spectrum the ring holds
 };
 
 // Descriptor after validation and sorting: scales descending, lags and
@@ -71,6 +101,7 @@ struct Params {
     int norm_mode = DDM_NORM_FRAMES;
     int lag_block = 0;
     int acc_mode = DDM_ACC_COMPENSATED;
+    int spectrum = DDM_SPECTRUM_MASKED;
     // derived
     int M = 0;
     int max_lag = 0;
@@ -91,7 +122,8 @@ int check_desc(const ddm_desc *d, Params *out, std::string *msg);
 size_t device_footprint(const Params &p);
 
 // Element lists of every q ring of every scale: for scale i and q j the
-// member elements (ky * (s/2+1) + kx, ascending) are
+// member elements (ky * (s/2+1) + kx, ascending), as places in the scale's
+// stored tile layout (ScaleGeom::lay), are
 // idx[off[i*Q + j] .. off[i*Q + j] + cnt[i*Q + j]).
 struct RingTables {
     std::vector<int> idx;

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ddm_amd.h"
+#include "ddm_internal.h"
 
 namespace ddm {
 
@@ -25,6 +26,9 @@ struct TileFftArgs {
     // row and column passes (big_fft_scratch_elems values); without it the
     // row spectra go through the ring slot
     float *xt = nullptr;
+    // the part of each tile's spectrum the ring holds; cols == 0: the full
+    // spectrum in row order
+    TileLayout lay{};
 };
 
 

@@ -243,7 +243,7 @@ __global__ __launch_bounds__(SmallCfg<S>::NTH) void tile_fft_small(PixSrc ps, Ri
         } else {
             val = buf[lp(y0 + ky * H + kx)];
         }
-        tile_out(rd, f, tile, OUT)[rem] = val;
+        tile_out(rd, f, tile, S).put(ky, kx, val);
     }
 }
 
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(NT) void tile_fft_rows(PixSrc ps, RingDst rd, int t
     __syncthreads();
     using Rows = Layout<S, P, P, S, 0, 1, false>;
     fft_passes<Rows, 1, NT>(buf, tw);
-    const TileOut out = tile_out(rd, f, tile, (long long)S * (H + 1));
+    const TileOut out = tile_out(rd, f, tile, S);  // the full row-order layout (the host checks)
     for (int i = tid; i < P * H; i += NT) {
         const int r = i / H, k = i % H;
         const float2 za = buf[lp(r * S + k)];
@@ -305,8 +305,8 @@ __global__ __launch_bounds__(NT) void tile_fft_rows(PixSrc ps, RingDst rd, int t
             yb = make_float2(0.5f * (za.y + zb.y), -0.5f * (za.x - zb.x));
         }
         const int ly = 2 * (g * P + r);
-        out[(long long)ly * (H + 1) + k] = ya;
-        out[(long long)(ly + 1) * (H + 1) + k] = yb;
+        out.full(ly, k) = ya;
+        out.full(ly + 1, k) = yb;
     }
 }
 
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(LargeCfg<S>::NTH_COLS) void tile_fft_cols(RingDst r
     const int cb = (blockIdx.x / nf) % NB;
     const int tile = blockIdx.x / (nf * NB);
     for (int i = tid; i < S; i += NT) tw[i] = twg[S + i];
-    const TileOut t = tile_out(rd, f, tile, (long long)S * (H + 1));
+    const TileOut t = tile_out(rd, f, tile, S);  // the full row-order layout (the host checks)
     const int c0 = cb * CB;
     {
         constexpr int KL = S * CB / NT;  // S * CB = 8192 = 8 * NT
@@ -333,7 +333,7 @@ __global__ __launch_bounds__(LargeCfg<S>::NTH_COLS) void tile_fft_cols(RingDst r
         for (int k = 0; k < KL; ++k) {
             const int i = tid + k * NT;
             const int ly = i / CB, c = i % CB;
-            v[k] = t[(long long)ly * (H + 1) + c0 + c];
+            v[k] = t.full(ly, c0 + c);
         }
 #pragma unroll
         for (int k = 0; k < KL; ++k) buf[lp(tid + k * NT)] = v[k];
@@ -346,10 +346,10 @@ __global__ __launch_bounds__(LargeCfg<S>::NTH_COLS) void tile_fft_cols(RingDst r
         if (c0 + c == 0) {
             const float2 w = buf[lp(ky * CB)];
             const float2 v = buf[lp(((S - ky) & (S - 1)) * CB)];
-            t[(long long)ky * (H + 1)] = make_float2(0.5f * (w.x + v.x), 0.5f * (w.y - v.y));
-            t[(long long)ky * (H + 1) + H] = make_float2(0.5f * (w.y + v.y), -0.5f * (w.x - v.x));
+            t.full(ky, 0) = make_float2(0.5f * (w.x + v.x), 0.5f * (w.y - v.y));
+            t.full(ky, H) = make_float2(0.5f * (w.y + v.y), -0.5f * (w.x - v.x));
         } else {
-            t[(long long)ky * (H + 1) + c0 + c] = buf[lp(i)];
+            t.full(ky, c0 + c) = buf[lp(i)];
         }
     }
 }
@@ -558,12 +558,16 @@ hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st) {
     if (e != hipSuccess) return e;
     PixSrc ps{(const uint8_t *)a.frames, a.frame_bytes, (long long)a.width * a.bpp, a.bpp, a.fmt, a.x_off,
               a.y_off};
-    RingDst rd{(float2 *)a.ring, a.elem_off, a.ring_slots, a.first_slot};
+    const TileLayout lay = a.lay.cols ? a.lay : TileLayout::full(a.scale);
+    RingDst rd{(float2 *)a.ring, a.elem_off, a.ring_slots, a.first_slot, lay};
     const int tps = a.largest_scale / a.scale, tiles = tps * tps;
     if (a.precision != DDM_PREC_FP32 && (a.scale == 64 || a.scale == 128))
         return launch_tile_dft_mfma(ps, rd, a.scale, tps, tiles, a.n_frames, a.precision, st);
     if (launch_tile_fft_reg(ps, rd, a.scale, tps, tiles, a.n_frames, tw, (float2 *)a.xt, st))
         return hipGetLastError();
+    // the 256 .. 2048 transforms below go through the ring slot in place:
+    // full row-order layout only
+    if (a.scale >= 256 && !lay.is_full(a.scale)) return hipErrorInvalidValue;
     switch (a.scale) {
     case 2: launch_small<2>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
     case 4: launch_small<4>(ps, rd, tps, tiles, a.n_frames, tw, st); break;

@@ -225,6 +225,9 @@ void runDDM(std::string file_in, std::string file_out, int *tau_arr, int tau_cou
     d.chunk_frames = chunk_frame_count;
     d.norm_mode = DDM_NORM_FRAMES;
     d.lag_block = 0;
+    // per-element output (-K) and the spectrum dump (-A) read every element;
+    // the averages alone need only the q-ring band
+    d.spectrum = (nonAveOutput || FFTOutput) ? DDM_SPECTRUM_FULL : DDM_SPECTRUM_MASKED;
 
     // every parameter check happens before the device is touched
     char msg[512];

@@ -312,7 +312,7 @@ static int run_tick(ddm_server *s, const std::vector<Taken> &taken) {
             xt_off += xt_need;
         }
         for (const ddm::ScaleGeom &gm : p.geom) {
-            ddm::RingDst rd{(float2 *)bs.ring, gm.elem_off, R, cslot};
+            ddm::RingDst rd{(float2 *)bs.ring, gm.elem_off, R, cslot, gm.lay};
             tmp.clear();
             if (bs.precision == DDM_PREC_FP32 && ddm::fft_batch_jobs(ps, rd, gm.s, gm.tps, gm.tiles, c.n, xt, &tmp)) {
                 for (auto &kj : tmp) fft[kj.first].push_back(kj.second);
@@ -332,6 +332,7 @@ static int run_tick(ddm_server *s, const std::vector<Taken> &taken) {
                 a.first_slot = cslot;
                 a.elems_per_frame = p.elems_padded;
                 a.elem_off = gm.elem_off;
+                a.lay = gm.lay;
                 a.precision = bs.precision;
                 a.xt = (float *)xt;
                 direct_fft.push_back(a);

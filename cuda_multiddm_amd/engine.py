@@ -29,6 +29,7 @@ PIX_U8, PIX_U16LE, PIX_U16BE = 0, 1, 2
 NORM_FRAMES, NORM_PAIRS = 0, 1
 PREC_FP32, PREC_BF16X2, PREC_FP16X2 = 0, 1, 2
 ACC_COMPENSATED, ACC_PLAIN = 0, 1
+SPECTRUM_MASKED, SPECTRUM_FULL = 0, 1
 SERVER_BATCHED, SERVER_NO_GRAPH, SERVER_GROUP = 1, 2, 4
 
 # Every symbol include/ddm_amd.h declares.
@@ -60,6 +61,7 @@ class Desc(ctypes.Structure):
         ("lambdas", ctypes.POINTER(ctypes.c_float)), ("lambda_count", ctypes.c_int),
         ("mask_tolerance", ctypes.c_float), ("chunk_frames", ctypes.c_int),
         ("norm_mode", ctypes.c_int), ("lag_block", ctypes.c_int), ("accumulation", ctypes.c_int),
+        ("spectrum", ctypes.c_int),
     ]
 
 
@@ -169,14 +171,15 @@ def _arr(ctype, values):
 
 
 def make_desc(width, height, scales, lags, lambdas, tolerance=1.2, chunk=30, bpp=1, pixfmt=PIX_U8,
-              x_off=0, y_off=0, norm=NORM_FRAMES, lag_block=0, accumulation=ACC_COMPENSATED):
+              x_off=0, y_off=0, norm=NORM_FRAMES, lag_block=0, accumulation=ACC_COMPENSATED,
+              spectrum=SPECTRUM_MASKED):
     """Builds a ddm_desc; the returned tuple keeps the parameter arrays alive."""
     sc = _arr(ctypes.c_int, [int(s) for s in scales])
     lg = _arr(ctypes.c_int, [int(t) for t in lags])
     lm = _arr(ctypes.c_float, [float(x) for x in lambdas])
     d = Desc(int(width), int(height), int(bpp), int(pixfmt), int(x_off), int(y_off), sc, len(scales), lg,
              len(lags), lm, len(lambdas), ctypes.c_float(tolerance).value, int(chunk), int(norm), int(lag_block),
-             int(accumulation))
+             int(accumulation), int(spectrum))
     return d, (sc, lg, lm)
 
 
@@ -193,11 +196,12 @@ class Engine:
     """Streaming DDM analysis on the current HIP device."""
 
     def __init__(self, width, height, scales, lags, lambdas, tolerance=1.2, chunk=30, bpp=1,
-                 pixfmt=PIX_U8, x_off=0, y_off=0, norm=NORM_FRAMES, lag_block=0, accumulation=ACC_COMPENSATED):
+                 pixfmt=PIX_U8, x_off=0, y_off=0, norm=NORM_FRAMES, lag_block=0, accumulation=ACC_COMPENSATED,
+                 spectrum=SPECTRUM_MASKED):
         self._h = None
         L = lib()
         d, self._keep = make_desc(width, height, scales, lags, lambdas, tolerance, chunk, bpp, pixfmt, x_off,
-                                  y_off, norm, lag_block, accumulation)
+                                  y_off, norm, lag_block, accumulation, spectrum)
         h = ctypes.c_void_p()
         rc = L.ddm_create(ctypes.byref(d), ctypes.byref(h))
         if rc != DDM_OK:
@@ -368,11 +372,12 @@ class Server:
         return rc
 
     def open(self, width, height, scales, lags, lambdas, tolerance=1.2, chunk=30, bpp=1, pixfmt=PIX_U8,
-             x_off=0, y_off=0, norm=NORM_FRAMES, lag_block=0, accumulation=ACC_COMPENSATED) -> int:
+             x_off=0, y_off=0, norm=NORM_FRAMES, lag_block=0, accumulation=ACC_COMPENSATED,
+             spectrum=SPECTRUM_MASKED) -> int:
         """Returns the stream id; raises DDMError with code DDM_EBUSY when the
         stream does not fit yet."""
         d, keep = make_desc(width, height, scales, lags, lambdas, tolerance, chunk, bpp, pixfmt, x_off, y_off,
-                            norm, lag_block, accumulation)
+                            norm, lag_block, accumulation, spectrum)
         sid = ctypes.c_int()
         self._check(lib().ddm_server_open(self._h, ctypes.byref(d), ctypes.byref(sid)))
         self._keep[sid.value] = keep

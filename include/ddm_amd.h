@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define DDM_ABI_VERSION 2
+#define DDM_ABI_VERSION 3
 
 /* status codes */
 #define DDM_OK 0
@@ -83,6 +83,18 @@ enum ddm_accumulation { DDM_ACC_COMPENSATED = 0, DDM_ACC_PLAIN = 1 };
  * FP64 (DDM_ACC_COMPENSATED). */
 #define DDM_ACC_FOLD_PAIRS 10000
 
+/* Which spectrum elements an engine keeps (ddm_desc.spectrum).
+ * DDM_SPECTRUM_MASKED (default): only the I(q, tau) averages read the lag
+ * sums, and they read only elements inside some q ring, so for each scale
+ * the ring and accumulators hold the smallest band of a tile's half
+ * spectrum (rows |ky| <= a, columns kx <= b) containing every ring member,
+ * when that saves at least a tenth of the elements (scales 8 .. 1024).
+ * Results are bit-identical to DDM_SPECTRUM_FULL; per-element reads
+ * (ddm_get_accumulators, ddm_get_spectra) return NaN for elements left
+ * out.  DDM_SPECTRUM_FULL: every element, as the reference computes them
+ * (needed for its non-averaged -K output and the -A spectrum dump). */
+enum ddm_spectrum { DDM_SPECTRUM_MASKED = 0, DDM_SPECTRUM_FULL = 1 };
+
 typedef struct ddm_desc {
     int width;            /* frame width in pixels */
     int height;           /* frame height in pixels */
@@ -100,6 +112,7 @@ typedef struct ddm_desc {
     int norm_mode;        /* enum ddm_norm_mode */
     int lag_block;        /* frames per lag-engine pass: 0 = automatic (256), else 32, 64, 128 or 256 */
     int accumulation;     /* enum ddm_accumulation (0 = compensated with FP64 folding) */
+    int spectrum;         /* enum ddm_spectrum (0 = keep the q-ring band only) */
 } ddm_desc;
 
 /* Sizes the engine derived from a descriptor (after sorting). */

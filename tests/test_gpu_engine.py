@@ -184,7 +184,7 @@ def test_results_mid_stream(gpu):
 def test_spectra_and_accumulators(gpu):
     frames = ddm.synth_uniform(100, 64, 64, 8, seed=6)
     lags = [1, 3, 8]
-    with ddm.Engine(64, 64, [64, 32], lags, [4.0, 8.0], 1.2, 8, lag_block=32) as e:
+    with ddm.Engine(64, 64, [64, 32], lags, [4.0, 8.0], 1.2, 8, lag_block=32, spectrum=ddm.SPECTRUM_FULL) as e:
         e.submit_host(frames)
         assert e.info().ring_slots == 8 + 32 + 8
         for fr in (99, 60):

@@ -29,7 +29,7 @@ def test_mfma_tile_spectra(gpu, mode, bits):
     frames = ddm.synth_brownian(3, M, M, bits, n_spots=300, amplitude=80.0 if bits == 8 else 1500.0,
                                 background=100.0 if bits == 8 else 1000.0, noise=5.0, seed=bits + mode)
     bpp, fmt = (1, ddm.PIX_U8) if bits == 8 else (2, ddm.PIX_U16LE)
-    with ddm.Engine(M, M, [256, 128, 64], [1, 2], [8.0], 1.2, 8, bpp, fmt) as e:
+    with ddm.Engine(M, M, [256, 128, 64], [1, 2], [8.0], 1.2, 8, bpp, fmt, spectrum=ddm.SPECTRUM_FULL) as e:
         e.set_precision(mode)
         e.submit_host(frames.reshape(-1).view(np.uint8), 3)
         e.sync()

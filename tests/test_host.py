@@ -36,13 +36,37 @@ def _v(**kw):
     return ddm.validate(**base)
 
 
+def _band(s, lambdas, tol):
+    """Elements per tile a masked engine keeps for scale s: the rows |ky| and
+    columns kx spanned by the union of the oracle's q-ring masks, or the whole
+    half spectrum when that saves less than a tenth (or s is outside 8..1024)."""
+    full = s * (s // 2 + 1)
+    if s < 8 or s > 1024:
+        return full
+    union = np.zeros((s, s // 2 + 1), dtype=bool)
+    for lam in lambdas:
+        union |= o.mask(s, lam, tol).astype(bool)
+    ky, kx = np.nonzero(union)
+    if len(ky) == 0:
+        return 1
+    lo = int(ky[ky < s // 2].max()) + 1 if np.any(ky < s // 2) else 1
+    hi = int(s - ky[ky >= s // 2].min()) if np.any(ky >= s // 2) else 0
+    band = (lo + hi) * (int(kx.max()) + 1)
+    return full if band * 10 > full * 9 else band
+
+
 def test_validate_accepts_config1_and_sorts():
-    rc, msg, info = _v()
+    rc, msg, info = _v(spectrum=ddm.SPECTRUM_FULL)
     assert rc == 0 and msg == ""
     # ring: one chunk + one lag block (256 by default) + the largest lag
     assert info.largest_scale == 128 and info.lag_block == 256 and info.ring_slots == 30 + 256 + 16
     assert info.elems_per_frame == 128 * 65 and info.elems_per_frame_padded == 8320
     assert info.result_count == 20 * 16
+    # masked (the default): rows |ky| <= 38 and columns kx <= 38 hold every
+    # member of the rings q = 128 / lambda .. 1.2 q, lambda >= 4
+    rc, msg, info = _v()
+    assert rc == 0 and info.elems_per_frame == 77 * 39 == _band(128, ddm.lin_lambdas(4, 64, 20), 1.2)
+    assert info.elems_per_frame_padded == 3008 and info.result_count == 20 * 16
     rc, _, info = _v(width=1100, height=1030, scales=[256, 1024, 512], lags=[30, 1, 2],
                      lambdas=[16.0, 1.0, 3.0])
     assert rc == 0 and info.largest_scale == 1024
@@ -50,6 +74,34 @@ def test_validate_accepts_config1_and_sorts():
     assert info.ring_slots == 30 + 256 + 30
     rc, _, info = _v(lag_block=128, chunk=7)
     assert rc == 0 and info.lag_block == 128 and info.ring_slots == 7 + 128 + 16
+
+
+@pytest.mark.parametrize("name", ["uniform128", "brownian512", "brownian2048_u16", "serving"])
+def test_masked_spectrum_sizes_of_the_workloads(name):
+    """Elements per frame a masked engine keeps, per workload, against the band
+    of the oracle's q-ring masks; the full-spectrum size is unchanged."""
+    from cuda_multiddm_amd.workloads import WORKLOADS, serving_streams
+    if name == "serving":
+        descs = [s.desc() for s in serving_streams(6)]
+    else:
+        w = WORKLOADS[name]
+        descs = [dict(width=w.size, height=w.size, scales=w.scales, lags=w.lags, lambdas=w.lambdas,
+                      tolerance=1.2, chunk=30)]
+    for d in descs:
+        rc, _, masked = ddm.validate(**d)
+        rc2, _, full = ddm.validate(spectrum=ddm.SPECTRUM_FULL, **d)
+        assert rc == 0 and rc2 == 0
+        M = max(d["scales"])
+        tol = float(np.float32(d.get("tolerance", 1.2)))
+        lams = [float(np.float32(x)) for x in d["lambdas"]]
+        assert full.elems_per_frame == sum((M // s) ** 2 * s * (s // 2 + 1) for s in d["scales"])
+        assert masked.elems_per_frame == sum((M // s) ** 2 * _band(s, lams, tol) for s in d["scales"])
+        print(name, M, masked.elems_per_frame / full.elems_per_frame)
+
+
+def test_validate_rejects_unknown_spectrum_mode():
+    rc, msg, _ = _v(spectrum=2)
+    assert rc == -1 and msg == "unknown spectrum mode"
 
 
 @pytest.mark.parametrize("kw,code,text", [
