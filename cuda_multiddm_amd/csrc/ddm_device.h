@@ -59,18 +59,22 @@ __device__ __forceinline__ int div_small(int n, int d, float rcp) {
 }
 
 // Writes the elements of tile `tile` of batch frame f into the ring,
-// addressed by (ky, kx) of the tile's half spectrum; elements the layout
-// leaves out are dropped.
+// addressed by (ky, kx) of the tile's half spectrum, for the row-order
+// layouts (LAY_FULL, LAY_ROWS); elements the layout leaves out are dropped.
 struct TileOut {
     float2 *ring;
     long long base;  // element index of the tile's first element
     int slot, R, S;
     TileLayout lay;
     __device__ __forceinline__ void put(int ky, int kx, float2 v) const {
-        const long long at = lay.pos(S, ky, kx);
-        if (at >= 0) ring[ring_index(base + at, slot, R)] = v;
+        if (lay.kind == LAY_FULL) {
+            ring[ring_index(base + ky * (S / 2 + 1) + kx, slot, R)] = v;
+            return;
+        }
+        const int a = lay.start[ky];
+        if (kx < lay.start[ky + 1] - a) ring[ring_index(base + a + kx, slot, R)] = v;
     }
-    // element (ky, kx) of a tile in the full row-order layout
+    // element (ky, kx) of a tile in the LAY_FULL layout
     __device__ __forceinline__ float2 &full(int ky, int kx) const {
         return ring[ring_index(base + (long long)ky * (S / 2 + 1) + kx, slot, R)];
     }
@@ -82,36 +86,70 @@ __device__ __forceinline__ int ring_slot(const RingDst &rd, int f) {
 }
 
 __device__ __forceinline__ TileOut tile_out(const RingDst &rd, int f, int tile, int S) {
-    return TileOut{rd.ring, rd.elem_off + (long long)tile * rd.lay.elems(), ring_slot(rd, f), rd.ring_slots, S,
+    return TileOut{rd.ring, rd.elem_off + (long long)tile * rd.lay.elems, ring_slot(rd, f), rd.ring_slots, S,
                    rd.lay};
+}
+
+// The largest k < n with tab[k] <= x, for tab ascending in LDS and
+// tab[0] <= x: the row (column) of kept element x when tab holds the row
+// (column) starts; empty rows share their start with the next row and are
+// skipped.
+__device__ __forceinline__ int find_start(const int *tab, int n, int x) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid] <= x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Copies the row starts of a LAY_ROWS layout (S + 1 entries) into LDS at
+// the start of a tile kernel, so that its store loop (after a barrier) does
+// not wait for them.
+template <int S>
+__device__ __forceinline__ void load_row_starts(const RingDst &rd, int *rtab, int tid, int nth) {
+    if (rd.lay.kind == LAY_ROWS)
+        for (int i = tid; i <= S; i += nth) rtab[i] = rd.lay.start[i];
 }
 
 // The spectra of ntile consecutive tiles (first, first + 1, ...) of batch
 // frame f, staged in LDS at lds[t * S * (S/2 + 1) + ky * (S/2 + 1) + kx],
-// stored to the ring in the row-order layout rd.lay (full or a band) by nth
-// threads: consecutive threads write consecutive ring elements.
+// stored to the ring in the row-order layout rd.lay (LAY_FULL or LAY_ROWS,
+// whose row starts rtab holds in LDS) by nth threads (a multiple of 64):
+// consecutive lanes write consecutive ring elements.
 template <int S>
-__device__ __forceinline__ void store_tiles(const RingDst &rd, const float2 *lds, int f, int first, int ntile,
-                                            int tid, int nth) {
-    constexpr int H = S / 2, TE = S * (H + 1);
+__device__ __forceinline__ void store_tiles(const RingDst &rd, const float2 *lds, const int *rtab, int f,
+                                            int first, int ntile, int tid, int nth) {
+    constexpr int TE = S * (S / 2 + 1);
     const int slot = ring_slot(rd, f);
     const long long gstride = (long long)rd.ring_slots * RING_GROUP;
-    if (rd.lay.is_full(S)) {
+    const int CE = rd.lay.elems;
+    const long long e0 = rd.elem_off + (long long)first * CE;
+    if (rd.lay.kind == LAY_FULL) {
         for (int i = tid; i < ntile * TE; i += nth) {
-            const long long e = rd.elem_off + (long long)first * TE + i;
+            const long long e = e0 + i;
             rd.ring[(e >> RING_GROUP_LOG2) * gstride + (long long)slot * RING_GROUP + (e & (RING_GROUP - 1))] = lds[i];
         }
         return;
     }
-    const int cols = rd.lay.cols, nr = rd.lay.rows(), CE = cols * nr;
-    const float rce = 1.0f / (float)CE, rcols = 1.0f / (float)cols;
-    for (int i = tid; i < ntile * CE; i += nth) {
-        const int t = div_small(i, CE, rce), ci = i - t * CE;
-        const int r = div_small(ci, cols, rcols), kx = ci - r * cols;
-        const int ky = r < rd.lay.row_lo ? r : r + (S - nr);
-        const long long e = rd.elem_off + (long long)first * CE + i;
-        rd.ring[(e >> RING_GROUP_LOG2) * gstride + (long long)slot * RING_GROUP + (e & (RING_GROUP - 1))] =
-            lds[t * TE + ky * (H + 1) + kx];
+    // LAY_ROWS: row_lanes lanes per row (64 / row_lanes rows per wave), each
+    // row's kept columns a contiguous run
+    const int lpr = rd.lay.row_lanes, rpw = 64 / lpr;
+    const int lane = tid & 63, sub = lane / lpr, kl = lane - sub * lpr;
+    const int step = (nth >> 6) * rpw;
+    for (int j = (tid >> 6) * rpw + sub; j < ntile * S; j += step) {
+        const int t = j / S, ky = j % S;
+        const int a = rtab[ky], n = rtab[ky + 1] - a;
+        const long long eb = e0 + (long long)t * CE + a;
+        const float2 *src = lds + t * TE + ky * (S / 2 + 1);
+        for (int kx = kl; kx < n; kx += lpr) {
+            const long long e = eb + kx;
+            rd.ring[(e >> RING_GROUP_LOG2) * gstride + (long long)slot * RING_GROUP + (e & (RING_GROUP - 1))] =
+                src[kx];
+        }
     }
 }
 

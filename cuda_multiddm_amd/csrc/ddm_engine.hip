@@ -61,6 +61,8 @@ struct ddm_engine {
     int az_s0 = 0;                  // first narrow-ring scale
     long long az_narrow_waves = 0;
     int *d_qoff = nullptr, *d_qcnt = nullptr, *d_idx = nullptr;
+    int *d_lay = nullptr;              // tile layout tables of every scale
+    std::vector<ddm::TileLayout> lay;  // per scale, pointing into d_lay
     double *d_norm = nullptr, *d_out = nullptr;
     double *d_tau = nullptr, *d_fit = nullptr;  // allocated by the first ddm_fit_results
     float *d_xt = nullptr;    // row spectra of 256 / 512 tiles between FFT passes
@@ -170,7 +172,7 @@ static void release(ddm_engine *e) {
     // device buffers come from the stream-ordered pool: freeing them does not
     // synchronise the device (other engines keep running)
     void *dev[] = {e->d_ring, e->d_acc, e->d_comp, e->d_acc64, e->d_norm, e->d_lags, e->d_az,  e->d_qoff,
-                   e->d_qcnt, e->d_idx, e->d_out,  e->d_tau,   e->d_fit,  e->d_xt,   e->d_wl};
+                   e->d_qcnt, e->d_idx, e->d_out,  e->d_tau,   e->d_fit,  e->d_xt,   e->d_wl, e->d_lay};
     for (void *ptr : dev)
         if (ptr && e->st) (void)hipFreeAsync(ptr, e->st);
     for (uint8_t *ptr : e->d_stage)
@@ -259,6 +261,36 @@ extern "C" int ddm_create(const ddm_desc *desc, ddm_engine **out) {
     TRY(dev_alloc(e, &e->d_idx, e->rt.idx.size(), "ring elements"));
     TRY(dev_alloc(e, &e->d_norm, L, "normalisation"));
     TRY(dev_alloc(e, &e->d_out, (size_t)p.result_count, "results"));
+    // tile layout tables: the kept elements of each scale's tiles
+    size_t lay_ints = 0;
+    for (const ddm::ScaleGeom &g : p.geom)
+        lay_ints += g.lay->start.size() + g.lay->band.size();
+    if (lay_ints) TRY(dev_alloc(e, &e->d_lay, lay_ints, "tile layout tables"));
+    {
+        int *at = e->d_lay;
+        hipError_t up = hipSuccess;
+        // the host tables are shared and outlive the upload
+        auto put = [&](const std::vector<int> &v) -> const int * {
+            if (v.empty()) return nullptr;
+            const int *d = at;
+            const hipError_t r = hipMemcpyAsync(at, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice, e->st);
+            if (up == hipSuccess) up = r;
+            at += v.size();
+            return d;
+        };
+        for (const ddm::ScaleGeom &g : p.geom) {
+            const ddm::LayoutTables &t = *g.lay;
+            ddm::TileLayout l;
+            l.kind = t.kind;
+            l.elems = t.elems;
+            l.pass_cols = t.pass_cols;
+            l.row_lanes = t.row_lanes;
+            l.start = put(t.start);
+            l.band = put(t.band);
+            e->lay.push_back(l);
+        }
+        HTRY(up);
+    }
     const size_t roi_frame = (size_t)p.M * p.M * p.bpp;
     // pinned host staging is allocated on the first host submission that needs it
     for (int i = 0; i < e->nstage; ++i) TRY(dev_alloc(e, &e->d_stage[i], roi_frame * p.chunk, "upload staging"));
@@ -435,7 +467,7 @@ static int process_chunk(ddm_engine *e, const uint8_t *src, int n, long long fra
         a.first_slot = first_slot;
         a.elems_per_frame = p.elems_padded;
         a.elem_off = g.elem_off;
-        a.lay = g.lay;
+        a.lay = e->lay[&g - p.geom.data()];
         a.precision = e->precision;
         a.xt = e->d_xt;
         HIP_TRY(e, ddm::launch_tile_fft(a, e->st));
@@ -634,6 +666,7 @@ void engine_batch_stream(ddm_engine *e, BatchStream *bs) {
     bs->lag_block = e->lag_block;
     bs->frames = e->frames;
     bs->precision = e->precision;
+    bs->lay = e->lay.data();
 }
 
 int engine_batch_upload(ddm_engine *e, const void *frames, int n, int *b, hipEvent_t *ev_up, const uint8_t **stage) {
@@ -844,7 +877,7 @@ static void expand_tiles(const ddm::ScaleGeom &g, const float *src, int comps, f
         float *to = out + (size_t)t * full * comps;
         for (int ky = 0; ky < s; ++ky)
             for (int kx = 0; kx <= H; ++kx) {
-                const long long at = g.lay.pos(s, ky, kx);
+                const int at = g.lay->pos[(size_t)ky * (H + 1) + kx];
                 float *o = to + ((size_t)ky * (H + 1) + kx) * comps;
                 for (int c = 0; c < comps; ++c) o[c] = at < 0 ? NAN : ts[(size_t)at * comps + c];
             }
@@ -1056,7 +1089,7 @@ extern "C" int ddm_autotune(ddm_engine *e, const char *cache_path, int *lag_bloc
                     a.first_slot = 0;
                     a.elems_per_frame = p.elems_padded;
                     a.elem_off = g.elem_off;
-                    a.lay = g.lay;
+                    a.lay = e->lay[&g - p.geom.data()];
                     a.precision = e->precision;
                     a.xt = e->d_xt;
                     const hipError_t err = ddm::launch_tile_fft(a, e->st);

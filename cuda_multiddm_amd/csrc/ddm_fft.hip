@@ -266,9 +266,11 @@ __device__ __forceinline__ void tile_fft_reg_body(const PixSrc &ps, const RingDs
     // row/column transpose and exchange regions; at the end the tiles'
     // spectra in ring element order
     __shared__ float2 lds[TPW * TE];
+    __shared__ int rtab[S + 1];  // row starts of a LAY_ROWS layout
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = tid >> 6;
+    load_row_starts<S>(rd, rtab, tid, REG_NTH);
     // frames vary fastest across the grid: concurrent workgroups write
     // neighbouring ring slots of the same element groups
     const int nf = nblk / groups;
@@ -388,7 +390,7 @@ __device__ __forceinline__ void tile_fft_reg_body(const PixSrc &ps, const RingDs
     }
     __syncthreads();
     const int first = (bid / nf) * TPW;  // the workgroup's first tile
-    store_tiles<S>(rd, lds, f, first, min(TPW, tiles - first), tid, REG_NTH);
+    store_tiles<S>(rd, lds, rtab, f, first, min(TPW, tiles - first), tid, REG_NTH);
 }
 
 template <int S, int PIX>
@@ -485,7 +487,9 @@ __device__ __forceinline__ void tile_fft_lane_body(const PixSrc &ps, const RingD
     constexpr int R = S / 2, TPG = SMALL_NTH / R;  // lanes per tile, tiles per workgroup
     constexpr int LS = R + 1;                      // LDS row stride (complex values)
     __shared__ float2 lds[TPG * S * LS];
+    __shared__ int rtab[S + 1];  // row starts of a LAY_ROWS layout
     const int tid = threadIdx.x, lt = tid / R, r = tid % R;
+    load_row_starts<S>(rd, rtab, tid, SMALL_NTH);
     const int nf = nblk / groups;  // frames vary fastest
     const int f = bid % nf;
     const int tile = (bid / nf) * TPG + lt;
@@ -538,7 +542,7 @@ __device__ __forceinline__ void tile_fft_lane_body(const PixSrc &ps, const RingD
     }
     __syncthreads();
     const int first = (bid / nf) * TPG;  // the workgroup's tiles are consecutive
-    store_tiles<S>(rd, lds, f, first, min(TPG, tiles - first), tid, SMALL_NTH);
+    store_tiles<S>(rd, lds, rtab, f, first, min(TPG, tiles - first), tid, SMALL_NTH);
 }
 
 template <int S, int PIX, bool A4>
@@ -690,8 +694,9 @@ __global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg(PixSrc ps, i
 // Columns: SEQ consecutive packed columns per workgroup, read from the pass
 // buffer's column block and written to the ring slot; the packed column is
 // separated into the DC (kx = 0) and Nyquist (kx = H) columns on the way out.
-// In a column-order layout the workgroup's columns are one contiguous run of
-// ring elements; blocks of columns the layout leaves out end at once.
+// In the column-order layout (LAY_COLS) each column's kept elements are one
+// contiguous run of the tile's ring elements, and the workgroup's columns
+// are adjacent; blocks with no kept element end at once.
 template <int S>
 __device__ __forceinline__ void tile_fft_cols_reg_body(const RingDst &rd, int tiles, const float2 *__restrict__ xt,
                                                        const float2 *__restrict__ twg, int bid, int nblk) {
@@ -702,14 +707,24 @@ __device__ __forceinline__ void tile_fft_cols_reg_body(const RingDst &rd, int ti
     // tile block [y][c ^ (y % SEQ)], SEQ columns per row; later the columns'
     // spectra, S + 1 apart so that reading one row across them is conflict free
     __shared__ float2 lds[SEQ * (S + 1)];
+    // LAY_COLS: starts of columns c0 .. c0 + SEQ, bands of c0 .. c0 + SEQ - 1,
+    // then start, end and band of the Nyquist column
+    __shared__ int ctab[2 * SEQ + 4];
     const int tid = threadIdx.x;
     const int nf = nblk / (NB * tiles);  // frames vary fastest
     const int f = bid % nf;
     const int cb = (bid / nf) % NB;
     const int tile = bid / (nf * NB);
     const int c0 = cb * SEQ;
-    if (c0 >= rd.lay.cols) return;  // the same for the whole workgroup
+    // the same for the whole workgroup (block 0 also holds the Nyquist column)
+    if (c0 >= rd.lay.pass_cols) return;
     const TileOut t = tile_out(rd, f, tile, S);
+    if (rd.lay.kind == LAY_COLS && tid < 2 * SEQ + 4) {
+        const int k = tid;
+        ctab[k] = k <= SEQ ? rd.lay.start[c0 + k]
+                           : (k < 2 * SEQ + 1 ? rd.lay.band[c0 + k - SEQ - 1]
+                                              : (k == 2 * SEQ + 3 ? rd.lay.band[H] : rd.lay.start[H + k - 2 * SEQ - 1]));
+    }
     const int col = tid % SEQ, row = tid / SEQ;  // staging: SEQ consecutive columns of RP rows per pass
     {
         float2 v[E1];
@@ -738,15 +753,28 @@ __device__ __forceinline__ void tile_fft_cols_reg_body(const RingDst &rd, int ti
     __syncthreads();
     // column c's spectrum is at lds[c * (S + 1) + ky]; the packed column's
     // DC and Nyquist of row ky come from V[ky] and V[S - ky]
-    if (t.lay.col_major) {
-        // the layout's rows of columns c0 .. c0 + nc - 1, in ring element
-        // order: consecutive threads write consecutive elements
-        const int nr = t.lay.rows(), nc = min(SEQ, t.lay.cols - c0);
-        const float rnr = 1.0f / (float)nr;
-        const long long e0 = t.base + (long long)c0 * nr;
-        for (int i = tid; i < nc * nr; i += NTH) {
-            const int c = div_small(i, nr, rnr), r = i - c * nr;
-            const int ky = r < t.lay.row_lo ? r : r + (S - nr);
+    if (t.lay.kind == LAY_COLS) {
+        // column by column: the kept rows of column kx are one contiguous
+        // run from start[kx], rows ky < band[kx] first, then the bottom ones
+        const int a0 = ctab[0];
+        if (ctab[SEQ] - a0 == SEQ * S) {
+            // every column of the block keeps every row
+            for (int j = tid; j < SEQ * S; j += NTH) {
+                const int c = j / S, ky = j & (S - 1);
+                float2 val;
+                if (c0 + c == 0) {
+                    const float2 w = lds[ky], pv = lds[(S - ky) & (S - 1)];
+                    val = make_float2(0.5f * (w.x + pv.x), 0.5f * (w.y - pv.y));
+                } else {
+                    val = lds[c * (S + 1) + ky];
+                }
+                t.ring[ring_index(t.base + a0 + j, t.slot, t.R)] = val;
+            }
+        } else {
+        for (int i = a0 + tid; i < ctab[SEQ]; i += NTH) {
+            const int c = find_start(ctab, SEQ, i), a = ctab[c], n = ctab[c + 1] - a, r = i - a;
+            const int lo = ctab[SEQ + 1 + c];
+            const int ky = r < lo ? r : r + (S - n);
             float2 val;
             if (c0 + c == 0) {
                 const float2 w = lds[ky], pv = lds[(S - ky) & (S - 1)];
@@ -754,14 +782,16 @@ __device__ __forceinline__ void tile_fft_cols_reg_body(const RingDst &rd, int ti
             } else {
                 val = lds[c * (S + 1) + ky];
             }
-            t.ring[ring_index(e0 + i, t.slot, t.R)] = val;
+            t.ring[ring_index(t.base + i, t.slot, t.R)] = val;
         }
-        if (c0 == 0 && t.lay.cols > H) {
-            for (int r = tid; r < nr; r += NTH) {
-                const int ky = r < t.lay.row_lo ? r : r + (S - nr);
+        }
+        if (c0 == 0) {
+            // the Nyquist column kx = H
+            const int a = ctab[2 * SEQ + 1], n = ctab[2 * SEQ + 2] - a, lo = ctab[2 * SEQ + 3];
+            for (int r = tid; r < n; r += NTH) {
+                const int ky = r < lo ? r : r + (S - n);
                 const float2 w = lds[ky], pv = lds[(S - ky) & (S - 1)];
-                t.ring[ring_index(t.base + (long long)H * nr + r, t.slot, t.R)] =
-                    make_float2(0.5f * (w.y + pv.y), -0.5f * (w.x - pv.x));
+                t.ring[ring_index(t.base + a + r, t.slot, t.R)] = make_float2(0.5f * (w.y + pv.y), -0.5f * (w.x - pv.x));
             }
         }
         return;
@@ -793,7 +823,7 @@ static void launch_big(const PixSrc &ps, const RingDst &rd, int tps, int tiles, 
                        float2 *xt, hipStream_t st) {
     using C = BigCfg<S>;
     const dim3 g1((unsigned)((C::H / C::SEQ) * tiles * n_frames)), block(C::NTH);
-    const int xcols = rd.lay.pass_cols(S);
+    const int xcols = rd.lay.pass_cols;
     switch (pixel_mode(ps)) {
     case PIX_U8_1:
         hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_U8_1>), g1, block, 0, st, ps, tps, tiles, xt, tw, xcols);
@@ -839,7 +869,7 @@ __global__ __launch_bounds__(BigCfg<S>::NTH) void tile_fft_rows_reg_batch(const 
                                                                          const float2 *__restrict__ twg) {
     const long long b = blockIdx.x;
     const FftJob &j = jobs[batch_job(jobs, hdr->njobs, b)];
-    tile_fft_rows_reg_body<S, PIX>(j.ps, j.tps, j.tiles, j.xt, twg, j.rd.lay.pass_cols(S), (int)(b - j.first),
+    tile_fft_rows_reg_body<S, PIX>(j.ps, j.tps, j.tiles, j.xt, twg, j.rd.lay.pass_cols, (int)(b - j.first),
                                    j.nblk);
 }
 

@@ -21,34 +21,102 @@ static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
 static std::shared_ptr<const std::vector<int>> ring_members(int s, float lambda, float tol);
 
-// The part of each s x s tile's half spectrum an engine's ring holds.  The
-// 256 .. 1024 transforms end in a column pass, so their tiles are stored
-// column after column (each column-pass workgroup then writes one contiguous
-// run); the others row after row.  With DDM_SPECTRUM_MASKED the rows and
-// columns shrink to the band holding every member of the scale's q rings
-// when that saves at least a tenth of the elements (8 .. 1024 tiles: the
-// 2 / 4 / 2048 transforms write the full spectrum).
-static TileLayout kept_layout(int s, const Params &p) {
-    TileLayout full = TileLayout::full(s);
-    const bool two_pass = s >= 256 && s <= 1024;
-    if (two_pass) full.col_major = 1;
-    if (p.spectrum != DDM_SPECTRUM_MASKED || s < 8 || s > 1024) return full;
-    const int H = s / 2;
-    int lo = 0, hi = 0, cols = 0;  // rows ky < lo and ky >= s - hi, columns kx < cols
-    for (float lam : p.lambdas) {
-        auto m = ring_members(s, lam, p.tol);
-        if (!m) return full;
-        for (int li : *m) {
-            const int ky = li / (H + 1), kx = li % (H + 1);
-            cols = std::max(cols, kx + 1);
-            if (ky < H)
-                lo = std::max(lo, ky + 1);
-            else
-                hi = std::max(hi, s - ky);
+// The part of each s x s tile's half spectrum an engine's ring holds
+// (LayoutTables).  The 256 .. 1024 tiles are kept column by column (their
+// transform ends in a column pass), the others row by row.  With
+// DDM_SPECTRUM_MASKED an 8 .. 1024 tile keeps, of every row (column), the
+// prefix of columns (the band of low |ky| rows) reaching its last q-ring
+// member, when that is less than 90 % of its elements (below that the
+// saving in lag work outweighs the costlier stores); the 2 / 4 / 2048
+// transforms write the full spectrum.  Tables are built once per (scale,
+// tolerance, lambdas, mode) and shared.
+static std::mutex g_lay_mu;
+static std::map<std::tuple<int, int, uint32_t, std::vector<float>>, std::shared_ptr<const LayoutTables>> g_lays;
+
+static std::shared_ptr<const LayoutTables> layout_tables(int s, const Params &p) {
+    uint32_t tb;
+    std::memcpy(&tb, &p.tol, 4);
+    const bool masked = p.spectrum == DDM_SPECTRUM_MASKED && s >= 8 && s <= 1024;
+    const auto key = std::make_tuple(s, masked ? 1 : 0, masked ? tb : 0u,
+                                     masked ? p.lambdas : std::vector<float>());
+    {
+        std::lock_guard<std::mutex> lk(g_lay_mu);
+        auto it = g_lays.find(key);
+        if (it != g_lays.end()) return it->second;
+    }
+    const int H = s / 2, full = s * (H + 1);
+    const bool cols = s >= 256 && s <= 1024;
+    // per row: columns kept (prefix); per column: rows kept from the top
+    // (ky < lo) and from the bottom (ky >= s - hi)
+    std::vector<int> width(s, H + 1), lo(H + 1, s), hi(H + 1, 0);
+    if (masked) {
+        std::fill(width.begin(), width.end(), 0);
+        std::fill(lo.begin(), lo.end(), 0);
+        bool any = false;
+        for (float lam : p.lambdas) {
+            auto m = ring_members(s, lam, p.tol);
+            if (!m) return nullptr;
+            for (int li : *m) {
+                const int ky = li / (H + 1), kx = li % (H + 1);
+                any = true;
+                width[ky] = std::max(width[ky], kx + 1);
+                if (ky < H)
+                    lo[kx] = std::max(lo[kx], ky + 1);
+                else
+                    hi[kx] = std::max(hi[kx], s - ky);
+            }
+        }
+        // a scale without ring members keeps one (unused) element: DC
+        if (!any) width[0] = lo[0] = 1;
+        long long n = 0;
+        if (cols)
+            for (int kx = 0; kx <= H; ++kx) n += lo[kx] + hi[kx];
+        else
+            for (int w : width) n += w;
+        if (n * 10 >= (long long)full * 9) {
+            std::fill(width.begin(), width.end(), H + 1);
+            std::fill(lo.begin(), lo.end(), s);
+            std::fill(hi.begin(), hi.end(), 0);
         }
     }
-    const TileLayout band{std::max(cols, 1), std::max(lo, 1), hi, two_pass ? 1 : 0};
-    return band.elems() * 10 > full.elems() * 9 ? full : band;
+    auto t = std::make_shared<LayoutTables>();
+    t->pos.assign(full, -1);
+    int n = 0;
+    if (cols) {
+        t->kind = LAY_COLS;
+        t->start.assign(H + 2, 0);
+        t->band.assign(H + 1, 0);
+        int last = 0;
+        for (int kx = 0; kx <= H; ++kx) {
+            t->start[kx] = n;
+            t->band[kx] = lo[kx];
+            for (int r = 0; r < lo[kx] + hi[kx]; ++r) {
+                const int ky = r < lo[kx] ? r : r + (s - lo[kx] - hi[kx]);
+                t->pos[(size_t)ky * (H + 1) + kx] = n++;
+            }
+            if (kx < H && lo[kx] + hi[kx] > 0) last = kx;
+        }
+        t->start[H + 1] = n;
+        t->pass_cols = std::min(H, (last / 16 + 1) * 16);
+    } else {
+        t->start.assign(s + 1, 0);
+        int widest = 1;
+        for (int ky = 0; ky < s; ++ky) {
+            t->start[ky] = n;
+            for (int kx = 0; kx < width[ky]; ++kx) t->pos[(size_t)ky * (H + 1) + kx] = n++;
+            widest = std::max(widest, width[ky]);
+        }
+        t->row_lanes = 1;
+        while (t->row_lanes < widest && t->row_lanes < 64) t->row_lanes *= 2;
+        t->start[s] = n;
+        t->kind = n == full ? LAY_FULL : LAY_ROWS;
+        t->pass_cols = H;
+        if (t->kind == LAY_FULL) t->start.clear();
+    }
+    t->elems = n;
+    if (n < 1) return nullptr;
+    std::lock_guard<std::mutex> lk(g_lay_mu);
+    return g_lays.emplace(key, t).first->second;
 }
 
 int check_desc(const ddm_desc *d, Params *out, std::string *msg) {
@@ -148,8 +216,9 @@ int check_desc(const ddm_desc *d, Params *out, std::string *msg) {
         g.s = s;
         g.tps = p.M / s;
         g.tiles = g.tps * g.tps;
-        g.lay = kept_layout(s, p);
-        g.tile_elems = g.lay.elems();
+        g.lay = layout_tables(s, p);
+        if (!g.lay) return fail(DDM_EINVAL, "q-ring tables could not be built");
+        g.tile_elems = g.lay->elems;
         g.elem_off = off;
         off += g.tile_elems * g.tiles;
         results += (long long)g.tiles * (long long)p.lambdas.size() * (long long)p.lags.size();
@@ -171,7 +240,10 @@ size_t device_footprint(const Params &p) {
         bytes += L * (size_t)p.elems_padded * (4 + 8);                  // accumulators they fold into
     bytes += UPLOAD_STAGES * (size_t)p.chunk * p.M * p.M * p.bpp;      // upload staging
     bytes += (size_t)p.result_count * 8 + L * 12;                      // results, lags, norms
-    for (const ScaleGeom &g : p.geom) bytes += Q * ((size_t)g.tile_elems * 4 + 8);  // ring tables (bound)
+    for (const ScaleGeom &g : p.geom) {
+        bytes += Q * ((size_t)g.tile_elems * 4 + 8);  // ring tables (bound)
+        bytes += (g.lay->start.size() + g.lay->band.size()) * 4;  // tile layout tables
+    }
     // upload staging and the between-pass buffer of 256 .. 1024 tile FFTs
     // grow to one FFT batch of frames when a call hands over more than a
     // chunk: counted at that size, so an admitted stream never outgrows it
@@ -213,8 +285,8 @@ int build_ring_tables(const Params &p, RingTables *out) {
     out->off.assign(p.geom.size() * Q, 0);
     out->cnt.assign(p.geom.size() * Q, 0);
     for (size_t si = 0; si < p.geom.size(); ++si) {
-        const int s = p.geom[si].s, H = s / 2;
-        const TileLayout &lay = p.geom[si].lay;
+        const int s = p.geom[si].s;
+        const LayoutTables &lay = *p.geom[si].lay;
         for (int q = 0; q < Q; ++q) {
             auto members = ring_members(s, p.lambdas[q], p.tol);
             if (!members) return DDM_EINVAL;
@@ -223,9 +295,9 @@ int build_ring_tables(const Params &p, RingTables *out) {
             // each member's place in the tile's stored layout, in member
             // order (the averages sum in list order, whatever the layout)
             for (int li : *members) {
-                const long long at = lay.pos(s, li / (H + 1), li % (H + 1));
+                const int at = lay.pos[li];
                 if (at < 0) return DDM_EINVAL;
-                out->idx.push_back((int)at);
+                out->idx.push_back(at);
             }
         }
     }

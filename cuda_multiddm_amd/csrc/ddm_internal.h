@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -35,32 +36,42 @@ inline long long big_fft_scratch_elems(int M, int scale, int n) {
 }
 
 // The part of an s x s tile's half spectrum [ky][kx <= s/2] a spectrum ring
-// holds, and its order: rows ky < row_lo and ky >= s - row_hi (the low
-// |ky| band, both signs), columns kx < cols, stored row after row
-// ([row][kx]) or, with col_major, column after column ([kx][row]).  Only
-// elements some q ring contains reach the averages, so an engine keeps the
-// smallest such band holding every ring member; the full spectrum in row
-// order is full(s).
+// holds, and its order.  Only elements some q ring contains reach the
+// averages, so an engine may keep just a superset of those:
+//  * LAY_FULL: every element, row after row (index ky * (s/2 + 1) + kx);
+//  * LAY_ROWS: row after row, row ky keeping columns kx < start[ky + 1] -
+//    start[ky], the first of them at start[ky] (s + 1 entries);
+//  * LAY_COLS: column after column, column kx keeping rows ky < band[kx]
+//    and the last start[kx + 1] - start[kx] - band[kx] rows (the low |ky|,
+//    both signs), the first of them at start[kx] (s/2 + 2 entries; band has
+//    s/2 + 1).  The 256 .. 1024 transforms end in a column pass, which then
+//    writes one contiguous run per column.
+// A launch's layout holds device pointers, LayoutTables host vectors.
+enum LayoutKind { LAY_FULL = 0, LAY_ROWS = 1, LAY_COLS = 2 };
 struct TileLayout {
-    int cols = 0, row_lo = 0, row_hi = 0, col_major = 0;
-    static constexpr TileLayout full(int s) { return TileLayout{s / 2 + 1, s, 0, 0}; }
-    constexpr int rows() const { return row_lo + row_hi; }
-    constexpr long long elems() const { return (long long)cols * (row_lo + row_hi); }
-    constexpr bool is_full(int s) const { return !col_major && cols == s / 2 + 1 && row_lo == s; }
-    // position of (ky, kx) inside the tile, or -1 when the layout leaves it out
-    constexpr long long pos(int s, int ky, int kx) const {
-        const int nr = row_lo + row_hi;
-        if (kx >= cols || (ky >= row_lo && ky < s - row_hi)) return -1;
-        const int r = ky < row_lo ? ky : ky - (s - nr);
-        return col_major ? (long long)kx * nr + r : (long long)r * cols + kx;
+    int kind = LAY_FULL;
+    int elems = 0;      // kept elements per tile
+    int pass_cols = 0;  // row-pass columns the column pass reads (multiple of 16, <= s/2)
+    int row_lanes = 64; // LAY_ROWS: lanes storing one row (a power of two >= the widest row, at most 64)
+    const int *start = nullptr;
+    const int *band = nullptr;
+    static TileLayout full(int s) {
+        TileLayout l;
+        l.elems = s * (s / 2 + 1);
+        l.pass_cols = s / 2;
+        return l;
     }
-    // row-pass columns the column pass reads (256 .. 1024 tiles): whole
-    // blocks of 16 up to cols, at most s/2 (the packed DC/Nyquist column
-    // is column 0)
-    constexpr int pass_cols(int s) const {
-        const int c = (cols + 15) / 16 * 16;
-        return c < s / 2 ? c : s / 2;
-    }
+};
+
+// Host tables of one scale's layout (shared by every engine with the same
+// scale, lambdas, tolerance and spectrum mode).  pos[full index] is the
+// kept index or -1, for every kind.
+struct LayoutTables {
+    int kind = LAY_FULL;
+    int elems = 0;
+    int pass_cols = 0;
+    int row_lanes = 64;
+    std::vector<int> start, band, pos;
 };
 
 // Frames of device-resident input (or of one large host call) transformed
@@ -83,9 +94,9 @@ struct ScaleGeom {
     int s = 0;            // tile size
     int tps = 0;          // tiles per side, M / s
     int tiles = 0;        // tps * tps
-    long long tile_elems = 0;  // lay.elems(): s * (s/2 + 1) for the full spectrum
+    long long tile_elems = 0;  // lay->elems: s * (s/2 + 1) for the full spectrum
     long long elem_off = 0;    // offset of this scale inside one frame's spectrum
-    TileLayout lay;            // the part of each tile's spectrum the ring holds
+    std::shared_ptr<const LayoutTables> lay;  // the part of each tile's spectrum the ring holds
 };
 
 // Descriptor after validation and sorting: scales descending, lags and

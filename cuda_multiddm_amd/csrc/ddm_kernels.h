@@ -26,8 +26,8 @@ struct TileFftArgs {
     // row and column passes (big_fft_scratch_elems values); without it the
     // row spectra go through the ring slot
     float *xt = nullptr;
-    // the part of each tile's spectrum the ring holds; cols == 0: the full
-    // spectrum in row order
+    // the part of each tile's spectrum the ring holds (device tables);
+    // elems == 0: the full spectrum in row order
     TileLayout lay{};
 };
 
@@ -105,6 +105,7 @@ struct BatchStream {
     int lag_block;
     long long frames;  // index of the next frame
     int precision;
+    const TileLayout *lay;  // per scale, device tables
 };
 void engine_batch_stream(ddm_engine *e, BatchStream *bs);
 // Uploads n host frames (n <= chunk, <= lag block) into the engine's next

@@ -558,7 +558,7 @@ hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st) {
     if (e != hipSuccess) return e;
     PixSrc ps{(const uint8_t *)a.frames, a.frame_bytes, (long long)a.width * a.bpp, a.bpp, a.fmt, a.x_off,
               a.y_off};
-    const TileLayout lay = a.lay.cols ? a.lay : TileLayout::full(a.scale);
+    const TileLayout lay = a.lay.elems ? a.lay : TileLayout::full(a.scale);
     RingDst rd{(float2 *)a.ring, a.elem_off, a.ring_slots, a.first_slot, lay};
     const int tps = a.largest_scale / a.scale, tiles = tps * tps;
     if (a.precision != DDM_PREC_FP32 && (a.scale == 64 || a.scale == 128))
@@ -567,7 +567,7 @@ hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st) {
         return hipGetLastError();
     // the 256 .. 2048 transforms below go through the ring slot in place:
     // full row-order layout only
-    if (a.scale >= 256 && !lay.is_full(a.scale)) return hipErrorInvalidValue;
+    if (lay.kind != LAY_FULL && (a.scale >= 256 || a.scale < 8)) return hipErrorInvalidValue;
     switch (a.scale) {
     case 2: launch_small<2>(ps, rd, tps, tiles, a.n_frames, tw, st); break;
     case 4: launch_small<4>(ps, rd, tps, tiles, a.n_frames, tw, st); break;

@@ -312,7 +312,8 @@ static int run_tick(ddm_server *s, const std::vector<Taken> &taken) {
             xt_off += xt_need;
         }
         for (const ddm::ScaleGeom &gm : p.geom) {
-            ddm::RingDst rd{(float2 *)bs.ring, gm.elem_off, R, cslot, gm.lay};
+            const ddm::TileLayout &lay = bs.lay[&gm - p.geom.data()];
+            ddm::RingDst rd{(float2 *)bs.ring, gm.elem_off, R, cslot, lay};
             tmp.clear();
             if (bs.precision == DDM_PREC_FP32 && ddm::fft_batch_jobs(ps, rd, gm.s, gm.tps, gm.tiles, c.n, xt, &tmp)) {
                 for (auto &kj : tmp) fft[kj.first].push_back(kj.second);
@@ -332,7 +333,7 @@ static int run_tick(ddm_server *s, const std::vector<Taken> &taken) {
                 a.first_slot = cslot;
                 a.elems_per_frame = p.elems_padded;
                 a.elem_off = gm.elem_off;
-                a.lay = gm.lay;
+                a.lay = lay;
                 a.precision = bs.precision;
                 a.xt = (float *)xt;
                 direct_fft.push_back(a);
@@ -595,13 +596,15 @@ static void sched_loop(ddm_server *s) {
     }
 }
 
-// Waits (holding lk on mu) until the stream's queued chunks have been taken
-// by ticks, and returns the stream or nullptr when it is gone.
-static Stream *wait_queue_empty(ddm_server *s, std::unique_lock<std::mutex> &lk, int stream_id) {
+// Waits (holding lk on mu) until every chunk of the stream has been
+// launched by a tick on the tick stream (taken from the queue is not enough:
+// the scheduler takes chunks before it launches their tick), and returns the
+// stream or nullptr when it is gone.
+static Stream *wait_chunks_launched(ddm_server *s, std::unique_lock<std::mutex> &lk, int stream_id) {
     for (;;) {
         auto it = s->streams.find(stream_id);
         if (it == s->streams.end()) return nullptr;
-        if (it->second.queue.empty()) return &it->second;
+        if (it->second.queue.empty() && it->second.outstanding == 0) return &it->second;
         s->idle.wait(lk);
     }
 }
@@ -868,7 +871,7 @@ extern "C" int ddm_server_results(ddm_server *s, int stream_id, double *results)
             auto it = s->streams.find(stream_id);
             it->second.urgent += 1;  // no holding its chunks back now
             s->work.notify_all();
-            Stream *st = wait_queue_empty(s, lk, stream_id);
+            Stream *st = wait_chunks_launched(s, lk, stream_id);
             if (st) st->urgent -= 1;
         }
     }

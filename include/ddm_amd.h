@@ -86,9 +86,11 @@ enum ddm_accumulation { DDM_ACC_COMPENSATED = 0, DDM_ACC_PLAIN = 1 };
 /* Which spectrum elements an engine keeps (ddm_desc.spectrum).
  * DDM_SPECTRUM_MASKED (default): only the I(q, tau) averages read the lag
  * sums, and they read only elements inside some q ring, so for each scale
- * the ring and accumulators hold the smallest band of a tile's half
- * spectrum (rows |ky| <= a, columns kx <= b) containing every ring member,
- * when that saves at least a tenth of the elements (scales 8 .. 1024).
+ * the ring and accumulators hold only the part of a tile's half spectrum
+ * that reaches the rings (for tiles up to 128, each row's columns up to its
+ * last ring member; for 256 .. 1024, each column's low |ky| rows down to
+ * its last ring member), when that saves at least a tenth of the elements
+ * (scales 8 .. 1024).
  * Results are bit-identical to DDM_SPECTRUM_FULL; per-element reads
  * (ddm_get_accumulators, ddm_get_spectra) return NaN for elements left
  * out.  DDM_SPECTRUM_FULL: every element, as the reference computes them

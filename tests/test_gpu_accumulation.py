@@ -47,7 +47,8 @@ def test_fold_into_fp64_over_a_long_stream(gpu):
     _, acc_ref = o.run(o.Config(64, 64, N2, scales, lags, lams), frames[:N2], acc_scale=0)
     res = {}
     for mode in (ddm.ACC_COMPENSATED, ddm.ACC_PLAIN):
-        with ddm.Engine(64, 64, scales, lags, lams, 1.2, 30, lag_block=32, accumulation=mode) as e:
+        with ddm.Engine(64, 64, scales, lags, lams, 1.2, 30, lag_block=32, accumulation=mode,
+                        spectrum=ddm.SPECTRUM_FULL) as e:
             print(f"mode {mode}: {N2} frames", flush=True)
             for i in range(0, N2, 30):
                 e.submit_host(frames[i:min(i + 30, N2)])

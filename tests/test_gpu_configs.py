@@ -120,7 +120,7 @@ def test_brownian2048_u16_full_size(gpu):
     # spectra of every tile of the last frame (still in the ring)
     last = W.frames - 1
     img = frames[last].astype(np.float64)
-    with ddm.Engine(*W.engine_args()) as e:
+    with ddm.Engine(*W.engine_args(), spectrum=ddm.SPECTRUM_FULL) as e:
         e.submit_host(np.ascontiguousarray(frames[W.frames - 40:]).reshape(-1).view(np.uint8), 40)
         for si, s in enumerate(W.scales):
             spec = e.spectra(39, si)

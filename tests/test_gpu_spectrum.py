@@ -1,16 +1,19 @@
 """Masked spectrum (DDM_SPECTRUM_MASKED, the default): the ring and the lag
-sums hold only the band of each tile's half spectrum (rows |ky| <= a,
-columns kx <= b) that contains every q-ring member; 256 .. 1024 tiles are
-stored column after column.  The averages read only ring members and sum
-them in member order, so I(q, tau) must be bit-identical to the full
-spectrum (DDM_SPECTRUM_FULL) for every scale, pixel format, lag block,
-precision mode and launch path, and per-element reads must match the full
-engine inside the band and be NaN outside it."""
+sums hold only the part of each tile's half spectrum that reaches the q rings
+(for 256 .. 1024 tiles, column after column, the low |ky| rows of each column
+down to its last ring member; for the others, row after row, the prefix of
+each row up to its last ring member).
+The averages read only ring members and sum them in member order, so
+I(q, tau) must be bit-identical to the full spectrum (DDM_SPECTRUM_FULL)
+for every scale, pixel format, lag block, precision mode and launch path,
+and per-element reads must match the full engine on the kept elements and
+be NaN elsewhere."""
 import numpy as np
 import pytest
 
 import cuda_multiddm_amd as ddm
 from oracle import oracle as o
+from tests.layout_util import kept_mask
 
 pytestmark = pytest.mark.gpu
 
@@ -22,7 +25,7 @@ CASES = {
     "multi256": dict(desc=dict(width=256, height=256, scales=[256, 128, 64, 32, 16, 8], lags=ddm.log_lags(200, 16),
                                lambdas=ddm.log_lambdas(4, 128, 12), chunk=30, lag_block=64),
                      n=150, bits=8, gen="brownian"),
-    # sparse rings with gaps between them (tolerance 5 %): the band is their hull
+    # sparse rings with gaps between them (tolerance 5 %); no member at all for 8 tiles
     "gaps": dict(desc=dict(width=256, height=256, scales=[256, 64, 8], lags=[1, 4, 9],
                            lambdas=[3.0, 11.0, 40.0], tolerance=1.05, chunk=20), n=60, bits=8, gen="uniform"),
     # 12-bit little-endian pixels in a wider frame at an ROI offset, 512 and 32 tiles
@@ -90,8 +93,9 @@ def test_masked_matrix_core_transforms(gpu, mode):
 
 
 def test_masked_spectra_and_accumulators(gpu):
-    """Per-element reads: equal to the full engine inside the band, NaN
-    outside it, and the band holds every q-ring member."""
+    """Per-element reads: equal to the full engine on the kept elements, NaN
+    elsewhere, and the kept elements are the rows' prefixes / columns' bands
+    reaching the last q-ring member (so every member is kept)."""
     case = "multi256"
     frames = _frames(case)
     d = dict(CASES[case]["desc"])
@@ -110,6 +114,7 @@ def test_masked_spectra_and_accumulators(gpu):
         fa, ma = out[ddm.SPECTRUM_FULL][1][si], out[ddm.SPECTRUM_MASKED][1][si]
         kept = ~np.isnan(ms.real[0])
         assert np.all(kept[union]), s  # every ring member is held
+        assert np.array_equal(kept, kept_mask(s, d["lambdas"], float(np.float32(1.2)))), s
         assert np.all(~np.isnan(ms.real)[:, kept]) and np.all(np.isnan(ms.real)[:, ~kept])
         assert np.array_equal(ms[:, kept], fs[:, kept]), s
         assert np.array_equal(ma[:, :, kept], fa[:, :, kept]), s

@@ -13,6 +13,7 @@ import pytest
 import cuda_multiddm_amd as ddm
 from cuda_multiddm_amd import engine
 from oracle import oracle as o
+from tests.layout_util import kept_mask
 from tests.movie_util import write_movie
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -36,23 +37,9 @@ def _v(**kw):
     return ddm.validate(**base)
 
 
-def _band(s, lambdas, tol):
-    """Elements per tile a masked engine keeps for scale s: the rows |ky| and
-    columns kx spanned by the union of the oracle's q-ring masks, or the whole
-    half spectrum when that saves less than a tenth (or s is outside 8..1024)."""
-    full = s * (s // 2 + 1)
-    if s < 8 or s > 1024:
-        return full
-    union = np.zeros((s, s // 2 + 1), dtype=bool)
-    for lam in lambdas:
-        union |= o.mask(s, lam, tol).astype(bool)
-    ky, kx = np.nonzero(union)
-    if len(ky) == 0:
-        return 1
-    lo = int(ky[ky < s // 2].max()) + 1 if np.any(ky < s // 2) else 1
-    hi = int(s - ky[ky >= s // 2].min()) if np.any(ky >= s // 2) else 0
-    band = (lo + hi) * (int(kx.max()) + 1)
-    return full if band * 10 > full * 9 else band
+def _kept(s, lambdas, tol):
+    """Elements per tile a masked engine keeps for scale s."""
+    return int(kept_mask(s, lambdas, tol).sum())
 
 
 def test_validate_accepts_config1_and_sorts():
@@ -62,11 +49,13 @@ def test_validate_accepts_config1_and_sorts():
     assert info.largest_scale == 128 and info.lag_block == 256 and info.ring_slots == 30 + 256 + 16
     assert info.elems_per_frame == 128 * 65 and info.elems_per_frame_padded == 8320
     assert info.result_count == 20 * 16
-    # masked (the default): rows |ky| <= 38 and columns kx <= 38 hold every
-    # member of the rings q = 128 / lambda .. 1.2 q, lambda >= 4
+    # masked (the default): the rows' prefixes holding the rings q = 128 /
+    # lambda .. 1.2 q, lambda = 4 .. 64 (a disc of radius 38: 28 % of the half
+    # spectrum)
     rc, msg, info = _v()
-    assert rc == 0 and info.elems_per_frame == 77 * 39 == _band(128, ddm.lin_lambdas(4, 64, 20), 1.2)
-    assert info.elems_per_frame_padded == 3008 and info.result_count == 20 * 16
+    n = _kept(128, [float(np.float32(x)) for x in ddm.lin_lambdas(4, 64, 20)], float(np.float32(1.2)))
+    assert rc == 0 and info.elems_per_frame == n and n < 0.29 * 128 * 65
+    assert info.elems_per_frame_padded == (n + 63) // 64 * 64 and info.result_count == 20 * 16
     rc, _, info = _v(width=1100, height=1030, scales=[256, 1024, 512], lags=[30, 1, 2],
                      lambdas=[16.0, 1.0, 3.0])
     assert rc == 0 and info.largest_scale == 1024
@@ -78,8 +67,8 @@ def test_validate_accepts_config1_and_sorts():
 
 @pytest.mark.parametrize("name", ["uniform128", "brownian512", "brownian2048_u16", "serving"])
 def test_masked_spectrum_sizes_of_the_workloads(name):
-    """Elements per frame a masked engine keeps, per workload, against the band
-    of the oracle's q-ring masks; the full-spectrum size is unchanged."""
+    """Elements per frame a masked engine keeps, per workload, against the
+    union of the oracle's q-ring masks; the full-spectrum size is unchanged."""
     from cuda_multiddm_amd.workloads import WORKLOADS, serving_streams
     if name == "serving":
         descs = [s.desc() for s in serving_streams(6)]
@@ -95,8 +84,22 @@ def test_masked_spectrum_sizes_of_the_workloads(name):
         tol = float(np.float32(d.get("tolerance", 1.2)))
         lams = [float(np.float32(x)) for x in d["lambdas"]]
         assert full.elems_per_frame == sum((M // s) ** 2 * s * (s // 2 + 1) for s in d["scales"])
-        assert masked.elems_per_frame == sum((M // s) ** 2 * _band(s, lams, tol) for s in d["scales"])
+        assert masked.elems_per_frame == sum((M // s) ** 2 * _kept(s, lams, tol) for s in d["scales"])
         print(name, M, masked.elems_per_frame / full.elems_per_frame)
+
+
+def test_masked_scale_without_ring_members_keeps_one_element():
+    """8 x 8 tiles with rings q = 8 / lambda .. 1.05 q that hold no element:
+    the scale keeps exactly one element per tile (its tables are never empty)."""
+    d = dict(width=256, height=256, scales=[256, 64, 8], lags=[1, 4, 9], lambdas=[3.0, 11.0, 40.0],
+             tolerance=1.05, chunk=20)
+    for lam in d["lambdas"]:
+        assert not o.mask(8, lam, float(np.float32(1.05))).any()
+    rc, _, info = ddm.validate(**d)
+    lams = [float(np.float32(x)) for x in d["lambdas"]]
+    tol = float(np.float32(1.05))
+    assert rc == 0 and _kept(8, lams, tol) == 1
+    assert info.elems_per_frame == _kept(256, lams, tol) + 16 * _kept(64, lams, tol) + 1024
 
 
 def test_validate_rejects_unknown_spectrum_mode():
