@@ -63,6 +63,7 @@ struct ddm_engine {
     int *d_qoff = nullptr, *d_qcnt = nullptr, *d_idx = nullptr;
     int *d_lay = nullptr;              // tile layout tables of every scale
     std::vector<ddm::TileLayout> lay;  // per scale, pointing into d_lay
+    std::vector<int> family;           // per scale, enum ddm_fft_family
     double *d_norm = nullptr, *d_out = nullptr;
     double *d_tau = nullptr, *d_fit = nullptr;  // allocated by the first ddm_fit_results
     float *d_xt = nullptr;    // row spectra of 256 / 512 tiles between FFT passes
@@ -291,6 +292,7 @@ extern "C" int ddm_create(const ddm_desc *desc, ddm_engine **out) {
         }
         HTRY(up);
     }
+    e->family.assign(p.geom.size(), DDM_FFT_REGISTER);
     const size_t roi_frame = (size_t)p.M * p.M * p.bpp;
     // pinned host staging is allocated on the first host submission that needs it
     for (int i = 0; i < e->nstage; ++i) TRY(dev_alloc(e, &e->d_stage[i], roi_frame * p.chunk, "upload staging"));
@@ -468,6 +470,7 @@ static int process_chunk(ddm_engine *e, const uint8_t *src, int n, long long fra
         a.elems_per_frame = p.elems_padded;
         a.elem_off = g.elem_off;
         a.lay = e->lay[&g - p.geom.data()];
+        a.family = e->family[&g - p.geom.data()];
         a.precision = e->precision;
         a.xt = e->d_xt;
         HIP_TRY(e, ddm::launch_tile_fft(a, e->st));
@@ -667,6 +670,7 @@ void engine_batch_stream(ddm_engine *e, BatchStream *bs) {
     bs->frames = e->frames;
     bs->precision = e->precision;
     bs->lay = e->lay.data();
+    bs->family = e->family.data();
 }
 
 int engine_batch_upload(ddm_engine *e, const void *frames, int n, int *b, hipEvent_t *ev_up, const uint8_t **stage) {
@@ -966,12 +970,14 @@ static std::string tune_key(const ddm_engine *e) {
                     " M" + std::to_string(p.M) + " bpp" + std::to_string(p.bpp) + " fmt" + std::to_string(p.fmt) +
                     " C" + std::to_string(p.chunk) + " J" + std::to_string(p.lag_block) + " S";
     for (const ddm::ScaleGeom &g : p.geom) k += std::to_string(g.s) + ",";
-    char hx[32];
-    std::snprintf(hx, sizeof hx, " L%zu:%016llx", p.lags.size(), (unsigned long long)h);
+    char hx[64];
+    std::snprintf(hx, sizeof hx, " L%zu:%016llx E%lld", p.lags.size(), (unsigned long long)h, p.elems_per_frame);
     return k + hx;
 }
 
-static bool tune_lookup(const char *path, const std::string &key, int *J, int *B) {
+// A cache line is `key<TAB>J B f0 f1 ...`: the lag block, the FFT batch and
+// the transform family of every scale (descending).
+static bool tune_lookup(const char *path, const std::string &key, int *J, int *B, std::vector<int> *fam) {
     if (!path) return false;
     FILE *f = std::fopen(path, "r");
     if (!f) return false;
@@ -980,16 +986,27 @@ static bool tune_lookup(const char *path, const std::string &key, int *J, int *B
     while (std::fgets(line, sizeof line, f)) {
         const char *tab = std::strchr(line, '\t');
         if (!tab || std::string(line, tab - line) != key) continue;
-        int j = 0, b = 0;
-        if (std::sscanf(tab + 1, "%d %d", &j, &b) == 2) {  // the last matching line wins
-            *J = j;
-            *B = b;
-            found = true;
+        int j = 0, b = 0, used = 0;
+        if (std::sscanf(tab + 1, "%d %d%n", &j, &b, &used) != 2) continue;
+        std::vector<int> fl;
+        const char *at = tab + 1 + used;
+        int v = 0, n = 0;
+        while (std::sscanf(at, "%d%n", &v, &n) == 1) {
+            fl.push_back(v);
+            at += n;
         }
+        if (fl.size() != fam->size()) continue;
+        *J = j;  // the last matching line wins
+        *B = b;
+        *fam = fl;
+        found = true;
     }
     std::fclose(f);
     return found;
 }
+
+// Scales whose transform family may be chosen: 8 .. 128 tiles.
+static bool family_choice(int s) { return s >= 8 && s <= 128; }
 
 // Milliseconds of `launch` on the engine's stream: best of three after one
 // untimed run.
@@ -1020,9 +1037,13 @@ extern "C" int ddm_autotune(ddm_engine *e, const char *cache_path, int *lag_bloc
     const Params &p = e->p;
     const std::string key = tune_key(e);
     int J = 0, B = 0;
-    const bool cached = tune_lookup(cache_path, key, &J, &B) && J >= 32 && J <= p.lag_block && (J & (J - 1)) == 0 &&
-                        B >= p.chunk && B % p.chunk == 0 && B <= p.chunk + p.lag_block;
+    std::vector<int> fam(p.geom.size(), DDM_FFT_REGISTER);
+    bool cached = tune_lookup(cache_path, key, &J, &B, &fam) && J >= 32 && J <= p.lag_block && (J & (J - 1)) == 0 &&
+                  B >= p.chunk && B % p.chunk == 0 && B <= p.chunk + p.lag_block;
+    for (size_t si = 0; cached && si < fam.size(); ++si)
+        cached = fam[si] == DDM_FFT_REGISTER || (fam[si] == DDM_FFT_LDS && family_choice(p.geom[si].s));
     if (!cached) {
+        std::fill(fam.begin(), fam.end(), DDM_FFT_REGISTER);
         // lag block: J frames against the full max-lag history, per frame; a
         // candidate must beat the descriptor's block by 2 % to replace it
         float best = 1e30f, def_lag = 1e30f;
@@ -1107,6 +1128,43 @@ extern "C" int ddm_autotune(ddm_engine *e, const char *cache_path, int *lag_bloc
             }
         }
         if (def_ms < 1e30f && best > 0.98f * def_ms) B = def_b;
+        // transform family per 8 .. 128 tile scale: that scale alone over
+        // the chosen batches, registers (default) against LDS
+        for (size_t si = 0; si < p.geom.size(); ++si) {
+            const ddm::ScaleGeom &g = p.geom[si];
+            if (!family_choice(g.s) || e->precision != DDM_PREC_FP32) continue;
+            float t[2] = {1e30f, 1e30f};
+            for (int fm : {DDM_FFT_REGISTER, DDM_FFT_LDS}) {
+                auto launch = [&]() -> hipError_t {
+                    for (int done = 0; done < bmax; done += B) {
+                        ddm::TileFftArgs a;
+                        a.frames = frames + (long long)done * fb;
+                        a.n_frames = std::min(B, bmax - done);
+                        a.frame_bytes = fb;
+                        a.width = p.width;
+                        a.bpp = p.bpp;
+                        a.fmt = p.fmt;
+                        a.x_off = p.x_off;
+                        a.y_off = p.y_off;
+                        a.largest_scale = p.M;
+                        a.scale = g.s;
+                        a.ring = e->d_ring;
+                        a.ring_slots = p.ring_slots;
+                        a.first_slot = 0;
+                        a.elems_per_frame = p.elems_padded;
+                        a.elem_off = g.elem_off;
+                        a.lay = e->lay[si];
+                        a.family = fm;
+                        const hipError_t err = ddm::launch_tile_fft(a, e->st);
+                        if (err != hipSuccess) return err;
+                    }
+                    return hipSuccess;
+                };
+                rc = time_ms(e, launch, &t[fm]);
+                if (rc) return rc;
+            }
+            if (t[DDM_FFT_LDS] < 0.98f * t[DDM_FFT_REGISTER]) fam[si] = DDM_FFT_LDS;
+        }
         HIP_TRY(e, hipFreeAsync(frames, e->st));
         e->dev_bytes -= (size_t)(fb * bmax);
         // the timing runs accumulated into the lag sums
@@ -1116,15 +1174,34 @@ extern "C" int ddm_autotune(ddm_engine *e, const char *cache_path, int *lag_bloc
         HIP_TRY(e, hipStreamSynchronize(e->st));
         if (cache_path) {
             if (FILE *f = std::fopen(cache_path, "a")) {
-                std::fprintf(f, "%s\t%d %d\n", key.c_str(), J, B);
+                std::fprintf(f, "%s\t%d %d", key.c_str(), J, B);
+                for (int v : fam) std::fprintf(f, " %d", v);
+                std::fprintf(f, "\n");
                 std::fclose(f);
             }
         }
     }
     e->lag_block = J;
     e->fft_batch = B;
+    e->family = fam;
     if (lag_block_out) *lag_block_out = J;
     if (fft_batch_out) *fft_batch_out = B;
+    return DDM_OK;
+}
+
+extern "C" int ddm_set_fft_family(ddm_engine *e, int scale_idx, int family) {
+    if (!e) return DDM_EINVAL;
+    if (scale_idx < 0 || scale_idx >= (int)e->p.geom.size()) return fail(e, DDM_ERANGE, "scale index out of range");
+    if (e->frames != 0) return fail(e, DDM_ESTATE, "the transform family is fixed once frames arrived");
+    if (family != DDM_FFT_REGISTER && !(family == DDM_FFT_LDS && family_choice(e->p.geom[scale_idx].s)))
+        return fail(e, DDM_EINVAL, "no kernel of that family for this scale");
+    e->family[scale_idx] = family;
+    return DDM_OK;
+}
+
+extern "C" int ddm_get_fft_families(const ddm_engine *e, int *families) {
+    if (!e || !families) return DDM_EINVAL;
+    for (size_t i = 0; i < e->family.size(); ++i) families[i] = e->family[i];
     return DDM_OK;
 }
 

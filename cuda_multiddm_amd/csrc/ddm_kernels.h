@@ -29,6 +29,7 @@ struct TileFftArgs {
     // the part of each tile's spectrum the ring holds (device tables);
     // elems == 0: the full spectrum in row order
     TileLayout lay{};
+    int family = DDM_FFT_REGISTER;  // enum ddm_fft_family (8 .. 128 tiles)
 };
 
 
@@ -106,6 +107,7 @@ struct BatchStream {
     long long frames;  // index of the next frame
     int precision;
     const TileLayout *lay;  // per scale, device tables
+    const int *family;      // per scale, enum ddm_fft_family
 };
 void engine_batch_stream(ddm_engine *e, BatchStream *bs);
 // Uploads n host frames (n <= chunk, <= lag block) into the engine's next

@@ -563,7 +563,8 @@ hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st) {
     const int tps = a.largest_scale / a.scale, tiles = tps * tps;
     if (a.precision != DDM_PREC_FP32 && (a.scale == 64 || a.scale == 128))
         return launch_tile_dft_mfma(ps, rd, a.scale, tps, tiles, a.n_frames, a.precision, st);
-    if (launch_tile_fft_reg(ps, rd, a.scale, tps, tiles, a.n_frames, tw, (float2 *)a.xt, st))
+    const bool lds_family = a.family == DDM_FFT_LDS && a.scale <= 128;
+    if (!lds_family && launch_tile_fft_reg(ps, rd, a.scale, tps, tiles, a.n_frames, tw, (float2 *)a.xt, st))
         return hipGetLastError();
     // the 256 .. 2048 transforms below go through the ring slot in place:
     // full row-order layout only

@@ -315,7 +315,9 @@ static int run_tick(ddm_server *s, const std::vector<Taken> &taken) {
             const ddm::TileLayout &lay = bs.lay[&gm - p.geom.data()];
             ddm::RingDst rd{(float2 *)bs.ring, gm.elem_off, R, cslot, lay};
             tmp.clear();
-            if (bs.precision == DDM_PREC_FP32 && ddm::fft_batch_jobs(ps, rd, gm.s, gm.tps, gm.tiles, c.n, xt, &tmp)) {
+            const int fam = bs.family[&gm - p.geom.data()];
+            if (bs.precision == DDM_PREC_FP32 && fam == DDM_FFT_REGISTER &&
+                ddm::fft_batch_jobs(ps, rd, gm.s, gm.tps, gm.tiles, c.n, xt, &tmp)) {
                 for (auto &kj : tmp) fft[kj.first].push_back(kj.second);
             } else {
                 ddm::TileFftArgs a;
@@ -334,6 +336,7 @@ static int run_tick(ddm_server *s, const std::vector<Taken> &taken) {
                 a.elems_per_frame = p.elems_padded;
                 a.elem_off = gm.elem_off;
                 a.lay = lay;
+                a.family = fam;
                 a.precision = bs.precision;
                 a.xt = (float *)xt;
                 direct_fft.push_back(a);

@@ -30,6 +30,7 @@ NORM_FRAMES, NORM_PAIRS = 0, 1
 PREC_FP32, PREC_BF16X2, PREC_FP16X2 = 0, 1, 2
 ACC_COMPENSATED, ACC_PLAIN = 0, 1
 SPECTRUM_MASKED, SPECTRUM_FULL = 0, 1
+FFT_REGISTER, FFT_LDS = 0, 1
 SERVER_BATCHED, SERVER_NO_GRAPH, SERVER_GROUP = 1, 2, 4
 
 # Every symbol include/ddm_amd.h declares.
@@ -43,6 +44,7 @@ EXPORTED = [
     "ddm_server_create", "ddm_server_destroy", "ddm_server_open", "ddm_server_submit", "ddm_server_results",
     "ddm_server_engine", "ddm_server_drain", "ddm_server_close", "ddm_server_latencies", "ddm_server_stats",
     "ddm_server_last_error", "ddm_server_create2", "ddm_server_set_hold", "ddm_fit_results", "ddm_k_fit", "ddm_set_precision", "ddm_autotune", "ddm_reserve",
+    "ddm_set_fft_family", "ddm_get_fft_families",
 ]
 
 
@@ -119,6 +121,8 @@ def lib():
     L.ddm_fit_results.argtypes = [vp, ctypes.c_float, dp, dp]
     L.ddm_k_fit.argtypes = [vp, vp, ctypes.c_int, ll, ctypes.c_int, vp, vp]
     L.ddm_autotune.argtypes = [vp, ctypes.c_char_p, ip, ip]
+    L.ddm_set_fft_family.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.ddm_get_fft_families.argtypes = [vp, ip]
     L.ddm_reserve.argtypes = [vp, ctypes.c_int]
     L.ddm_reset.argtypes = [vp]
     L.ddm_get_info.argtypes = [vp, ctypes.POINTER(Info)]
@@ -298,12 +302,23 @@ class Engine:
         self._check(lib().ddm_set_precision(self._h, int(mode)))
 
     def autotune(self, cache_path: Optional[str] = None):
-        """Times the lag block and the FFT batch on this engine (before the
-        first frame) and keeps the fastest; returns (lag_block, fft_batch_frames)."""
+        """Times the lag block, the FFT batch and each 8..128 scale's transform
+        family on this engine (before the first frame) and keeps the fastest;
+        returns (lag_block, fft_batch_frames); fft_families() reads the families."""
         j, b = ctypes.c_int(0), ctypes.c_int(0)
         path = cache_path.encode() if cache_path else None
         self._check(lib().ddm_autotune(self._h, path, ctypes.byref(j), ctypes.byref(b)))
         return j.value, b.value
+
+    def set_fft_family(self, scale_idx: int, family: int):
+        """Transform family (FFT_REGISTER or FFT_LDS) of scale scale_idx (sorted
+        descending), before the first frame."""
+        self._check(lib().ddm_set_fft_family(self._h, int(scale_idx), int(family)))
+
+    def fft_families(self):
+        out = (ctypes.c_int * len(self.scales))()
+        self._check(lib().ddm_get_fft_families(self._h, out))
+        return list(out)
 
     def reserve(self, max_frames: int):
         """Allocates now the transient buffers a submission of up to

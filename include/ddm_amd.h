@@ -97,6 +97,14 @@ enum ddm_accumulation { DDM_ACC_COMPENSATED = 0, DDM_ACC_PLAIN = 1 };
  * (needed for its non-averaged -K output and the -A spectrum dump). */
 enum ddm_spectrum { DDM_SPECTRUM_MASKED = 0, DDM_SPECTRUM_FULL = 1 };
 
+/* Tile transform family of a scale (ddm_set_fft_family, ddm_autotune).
+ * DDM_FFT_REGISTER (default): register FFTs, radix 16 / 32 with a lane per
+ * row pair for 8 .. 32 tiles and 16 / 32 lanes per sequence for 64 .. 1024.
+ * DDM_FFT_LDS: Stockham radix 4 / 2 passes in LDS, several tiles per
+ * workgroup (available for 8 .. 128 tiles; 2, 4 and 2048 tiles always use
+ * it).  Both compute the same transform; the autotuner keeps the faster. */
+enum ddm_fft_family { DDM_FFT_REGISTER = 0, DDM_FFT_LDS = 1 };
+
 typedef struct ddm_desc {
     int width;            /* frame width in pixels */
     int height;           /* frame height in pixels */
@@ -216,15 +224,27 @@ int ddm_set_timing(ddm_engine *eng, int on);
 int ddm_set_precision(ddm_engine *eng, int mode);
 
 /* Autotuner: before the first frame, times the lag-engine block (64, 128 or
- * 256 frames, at most the descriptor's lag_block) and the FFT batch of
- * device-resident input (1, 2, 4 or 8 chunks) on this engine's buffers with
- * synthetic frames, and keeps the fastest per frame.  cache_path (may be
- * NULL): text file of earlier choices, keyed by device and geometry; a hit
- * skips the timing, a miss appends the new choice.  Writes the choice to
- * *lag_block / *fft_batch_frames (either may be NULL).  Results do not depend
- * on the choice.  No counterpart in the reference, whose launch shapes are
- * compile-time constants (src/constants.hpp:11-13). */
+ * 256 frames, at most the descriptor's lag_block), the FFT batch of
+ * device-resident input (1, 2, 4 or 8 chunks) and, for every 8 .. 128 tile
+ * scale, the tile transform family (enum ddm_fft_family: radix 16 / 32 in
+ * registers or radix 4 / 2 in LDS) on this engine's buffers with synthetic
+ * frames, and keeps the fastest per frame (a candidate must beat the default
+ * by 2 %).  cache_path (may be NULL): text file of earlier choices, keyed by
+ * device and geometry; a hit skips the timing, a miss appends the new
+ * choice.  Writes the choice to *lag_block / *fft_batch_frames (either may
+ * be NULL; ddm_get_fft_families reads the families).  Results do not depend
+ * on the choice beyond FP32 rounding of the transforms.  No counterpart in
+ * the reference, whose launch shapes are compile-time constants
+ * (src/constants.hpp:11-13). */
 int ddm_autotune(ddm_engine *eng, const char *cache_path, int *lag_block, int *fft_batch_frames);
+
+/* The tile transform family of scale scale_idx (sorted descending), before
+ * the first frame: DDM_EINVAL for a family the scale has no kernel for.  No
+ * counterpart in the reference (cuFFT picks its own kernels). */
+int ddm_set_fft_family(ddm_engine *eng, int scale_idx, int family);
+
+/* Each scale's transform family (scale_count entries, scales descending). */
+int ddm_get_fft_families(const ddm_engine *eng, int *families);
 
 /* Allocates now the transient device buffers that a submission of up to
  * max_frames frames per call would otherwise allocate on first use (the FFT

@@ -259,8 +259,10 @@ def test_autotune_schedule_and_cache(gpu, tmp_path):
     torch.cuda.synchronize()
     with ddm.Engine(128, 128, scales, lags, lams, 1.2, 20) as e:
         j, b = e.autotune(cache)
-        print(f"autotune: lag block {j}, FFT batch {b}", flush=True)
+        fam = e.fft_families()
+        print(f"autotune: lag block {j}, FFT batch {b}, transform families {fam}", flush=True)
         assert j in (64, 128, 256) and b in (20, 40, 80, 160)
+        assert all(f in (ddm.FFT_REGISTER, ddm.FFT_LDS) for f in fam)
         assert e.info().lag_block == j
         e.submit_device(t.data_ptr(), frames.shape[0])
         e.sync()
@@ -270,7 +272,38 @@ def test_autotune_schedule_and_cache(gpu, tmp_path):
     ref = o.run(o.Config(128, 128, frames.shape[0], scales, lags, lams, np.float32(1.2).item()), frames)
     assert o.max_rel_err(got, ref) <= TOL
     lines = open(cache).read().splitlines()
-    assert len(lines) == 1 and lines[0].endswith(f"\t{j} {b}")
+    assert len(lines) == 1 and lines[0].endswith(f"\t{j} {b} " + " ".join(str(f) for f in fam))
     with ddm.Engine(128, 128, scales, lags, lams, 1.2, 20) as e2:
-        assert e2.autotune(cache) == (j, b)
+        assert e2.autotune(cache) == (j, b) and e2.fft_families() == fam
     assert len(open(cache).read().splitlines()) == 1  # a hit appends nothing
+
+
+@pytest.mark.parametrize("spectrum", [ddm.SPECTRUM_MASKED, ddm.SPECTRUM_FULL])
+def test_lds_transform_family(gpu, spectrum):
+    """The radix-4/2 LDS transforms of the 8 .. 128 tiles (the autotuner's
+    alternative to the register FFTs) give the same I(q, tau) to FP32
+    rounding and match the oracle; scales without the family and changes
+    after the first frame are refused."""
+    frames = ddm.synth_uniform(90, 256, 256, 12, seed=29)
+    scales, lags, lams = [256, 128, 64, 32, 16, 8], [1, 2, 5, 11, 30], ddm.log_lambdas(3, 64, 9)
+    args = (256, 256, scales, lags, lams, 1.2, 30, 2, ddm.PIX_U16LE)
+    out = {}
+    for fam in (ddm.FFT_REGISTER, ddm.FFT_LDS):
+        with ddm.Engine(*args, spectrum=spectrum) as e:
+            for si, s in enumerate(e.scales):
+                if 8 <= s <= 128:
+                    e.set_fft_family(si, fam)
+                elif fam == ddm.FFT_LDS:
+                    with pytest.raises(ddm.DDMError):
+                        e.set_fft_family(si, fam)
+            assert e.fft_families() == [fam if 8 <= s <= 128 else ddm.FFT_REGISTER for s in e.scales]
+            e.submit_host(frames.reshape(-1).view(np.uint8), len(frames))
+            out[fam] = e.results()
+            with pytest.raises(ddm.DDMError):
+                e.set_fft_family(1, ddm.FFT_REGISTER)
+    ref = o.run(o.Config(256, 256, 90, scales, lags, lams, np.float32(1.2).item(), 2, ddm.PIX_U16LE), frames)
+    for fam in out:
+        err = o.max_rel_err(out[fam], ref)
+        print(f"spectrum {spectrum} family {fam}: max rel err {err:.2e}")
+        assert err <= TOL
+    assert o.max_rel_err(out[ddm.FFT_LDS], out[ddm.FFT_REGISTER]) <= 2e-6
