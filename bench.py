@@ -429,14 +429,19 @@ def main():
         # buffers the first submission would allocate (pass buffer, staging)
         # belong to engine creation, outside the timed region
         e.reserve(N)
+        if "buf" not in res_host:
+            res_host["buf"] = torch.empty(e.result_count, dtype=torch.float64, pin_memory=True).numpy()
         return e
+
+    res_host = {}
 
     def step(e):
         if args.host_input:
             e.submit_host(h_frames, N)  # ROI rows only cross PCIe
         else:
             e.submit_device(d_frames.data_ptr(), N)
-        return e.results()
+        # results land in pinned host memory, allocated once outside the timed steps
+        return e.results(out=res_host["buf"])
 
     for _ in range(args.warmup):
         with make() as e:
@@ -453,6 +458,7 @@ def main():
         torch.cuda.synchronize()
         elapsed += time.perf_counter() - t0
         e.close()
+    res = res.copy()  # the pinned buffer is reused by the runs below
     fps = args.steps * N / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     if args.profile_steps:

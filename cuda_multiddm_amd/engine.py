@@ -266,8 +266,15 @@ class Engine:
     def sync(self):
         self._check(lib().ddm_sync(self._h))
 
-    def results(self) -> np.ndarray:
-        out = np.zeros(self.result_count, dtype=np.float64)
+    def results(self, out: Optional[np.ndarray] = None) -> np.ndarray:
+        """I(q, tau) of every (scale, tile, q, lag), flat.  `out` (optional): a
+        contiguous float64 array of result_count values to write into, e.g. in
+        pinned host memory, which the copy from the device then reaches
+        directly."""
+        if out is None:
+            out = np.zeros(self.result_count, dtype=np.float64)
+        elif out.dtype != np.float64 or out.size != self.result_count or not out.flags.c_contiguous:
+            raise ValueError(f"out must be a contiguous float64 array of {self.result_count} values")
         self._check(lib().ddm_results(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
