@@ -11,8 +11,12 @@
 
 namespace ddm {
 
+// Complex product with its fused multiply-adds written out: the rounding is
+// then the same in every kernel the transforms are inlined into (left to the
+// compiler, a.x * b.x - a.y * b.y may fuse either product), so a tile's
+// spectrum does not depend on which launch computed it.
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return make_float2(__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -115,23 +119,48 @@ __device__ __forceinline__ void load_row_starts(const RingDst &rd, int *rtab, in
         for (int i = tid; i <= S; i += nth) rtab[i] = rd.lay.start[i];
 }
 
-// The spectra of ntile consecutive tiles (first, first + 1, ...) of batch
-// frame f, staged in LDS at lds[t * S * (S/2 + 1) + ky * (S/2 + 1) + kx],
-// stored to the ring in the row-order layout rd.lay (LAY_FULL or LAY_ROWS,
-// whose row starts rtab holds in LDS) by nth threads (a multiple of 64):
-// consecutive lanes write consecutive ring elements.
+// The tiles one workgroup transforms: local tile t < ntile is tile
+// first + t % 2^wsh + (t >> wsh) * tps of its scale (tps tiles per side), so
+// a per-scale launch takes runs of consecutive tiles (2^wsh >= ntile) and
+// the multi-scale tiler the (2^wsh)^2 tiles of one block.
+struct TileSet {
+    int first, wsh, ntile;
+    __device__ __forceinline__ int tile(int t, int tps) const {
+        return first + (t & ((1 << wsh) - 1)) + (t >> wsh) * tps;
+    }
+};
+
+// The spectra of the tiles of ts for batch frame f, staged in LDS at
+// lds[t * S * (S/2 + 1) + ky * (S/2 + 1) + kx], stored to the ring in the
+// row-order layout rd.lay (LAY_FULL or LAY_ROWS, whose row starts rtab holds
+// in LDS) by nth threads (a multiple of 64): consecutive lanes write
+// consecutive ring elements.
 template <int S>
 __device__ __forceinline__ void store_tiles(const RingDst &rd, const float2 *lds, const int *rtab, int f,
-                                            int first, int ntile, int tid, int nth) {
+                                            const TileSet &ts, int tps, int tid, int nth) {
     constexpr int TE = S * (S / 2 + 1);
     const int slot = ring_slot(rd, f);
     const long long gstride = (long long)rd.ring_slots * RING_GROUP;
     const int CE = rd.lay.elems;
-    const long long e0 = rd.elem_off + (long long)first * CE;
+    const int ntile = ts.ntile;
     if (rd.lay.kind == LAY_FULL) {
-        for (int i = tid; i < ntile * TE; i += nth) {
-            const long long e = e0 + i;
-            rd.ring[(e >> RING_GROUP_LOG2) * gstride + (long long)slot * RING_GROUP + (e & (RING_GROUP - 1))] = lds[i];
+        if (ntile <= (1 << ts.wsh)) {
+            // consecutive tiles: one run of ring elements
+            const long long e0 = rd.elem_off + (long long)ts.first * CE;
+            for (int i = tid; i < ntile * TE; i += nth) {
+                const long long e = e0 + i;
+                rd.ring[(e >> RING_GROUP_LOG2) * gstride + (long long)slot * RING_GROUP + (e & (RING_GROUP - 1))] =
+                    lds[i];
+            }
+            return;
+        }
+        for (int t = 0; t < ntile; ++t) {
+            const long long e0 = rd.elem_off + (long long)ts.tile(t, tps) * CE;
+            for (int i = tid; i < TE; i += nth) {
+                const long long e = e0 + i;
+                rd.ring[(e >> RING_GROUP_LOG2) * gstride + (long long)slot * RING_GROUP + (e & (RING_GROUP - 1))] =
+                    lds[t * TE + i];
+            }
         }
         return;
     }
@@ -143,7 +172,7 @@ __device__ __forceinline__ void store_tiles(const RingDst &rd, const float2 *lds
     for (int j = (tid >> 6) * rpw + sub; j < ntile * S; j += step) {
         const int t = j / S, ky = j % S;
         const int a = rtab[ky], n = rtab[ky + 1] - a;
-        const long long eb = e0 + (long long)t * CE + a;
+        const long long eb = rd.elem_off + (long long)ts.tile(t, tps) * CE + a;
         const float2 *src = lds + t * TE + ky * (S / 2 + 1);
         for (int kx = kl; kx < n; kx += lpr) {
             const long long e = eb + kx;
@@ -152,6 +181,27 @@ __device__ __forceinline__ void store_tiles(const RingDst &rd, const float2 *lds
         }
     }
 }
+
+// Multi-scale tiler (ddm_fft.hip).  One scale of a batch: its ring
+// destination, tile size and (256 .. 1024) its own pass-buffer region.
+struct TilerScale {
+    RingDst rd;
+    int s;
+    float2 *xt;
+};
+// Scales the tiler's launches cover in an M x M region: 8 .. 128 tiles
+// (register transforms) from 128 x 128 blocks; 256 .. 1024 row passes from
+// row bands.
+bool tiler_block_scale(int scale, int M);
+bool tiler_band_scale(int scale, int M);
+// The 8 .. 128 scales sc[0 .. n) (descending) of n_frames frames in one
+// launch, every 128 x 128 block's pixels read once.
+hipError_t launch_tiler_blocks(const PixSrc &ps, const TilerScale *sc, int n, int M, int n_frames, const float2 *tw,
+                               hipStream_t st);
+// The row passes of the 256 .. 1024 scales sc[0 .. n) (descending) in one
+// launch, every row band's pixels read once, then each scale's column pass.
+hipError_t launch_tiler_bands(const PixSrc &ps, const TilerScale *sc, int n, int M, int n_frames, const float2 *tw,
+                              hipStream_t st);
 
 // Register-resident tile FFT (ddm_fft.hip) for the scales it covers;
 // returns false for other scales.  xt (optional) holds the row spectra of

@@ -64,6 +64,7 @@ struct ddm_engine {
     int *d_lay = nullptr;              // tile layout tables of every scale
     std::vector<ddm::TileLayout> lay;  // per scale, pointing into d_lay
     std::vector<int> family;           // per scale, enum ddm_fft_family
+    int tiler = DDM_TILER_PER_SCALE;   // enum ddm_tiler
     double *d_norm = nullptr, *d_out = nullptr;
     double *d_tau = nullptr, *d_fit = nullptr;  // allocated by the first ddm_fit_results
     float *d_xt = nullptr;    // row spectra of 256 / 512 tiles between FFT passes
@@ -423,8 +424,7 @@ static int flush_lags(ddm_engine *e) {
 // batch of n frames.  Grown on the engine's stream, so work already queued
 // on it keeps the old buffer until it is done.
 static int ensure_xt(ddm_engine *e, int n) {
-    long long need = 0;
-    for (const ddm::ScaleGeom &g : e->p.geom) need = std::max(need, ddm::big_fft_scratch_elems(e->p.M, g.s, n));
+    const long long need = ddm::fft_pass_elems(e->p.M, e->p.scales, n, e->tiler == DDM_TILER_FUSED);
     if (need <= e->xt_elems) return DDM_OK;
     if (e->d_xt) {
         HIP_TRY(e, hipFreeAsync(e->d_xt, e->st));
@@ -452,8 +452,10 @@ static int process_chunk(ddm_engine *e, const uint8_t *src, int n, long long fra
     Mark chunk, fft;
     mark_begin(e, &chunk, MK_CHUNK);
     mark_begin(e, &fft, MK_FFT);
-    for (const ddm::ScaleGeom &g : p.geom) {
-        ddm::TileFftArgs a;
+    std::vector<ddm::TileFftArgs> args(p.geom.size());
+    for (size_t i = 0; i < p.geom.size(); ++i) {
+        const ddm::ScaleGeom &g = p.geom[i];
+        ddm::TileFftArgs &a = args[i];
         a.frames = src;
         a.n_frames = n;
         a.frame_bytes = frame_bytes;
@@ -469,13 +471,15 @@ static int process_chunk(ddm_engine *e, const uint8_t *src, int n, long long fra
         a.first_slot = first_slot;
         a.elems_per_frame = p.elems_padded;
         a.elem_off = g.elem_off;
-        a.lay = e->lay[&g - p.geom.data()];
-        a.family = e->family[&g - p.geom.data()];
+        a.lay = e->lay[i];
+        a.family = e->family[i];
         a.precision = e->precision;
         a.xt = e->d_xt;
-        HIP_TRY(e, ddm::launch_tile_fft(a, e->st));
-        e->stats.fft_launches += (g.s >= 256) ? 2 : 1;
     }
+    int launches = 0;
+    HIP_TRY(e, ddm::launch_tile_fft_scales(args.data(), (int)args.size(), e->tiler == DDM_TILER_FUSED, e->st,
+                                           &launches));
+    e->stats.fft_launches += launches;
     mark_end(e, &fft);
     if (consumed) HIP_TRY(e, hipEventRecord(consumed, e->st));
     // pairs (t - tau, t) formed for t in [t0, t0 + n)
@@ -1188,6 +1192,16 @@ extern "C" int ddm_autotune(ddm_engine *e, const char *cache_path, int *lag_bloc
     if (fft_batch_out) *fft_batch_out = B;
     return DDM_OK;
 }
+
+extern "C" int ddm_set_tiler(ddm_engine *e, int mode) {
+    if (!e) return DDM_EINVAL;
+    if (mode != DDM_TILER_FUSED && mode != DDM_TILER_PER_SCALE) return fail(e, DDM_EINVAL, "unknown tiler mode");
+    if (e->frames != 0) return fail(e, DDM_ESTATE, "the tiler is fixed once frames arrived");
+    e->tiler = mode;
+    return DDM_OK;
+}
+
+extern "C" int ddm_get_tiler(const ddm_engine *e) { return e ? e->tiler : DDM_EINVAL; }
 
 extern "C" int ddm_set_fft_family(ddm_engine *e, int scale_idx, int family) {
     if (!e) return DDM_EINVAL;

@@ -24,6 +24,8 @@
 // group of 16 lanes touches 16 distinct 8-byte bank slots in each access.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "ddm_batch.h"
 #include "ddm_device.h"
 #include "ddm_kernels.h"
@@ -35,12 +37,16 @@ namespace {
 constexpr int REG_NTH = 512;  // 8 waves
 constexpr int REG_WAVE_SLOTS = 1024;  // complex values of LDS per wave
 
+// log2 of a power of two
+__host__ __device__ constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
+
 constexpr float C16 = 0.92387953251128674f;  // cos(pi/8)
 constexpr float S16 = 0.38268343236508977f;  // sin(pi/8)
 constexpr float R2 = 0.70710678118654752f;   // sqrt(2)/2
 
-__device__ __forceinline__ float2 mulw(float2 v, float c, float s) {  // v * (c + i s)
-    return make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
+// v * (c + i s), fused multiply-adds written out as in cmul
+__device__ __forceinline__ float2 mulw(float2 v, float c, float s) {
+    return make_float2(__builtin_fmaf(v.x, c, -(v.y * s)), __builtin_fmaf(v.x, s, v.y * c));
 }
 __device__ __forceinline__ float2 mul_mi(float2 v) { return make_float2(v.y, -v.x); }  // v * (-i)
 
@@ -252,32 +258,109 @@ __device__ __forceinline__ void load_row_pair(const uint8_t *fr, const PixSrc &p
     }
 }
 
+// S consecutive pixels of row y from x0: with A4 every row start is 4-byte
+// aligned, so 1- and 2-byte pixels come in 4-byte words.
+template <int PIX, int S, bool A4>
+__device__ __forceinline__ void load_row(const uint8_t *fr, const PixSrc &ps, int x0, int y, float (&out)[S]) {
+    const uint8_t *row = fr + (long long)y * ps.row_bytes;
+    if constexpr (PIX == PIX_U8_1 && A4 && S % 4 == 0) {
+        const uint32_t *w = reinterpret_cast<const uint32_t *>(row + x0);
+#pragma unroll
+        for (int i = 0; i < S / 4; ++i) {
+            const uint32_t u = w[i];
+            out[4 * i] = (float)(u & 0xff);
+            out[4 * i + 1] = (float)((u >> 8) & 0xff);
+            out[4 * i + 2] = (float)((u >> 16) & 0xff);
+            out[4 * i + 3] = (float)(u >> 24);
+        }
+    } else if constexpr (PIX == PIX_U16LE_2 && A4 && S % 2 == 0) {
+        const uint32_t *w = reinterpret_cast<const uint32_t *>(row + 2 * x0);
+#pragma unroll
+        for (int i = 0; i < S / 2; ++i) {
+            const uint32_t u = w[i];
+            out[2 * i] = (float)(u & 0xffff);
+            out[2 * i + 1] = (float)(u >> 16);
+        }
+    } else if constexpr (PIX == PIX_U8_1) {
+#pragma unroll
+        for (int i = 0; i < S; ++i) out[i] = (float)row[x0 + i];
+    } else if constexpr (PIX == PIX_U16LE_2) {
+        const uint16_t *h = reinterpret_cast<const uint16_t *>(row) + x0;
+#pragma unroll
+        for (int i = 0; i < S; ++i) out[i] = (float)h[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < S; ++i) out[i] = load_px(fr, ps, x0 + i, y);
+    }
+}
+
+// Pixel sources of the transform bodies; (x, y) are coordinates in the
+// region of interest.
+// A frame in device memory (PIX as in load_row_pair / load_row).
+template <int PIX>
+struct FramePix {
+    const uint8_t *fr;
+    PixSrc ps;
+    // pixels (x + n STEP, y) and (x + n STEP, y + 1) as x_{2r} + i x_{2r+1}
+    template <int N, int STEP>
+    __device__ __forceinline__ void row_pair(int x, int y, float2 (&v)[N]) const {
+        load_row_pair<PIX, N, STEP>(fr, ps, ps.x_off + x, ps.y_off + y, v);
+    }
+    // S consecutive pixels of row y from x
+    template <int S, bool A4>
+    __device__ __forceinline__ void row(int x, int y, float (&out)[S]) const {
+        load_row<PIX, S, A4>(fr, ps, ps.x_off + x, ps.y_off + y, out);
+    }
+};
+
+// A 128 x 128 block of 16-bit pixel values staged in LDS (multi-scale
+// tiler): row pitch 132 values (66 dwords), so that the rows 2r that the
+// lanes of one LDS access read fall in distinct banks for every body's
+// access pattern; (ox, oy) is the block's origin.
+constexpr int BLK = 128;
+constexpr int BLK_PITCH = 132;
+struct BlockPix {
+    const uint16_t *blk;
+    int ox, oy;
+    template <int N, int STEP>
+    __device__ __forceinline__ void row_pair(int x, int y, float2 (&v)[N]) const {
+        const uint16_t *r0 = blk + (y - oy) * BLK_PITCH + (x - ox);
+#pragma unroll
+        for (int n = 0; n < N; ++n) v[n] = make_float2((float)r0[n * STEP], (float)r0[BLK_PITCH + n * STEP]);
+    }
+    // x - ox is a multiple of S >= 8, so every row piece is 8-byte aligned
+    template <int S, bool A4>
+    __device__ __forceinline__ void row(int x, int y, float (&out)[S]) const {
+        const uint2 *r = reinterpret_cast<const uint2 *>(blk + (y - oy) * BLK_PITCH + (x - ox));
+#pragma unroll
+        for (int i = 0; i < S / 4; ++i) {
+            const uint2 u = r[i];
+            out[4 * i] = (float)(u.x & 0xffff);
+            out[4 * i + 1] = (float)(u.x >> 16);
+            out[4 * i + 2] = (float)(u.y & 0xffff);
+            out[4 * i + 3] = (float)(u.y >> 16);
+        }
+    }
+};
+
 }  // namespace
 
-// PIX: PIX_U8_1 (bpp 1, DDM_PIX_U8) and PIX_U16LE_2 (bpp 2, DDM_PIX_U16LE)
-// load pixels with constant offsets from two row pointers; PIX_ANY sends
-// every pixel through load_px.
-template <int S, int PIX>
-__device__ __forceinline__ void tile_fft_reg_body(const PixSrc &ps, const RingDst &rd, int tps, int tiles,
-                                                  int groups, const float2 *__restrict__ twg, int bid, int nblk) {
+// 64 x 64 and 128 x 128 tiles: the TPW tiles of ts on REG_NTH threads (tid),
+// pixels from src, for batch frame f.  lds holds TPW * S * (S/2 + 1)
+// complex values (row/column transpose and exchange regions; at the end
+// the tiles' spectra in ring element order), rtab S + 1 ints.
+template <int S, class Src>
+__device__ __forceinline__ void reg_tiles(const Src &src, const RingDst &rd, int tps, const TileSet &ts,
+                                          const float2 *__restrict__ twg, float2 *lds, int *rtab, int tid, int f) {
     using C = RegCfg<S>;
-    constexpr int H = C::H, L = C::L, P = C::P, Q = C::Q, WPT = C::WPT, TPW = C::TPW, TILE = C::TILE;
+    constexpr int H = C::H, L = C::L, P = C::P, Q = C::Q, WPT = C::WPT, TILE = C::TILE;
     constexpr int TE = S * (H + 1);  // spectrum elements per tile
-    // row/column transpose and exchange regions; at the end the tiles'
-    // spectra in ring element order
-    __shared__ float2 lds[TPW * TE];
-    __shared__ int rtab[S + 1];  // row starts of a LAY_ROWS layout
-    const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = tid >> 6;
     load_row_starts<S>(rd, rtab, tid, REG_NTH);
-    // frames vary fastest across the grid: concurrent workgroups write
-    // neighbouring ring slots of the same element groups
-    const int nf = nblk / groups;
-    const int f = bid % nf;
     const int t = w / WPT, wit = w % WPT;
-    const int tile = (bid / nf) * TPW + t;
-    const bool valid = tile < tiles;
+    const int tile = ts.tile(t, tps);
+    const bool valid = t < ts.ntile;
     const int q = lane % L, sq = lane / L;  // lane within the sequence, sequence within the wave
     float2 *reg = lds + w * REG_WAVE_SLOTS;  // this wave's region
     float2 *tb = lds + t * TILE;             // this wave's tile
@@ -292,10 +375,9 @@ __device__ __forceinline__ void tile_fft_reg_body(const PixSrc &ps, const RingDs
     {
         const int r = wit * Q + sq;
         if (valid) {
-            const uint8_t *fr = ps.frames + (long long)f * ps.frame_bytes;
-            const int x0 = ps.x_off + (tile % tps) * S + q;
-            const int y0 = ps.y_off + (tile / tps) * S + 2 * r;
-            load_row_pair<PIX, 16, L>(fr, ps, x0, y0, v);
+            const int x0 = (tile % tps) * S + q;
+            const int y0 = (tile / tps) * S + 2 * r;
+            src.template row_pair<16, L>(x0, y0, v);
         } else {
 #pragma unroll
             for (int n1 = 0; n1 < 16; ++n1) v[n1] = make_float2(0.f, 0.f);
@@ -389,8 +471,27 @@ __device__ __forceinline__ void tile_fft_reg_body(const PixSrc &ps, const RingDs
         }
     }
     __syncthreads();
+    store_tiles<S>(rd, lds, rtab, f, ts, tps, tid, REG_NTH);
+}
+
+// A per-scale launch: workgroup bid of nblk takes TPW consecutive tiles of
+// one frame.  PIX: PIX_U8_1 (bpp 1, DDM_PIX_U8) and PIX_U16LE_2 (bpp 2,
+// DDM_PIX_U16LE) load pixels with constant offsets from two row pointers;
+// PIX_ANY sends every pixel through load_px.
+template <int S, int PIX>
+__device__ __forceinline__ void tile_fft_reg_body(const PixSrc &ps, const RingDst &rd, int tps, int tiles,
+                                                  int groups, const float2 *__restrict__ twg, int bid, int nblk) {
+    constexpr int TPW = RegCfg<S>::TPW;
+    __shared__ float2 lds[TPW * S * (S / 2 + 1)];
+    __shared__ int rtab[S + 1];  // row starts of a LAY_ROWS layout
+    // frames vary fastest across the grid: concurrent workgroups write
+    // neighbouring ring slots of the same element groups
+    const int nf = nblk / groups;
+    const int f = bid % nf;
     const int first = (bid / nf) * TPW;  // the workgroup's first tile
-    store_tiles<S>(rd, lds, rtab, f, first, min(TPW, tiles - first), tid, REG_NTH);
+    const FramePix<PIX> src{ps.frames + (long long)f * ps.frame_bytes, ps};
+    reg_tiles<S>(src, rd, tps, TileSet{first, ilog2(TPW), min(TPW, tiles - first)}, twg, lds,
+                 rtab, threadIdx.x, f);
 }
 
 template <int S, int PIX>
@@ -445,65 +546,29 @@ __device__ __forceinline__ void dftS<32>(float2 (&v)[32]) {
     dft32(v);
 }
 
-// S consecutive pixels of row y from x0: with A4 every row start is 4-byte
-// aligned, so 1- and 2-byte pixels come in 4-byte words.
-template <int PIX, int S, bool A4>
-__device__ __forceinline__ void load_row(const uint8_t *fr, const PixSrc &ps, int x0, int y, float (&out)[S]) {
-    const uint8_t *row = fr + (long long)y * ps.row_bytes;
-    if constexpr (PIX == PIX_U8_1 && A4 && S % 4 == 0) {
-        const uint32_t *w = reinterpret_cast<const uint32_t *>(row + x0);
-#pragma unroll
-        for (int i = 0; i < S / 4; ++i) {
-            const uint32_t u = w[i];
-            out[4 * i] = (float)(u & 0xff);
-            out[4 * i + 1] = (float)((u >> 8) & 0xff);
-            out[4 * i + 2] = (float)((u >> 16) & 0xff);
-            out[4 * i + 3] = (float)(u >> 24);
-        }
-    } else if constexpr (PIX == PIX_U16LE_2 && A4 && S % 2 == 0) {
-        const uint32_t *w = reinterpret_cast<const uint32_t *>(row + 2 * x0);
-#pragma unroll
-        for (int i = 0; i < S / 2; ++i) {
-            const uint32_t u = w[i];
-            out[2 * i] = (float)(u & 0xffff);
-            out[2 * i + 1] = (float)(u >> 16);
-        }
-    } else if constexpr (PIX == PIX_U8_1) {
-#pragma unroll
-        for (int i = 0; i < S; ++i) out[i] = (float)row[x0 + i];
-    } else if constexpr (PIX == PIX_U16LE_2) {
-        const uint16_t *h = reinterpret_cast<const uint16_t *>(row) + x0;
-#pragma unroll
-        for (int i = 0; i < S; ++i) out[i] = (float)h[i];
-    } else {
-#pragma unroll
-        for (int i = 0; i < S; ++i) out[i] = load_px(fr, ps, x0 + i, y);
-    }
-}
-
-template <int S, int PIX, bool A4>
-__device__ __forceinline__ void tile_fft_lane_body(const PixSrc &ps, const RingDst &rd, int tps, int tiles,
-                                                   int groups, int bid, int nblk) {
-    constexpr int R = S / 2, TPG = SMALL_NTH / R;  // lanes per tile, tiles per workgroup
-    constexpr int LS = R + 1;                      // LDS row stride (complex values)
-    __shared__ float2 lds[TPG * S * LS];
-    __shared__ int rtab[S + 1];  // row starts of a LAY_ROWS layout
-    const int tid = threadIdx.x, lt = tid / R, r = tid % R;
-    load_row_starts<S>(rd, rtab, tid, SMALL_NTH);
-    const int nf = nblk / groups;  // frames vary fastest
-    const int f = bid % nf;
-    const int tile = (bid / nf) * TPG + lt;
-    const bool live = tile < tiles;
+// 8, 16 and 32 tiles: the tiles of ts (at most NTH / (S/2)) on threads
+// tid < NTH of a workgroup of WG threads (WG >= NTH: the others only meet
+// the barriers), pixels from src, for batch frame f.  lds holds
+// NTH / (S/2) * S * (S/2 + 1) complex values, rtab S + 1 ints.
+template <int S, bool A4, int NTH, int WG, class Src>
+__device__ __forceinline__ void lane_tiles(const Src &src, const RingDst &rd, int tps, const TileSet &ts,
+                                           float2 *lds, int *rtab, int tid, int f) {
+    constexpr int R = S / 2;   // lanes per tile
+    constexpr int LS = R + 1;  // LDS row stride (complex values)
+    const int lt = tid / R, r = tid % R;
+    const bool act = WG == NTH || tid < NTH;
+    load_row_starts<S>(rd, rtab, tid, WG);
+    const int tile = ts.tile(lt, tps);
+    const bool live = act && lt < ts.ntile;
     float2 *blk = lds + lt * S * LS;
     if (live) {
-        const uint8_t *fr = ps.frames + (long long)f * ps.frame_bytes;
-        const int x0 = ps.x_off + (tile % tps) * S;
-        const int y0 = ps.y_off + (tile / tps) * S + 2 * r;
+        const int x0 = (tile % tps) * S;
+        const int y0 = (tile / tps) * S + 2 * r;
         float2 z[S];
         {
             float a[S], b[S];
-            load_row<PIX, S, A4>(fr, ps, x0, y0, a);
-            load_row<PIX, S, A4>(fr, ps, x0, y0 + 1, b);
+            src.template row<S, A4>(x0, y0, a);
+            src.template row<S, A4>(x0, y0 + 1, b);
 #pragma unroll
             for (int n = 0; n < S; ++n) z[n] = make_float2(a[n], b[n]);
         }
@@ -522,27 +587,46 @@ __device__ __forceinline__ void tile_fft_lane_body(const PixSrc &ps, const RingD
     __syncthreads();
     // column c = r of the tile
     float2 v[S];
+    if (act) {
 #pragma unroll
-    for (int ky = 0; ky < S; ++ky) v[ky] = blk[ky * LS + r];
-    dftS<S>(v);
+        for (int ky = 0; ky < S; ++ky) v[ky] = blk[ky * LS + r];
+        dftS<S>(v);
+    }
     __syncthreads();
     // the spectrum [ky][kx <= R] back into the tile's block (LS = R + 1 is
     // its row length), then out to the ring in element order
-    if (r != 0) {
+    if (act) {
+        if (r != 0) {
 #pragma unroll
-        for (int ky = 0; ky < S; ++ky) blk[ky * LS + r] = v[ky];
-    } else {
-        // packed column: DC (kx = 0) and Nyquist (kx = R) columns
+            for (int ky = 0; ky < S; ++ky) blk[ky * LS + r] = v[ky];
+        } else {
+            // packed column: DC (kx = 0) and Nyquist (kx = R) columns
 #pragma unroll
-        for (int ky = 0; ky < S; ++ky) {
-            const float2 w = v[ky], pv = v[(S - ky) & (S - 1)];
-            blk[ky * LS] = make_float2(0.5f * (w.x + pv.x), 0.5f * (w.y - pv.y));
-            blk[ky * LS + R] = make_float2(0.5f * (w.y + pv.y), -0.5f * (w.x - pv.x));
+            for (int ky = 0; ky < S; ++ky) {
+                const float2 w = v[ky], pv = v[(S - ky) & (S - 1)];
+                blk[ky * LS] = make_float2(0.5f * (w.x + pv.x), 0.5f * (w.y - pv.y));
+                blk[ky * LS + R] = make_float2(0.5f * (w.y + pv.y), -0.5f * (w.x - pv.x));
+            }
         }
     }
     __syncthreads();
+    if (act) store_tiles<S>(rd, lds, rtab, f, ts, tps, tid, NTH);
+}
+
+// A per-scale launch: workgroup bid of nblk takes TPG consecutive tiles of
+// one frame.
+template <int S, int PIX, bool A4>
+__device__ __forceinline__ void tile_fft_lane_body(const PixSrc &ps, const RingDst &rd, int tps, int tiles,
+                                                   int groups, int bid, int nblk) {
+    constexpr int R = S / 2, TPG = SMALL_NTH / R;  // lanes per tile, tiles per workgroup
+    __shared__ float2 lds[TPG * S * (R + 1)];
+    __shared__ int rtab[S + 1];  // row starts of a LAY_ROWS layout
+    const int nf = nblk / groups;  // frames vary fastest
+    const int f = bid % nf;
     const int first = (bid / nf) * TPG;  // the workgroup's tiles are consecutive
-    store_tiles<S>(rd, lds, rtab, f, first, min(TPG, tiles - first), tid, SMALL_NTH);
+    const FramePix<PIX> src{ps.frames + (long long)f * ps.frame_bytes, ps};
+    lane_tiles<S, A4, SMALL_NTH, SMALL_NTH>(src, rd, tps, TileSet{first, ilog2(TPG), min(TPG, tiles - first)}, lds,
+                                            rtab, threadIdx.x, f);
 }
 
 template <int S, int PIX, bool A4>
@@ -641,28 +725,15 @@ __device__ __forceinline__ void big_fft(float2 (&v)[BigCfg<S>::E1], float2 *seqb
 // so every 16 lanes of a row piece write 128 aligned bytes and the column
 // pass reads each block as one contiguous run.  Only columns kx < xcols (a
 // multiple of 16: the blocks the column pass reads) are written.
-template <int S, int PIX>
-__device__ __forceinline__ void tile_fft_rows_reg_body(const PixSrc &ps, int tps, int tiles, float2 *__restrict__ xt,
-                                                       const float2 *__restrict__ twg, int xcols, int bid,
-                                                       int nblk) {
+// The row pair r of tile `tile` (frame f) transformed by big_fft into
+// seqbuf, separated and written to the pass buffer by the sequence's lane q.
+template <int S>
+__device__ __forceinline__ void rows_to_xt(const float2 *seqbuf, float2 *__restrict__ xt, int f, int tiles, int tile,
+                                           int r, int q, int xcols) {
     using C = BigCfg<S>;
-    constexpr int H = C::H, L = C::L, E1 = C::E1, SEQ = C::SEQ, GROUPS = H / SEQ;
-    __shared__ float2 lds[SEQ * S];
-    const int tid = threadIdx.x, q = tid & (L - 1), sq = tid / L;
-    const int nf = nblk / (GROUPS * tiles);  // frames vary fastest
-    const int f = bid % nf;
-    const int g = (bid / nf) % GROUPS;
-    const int tile = bid / (nf * GROUPS);
-    const int r = g * SEQ + sq;  // row pair within the tile
-    const uint8_t *fr = ps.frames + (long long)f * ps.frame_bytes;
-    const int x0 = ps.x_off + (tile % tps) * S + q;
-    const int y0 = ps.y_off + (tile / tps) * S + 2 * r;
-    float2 v[E1];
-    load_row_pair<PIX, E1, L>(fr, ps, x0, y0, v);
-    float2 *seqbuf = lds + sq * S;
-    big_fft<S>(v, seqbuf, q, twg);
-    // separate the row pair: lane q writes columns k = q + L m, block
-    // k / 16 = (L / 16) m + q / 16, column q % 16 of it
+    constexpr int H = C::H, L = C::L;
+    // lane q writes columns k = q + L m, block k / 16 = (L / 16) m + q / 16,
+    // column q % 16 of it
     float2 *xa = xt + ((long long)f * tiles + tile) * ((long long)S * H) + (2 * r) * 16 + (q & 15) +
                  (long long)(q >> 4) * S * 16;
 #pragma unroll
@@ -682,6 +753,29 @@ __device__ __forceinline__ void tile_fft_rows_reg_body(const PixSrc &ps, int tps
             xa[(long long)m * (L / 16) * S * 16 + 16] = b;
         }
     }
+}
+
+template <int S, int PIX>
+__device__ __forceinline__ void tile_fft_rows_reg_body(const PixSrc &ps, int tps, int tiles, float2 *__restrict__ xt,
+                                                       const float2 *__restrict__ twg, int xcols, int bid,
+                                                       int nblk) {
+    using C = BigCfg<S>;
+    constexpr int L = C::L, E1 = C::E1, SEQ = C::SEQ, GROUPS = C::H / SEQ;
+    __shared__ float2 lds[SEQ * S];
+    const int tid = threadIdx.x, q = tid & (L - 1), sq = tid / L;
+    const int nf = nblk / (GROUPS * tiles);  // frames vary fastest
+    const int f = bid % nf;
+    const int g = (bid / nf) % GROUPS;
+    const int tile = bid / (nf * GROUPS);
+    const int r = g * SEQ + sq;  // row pair within the tile
+    const uint8_t *fr = ps.frames + (long long)f * ps.frame_bytes;
+    const int x0 = ps.x_off + (tile % tps) * S + q;
+    const int y0 = ps.y_off + (tile / tps) * S + 2 * r;
+    float2 v[E1];
+    load_row_pair<PIX, E1, L>(fr, ps, x0, y0, v);
+    float2 *seqbuf = lds + sq * S;
+    big_fft<S>(v, seqbuf, q, twg);
+    rows_to_xt<S>(seqbuf, xt, f, tiles, tile, r, q, xcols);
 }
 
 template <int S, int PIX>
@@ -819,6 +913,14 @@ __global__ __launch_bounds__(BigCfg<S>::NTH_C) void tile_fft_cols_reg(RingDst rd
 }
 
 template <int S>
+static void launch_cols(const RingDst &rd, int tiles, int n_frames, const float2 *tw, const float2 *xt,
+                        hipStream_t st) {
+    using C = BigCfg<S>;
+    const dim3 g2((unsigned)((C::H / C::SEQ_C) * tiles * n_frames)), block2(C::NTH_C);
+    hipLaunchKernelGGL(tile_fft_cols_reg<S>, g2, block2, 0, st, rd, tiles, xt, tw);
+}
+
+template <int S>
 static void launch_big(const PixSrc &ps, const RingDst &rd, int tps, int tiles, int n_frames, const float2 *tw,
                        float2 *xt, hipStream_t st) {
     using C = BigCfg<S>;
@@ -833,8 +935,7 @@ static void launch_big(const PixSrc &ps, const RingDst &rd, int tps, int tiles, 
         break;
     default: hipLaunchKernelGGL((tile_fft_rows_reg<S, PIX_ANY>), g1, block, 0, st, ps, tps, tiles, xt, tw, xcols);
     }
-    const dim3 g2((unsigned)((C::H / C::SEQ_C) * tiles * n_frames)), block2(C::NTH_C);
-    hipLaunchKernelGGL(tile_fft_cols_reg<S>, g2, block2, 0, st, rd, tiles, xt, tw);
+    launch_cols<S>(rd, tiles, n_frames, tw, xt, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -998,6 +1099,359 @@ bool launch_tile_fft_reg(const PixSrc &ps, const RingDst &rd, int scale, int tps
         return true;
     default: return false;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Multi-scale tiler: each frame's pixels are read once for all 8 .. 128 tile
+// scales and once for the row passes of all 256 .. 1024 scales (the
+// reference cuts and transforms every scale from the frame separately,
+// src/DDM.cu:171-192 with parseBufferScalePow2, src/DDM_kernel.cuh:43-78).
+//
+// tile_fft_blocks: a workgroup per (frame, 128 x 128 block of the region):
+// the block's pixels are staged once in LDS as 16-bit values, then the
+// register transforms of every scale run on it in turn (the 128 tile, the
+// four 64 tiles, the sixteen 32 tiles, ...) and store to the ring exactly as
+// their per-scale launches do, so the spectra are the same bit for bit.
+//
+// tile_fft_bands: a workgroup per (frame, band of 16384 / M rows across the
+// whole M-pixel-wide region): each thread loads 64 pixels once and keeps them
+// in registers; for every 256 .. 1024 scale (and every pass of 256 threads)
+// the band is restaged in LDS with that scale's row padding, each sequence
+// reads its row pair, and the row FFT and pass-buffer writes are those of
+// tile_fft_rows_reg.  The column passes then run per scale.
+// ---------------------------------------------------------------------------
+constexpr int BLK_NTH = 512;
+// LDS of the bodies run on a block (complex values): the largest is the
+// 16-tile lane body (64 tiles of 16 x 9)
+constexpr int BLK_SLOTS = 64 * 16 * 9;
+static_assert(BLK_SLOTS >= 128 * 65 && BLK_SLOTS >= 4 * 64 * 33 && BLK_SLOTS >= 16 * 32 * 17 &&
+                  BLK_SLOTS >= 128 * 8 * 5,
+              "every body's LDS fits");
+
+struct BlockScale {
+    RingDst rd;
+    int s, tps;
+};
+constexpr int MAX_BLOCK_SCALES = 5;  // 128, 64, 32, 16, 8
+struct BlockArgs {
+    PixSrc ps;
+    int bpr;   // blocks per region row: M / 128
+    int nblk;  // blocks per frame
+    BlockScale sc[MAX_BLOCK_SCALES];  // slot k: tile size 128 >> k, tps = 0 when absent
+};
+
+// Global loads of the tiler: 16-byte and 4-byte words (every row piece
+// aligned to that size), or pixel by pixel through load_px.
+enum { LD_PX = 0, LD_4 = 1, LD_16 = 2 };
+
+// 16 8-bit pixels (four little-endian words) as 16-bit values in eight words
+__device__ __forceinline__ uint2 widen_u8(uint32_t w) {
+    return make_uint2((w & 0xffu) | ((w & 0xff00u) << 8), ((w >> 16) & 0xffu) | ((w >> 8) & 0xff0000u));
+}
+
+// The block at region coordinates (X0, Y0) of frame fr into blk (BLK rows of
+// BLK_PITCH 16-bit values), by BLK_NTH threads.
+template <int PIX, int LD>
+__device__ __forceinline__ void stage_block(const uint8_t *fr, const PixSrc &ps, int X0, int Y0, uint16_t *blk,
+                                            int tid) {
+    const uint8_t *base = fr + (long long)(ps.y_off + Y0) * ps.row_bytes + (long long)(ps.x_off + X0) * ps.bpp;
+    if constexpr (PIX == PIX_U16LE_2 && LD == LD_16) {
+        // 16 pieces of 16 bytes per row, 4 per thread
+        uint4 u[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = tid + BLK_NTH * k, y = i >> 4, c = i & 15;
+            u[k] = *reinterpret_cast<const uint4 *>(base + (long long)y * ps.row_bytes + c * 16);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = tid + BLK_NTH * k, y = i >> 4, c = i & 15;
+            uint2 *d = reinterpret_cast<uint2 *>(blk + y * BLK_PITCH + c * 8);
+            d[0] = make_uint2(u[k].x, u[k].y);
+            d[1] = make_uint2(u[k].z, u[k].w);
+        }
+    } else if constexpr (PIX == PIX_U8_1 && LD == LD_16) {
+        // 8 pieces of 16 bytes per row, 2 per thread
+        uint4 u[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int i = tid + BLK_NTH * k, y = i >> 3, c = i & 7;
+            u[k] = *reinterpret_cast<const uint4 *>(base + (long long)y * ps.row_bytes + c * 16);
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int i = tid + BLK_NTH * k, y = i >> 3, c = i & 7;
+            uint2 *d = reinterpret_cast<uint2 *>(blk + y * BLK_PITCH + c * 16);
+            d[0] = widen_u8(u[k].x);
+            d[1] = widen_u8(u[k].y);
+            d[2] = widen_u8(u[k].z);
+            d[3] = widen_u8(u[k].w);
+        }
+    } else if constexpr (PIX == PIX_U16LE_2 && LD == LD_4) {
+        // 64 words per row, 16 per thread
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int i = tid + BLK_NTH * k, y = i >> 6, c = i & 63;
+            const uint32_t w = *reinterpret_cast<const uint32_t *>(base + (long long)y * ps.row_bytes + c * 4);
+            *reinterpret_cast<uint32_t *>(blk + y * BLK_PITCH + c * 2) = w;
+        }
+    } else if constexpr (PIX == PIX_U8_1 && LD == LD_4) {
+        // 32 words per row, 8 per thread
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int i = tid + BLK_NTH * k, y = i >> 5, c = i & 31;
+            const uint32_t w = *reinterpret_cast<const uint32_t *>(base + (long long)y * ps.row_bytes + c * 4);
+            *reinterpret_cast<uint2 *>(blk + y * BLK_PITCH + c * 4) = widen_u8(w);
+        }
+    } else {
+        // 128 pixels per row, 32 per thread, consecutive threads on consecutive pixels
+#pragma unroll 8
+        for (int k = 0; k < 32; ++k) {
+            const int i = tid + BLK_NTH * k, y = i >> 7, x = i & 127;
+            blk[y * BLK_PITCH + x] = (uint16_t)load_px(fr, ps, ps.x_off + X0 + x, ps.y_off + Y0 + y);
+        }
+    }
+}
+
+template <int PIX, int LD>
+__global__ __launch_bounds__(BLK_NTH) void tile_fft_blocks(BlockArgs a, const float2 *__restrict__ twg) {
+    __shared__ __align__(16) uint16_t blk[BLK * BLK_PITCH];
+    __shared__ float2 lds[BLK_SLOTS];
+    __shared__ int rtab[BLK + 1];
+    const int tid = threadIdx.x;
+    // frames vary fastest across the grid, as in the per-scale launches
+    const int nf = gridDim.x / a.nblk;
+    const int f = blockIdx.x % nf, b = blockIdx.x / nf;
+    const int bx = b % a.bpr, by = b / a.bpr;
+    const int X0 = bx * BLK, Y0 = by * BLK;
+    stage_block<PIX, LD>(a.ps.frames + (long long)f * a.ps.frame_bytes, a.ps, X0, Y0, blk, tid);
+    __syncthreads();
+    const BlockPix src{blk, X0, Y0};
+    // each scale's tiles of the block in turn, largest first; slot k of
+    // a.sc holds tile size 128 >> k (tps = 0: scale absent)
+    auto tiles_of = [&](int k) { return TileSet{by * (1 << k) * a.sc[k].tps + bx * (1 << k), k, 1 << (2 * k)}; };
+    if (a.sc[0].tps) {
+        reg_tiles<128>(src, a.sc[0].rd, a.sc[0].tps, tiles_of(0), twg, lds, rtab, tid, f);
+        __syncthreads();  // the next scale reuses lds and rtab
+    }
+    if (a.sc[1].tps) {
+        reg_tiles<64>(src, a.sc[1].rd, a.sc[1].tps, tiles_of(1), twg, lds, rtab, tid, f);
+        __syncthreads();
+    }
+    if (a.sc[2].tps) {
+        // sixteen 32 tiles take 256 of the 512 threads
+        lane_tiles<32, true, 256, BLK_NTH>(src, a.sc[2].rd, a.sc[2].tps, tiles_of(2), lds, rtab, tid, f);
+        __syncthreads();
+    }
+    if (a.sc[3].tps) {
+        lane_tiles<16, true, BLK_NTH, BLK_NTH>(src, a.sc[3].rd, a.sc[3].tps, tiles_of(3), lds, rtab, tid, f);
+        __syncthreads();
+    }
+    if (a.sc[4].tps) {
+        // 256 tiles: two rounds of 128 (8 rows of 16 tiles each)
+        const TileSet ts = tiles_of(4);
+        lane_tiles<8, true, BLK_NTH, BLK_NTH>(src, a.sc[4].rd, a.sc[4].tps, TileSet{ts.first, 4, 128}, lds, rtab, tid,
+                                              f);
+        __syncthreads();
+        lane_tiles<8, true, BLK_NTH, BLK_NTH>(src, a.sc[4].rd, a.sc[4].tps, TileSet{ts.first + 8 * a.sc[4].tps, 4, 128},
+                                              lds, rtab, tid, f);
+    }
+}
+
+constexpr int BAND_NTH = 256;
+constexpr int BAND_PX = 16384;  // pixels per band: M columns x 16384 / M rows
+struct BandScale {
+    float2 *xt;  // this scale's pass buffer region
+    int s, tps, xcols;
+};
+constexpr int MAX_BAND_SCALES = 3;  // 1024, 512, 256
+struct BandArgs {
+    PixSrc ps;
+    int M;
+    int nband;  // bands per frame
+    BandScale sc[MAX_BAND_SCALES];  // slot k: tile size 1024 >> k, tps = 0 when absent
+};
+
+// This thread's 64 pixels (row y, columns x .. x + 63 of the region) as
+// 16-bit values, two per word.
+template <int PIX, int LD>
+__device__ __forceinline__ void load_band(const uint8_t *fr, const PixSrc &ps, int x, int y, uint32_t (&raw)[32]) {
+    const uint8_t *p = fr + (long long)(ps.y_off + y) * ps.row_bytes + (long long)(ps.x_off + x) * ps.bpp;
+    if constexpr (PIX == PIX_U16LE_2 && LD == LD_16) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint4 u = reinterpret_cast<const uint4 *>(p)[k];
+            raw[4 * k] = u.x;
+            raw[4 * k + 1] = u.y;
+            raw[4 * k + 2] = u.z;
+            raw[4 * k + 3] = u.w;
+        }
+    } else if constexpr (PIX == PIX_U8_1 && LD == LD_16) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint4 u = reinterpret_cast<const uint4 *>(p)[k];
+            const uint2 a = widen_u8(u.x), b = widen_u8(u.y), c = widen_u8(u.z), d = widen_u8(u.w);
+            raw[8 * k] = a.x;
+            raw[8 * k + 1] = a.y;
+            raw[8 * k + 2] = b.x;
+            raw[8 * k + 3] = b.y;
+            raw[8 * k + 4] = c.x;
+            raw[8 * k + 5] = c.y;
+            raw[8 * k + 6] = d.x;
+            raw[8 * k + 7] = d.y;
+        }
+    } else if constexpr (PIX == PIX_U16LE_2 && LD == LD_4) {
+#pragma unroll
+        for (int k = 0; k < 32; ++k) raw[k] = reinterpret_cast<const uint32_t *>(p)[k];
+    } else if constexpr (PIX == PIX_U8_1 && LD == LD_4) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint2 a = widen_u8(reinterpret_cast<const uint32_t *>(p)[k]);
+            raw[2 * k] = a.x;
+            raw[2 * k + 1] = a.y;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const uint32_t lo = (uint32_t)load_px(fr, ps, ps.x_off + x + 2 * k, ps.y_off + y);
+            const uint32_t hi = (uint32_t)load_px(fr, ps, ps.x_off + x + 2 * k + 1, ps.y_off + y);
+            raw[k] = lo | (hi << 16);
+        }
+    }
+}
+
+// The row passes of scale S for band rows Y0 .. Y0 + 16384 / M of frame f.
+// The band goes to LDS with rows of M + PAD 16-bit values, PAD chosen so
+// that the sequences of one wave (consecutive row pairs) read distinct
+// banks: 8 dwords apart for 16-lane sequences, 16 for 32-lane ones.
+template <int S>
+__device__ __forceinline__ void band_rows(const uint32_t (&raw)[32], int M, const BandScale &sc, float2 *lds,
+                                          const float2 *__restrict__ twg, int tid, int f, int Y0, int ry, int cx) {
+    using C = BigCfg<S>;
+    constexpr int L = C::L, E1 = C::E1;
+    constexpr int PAD = L == 32 ? 16 : 8;
+    constexpr int SEQ = BAND_NTH / L;  // sequences per pass
+    // sequences of the band: NP row pairs x M / S tiles = 8192 / S, in
+    // passes of SEQ
+    constexpr int PASSES = 8192 / S / SEQ;
+    static_assert(PASSES >= 1 && PASSES * SEQ * S == 8192, "whole passes");
+    const int PW = M + PAD;
+    const int NP = BAND_PX / M / 2;  // row pairs per band
+    uint16_t *band = reinterpret_cast<uint16_t *>(lds);
+    const int q = tid & (L - 1);
+#pragma unroll
+    for (int p = 0; p < PASSES; ++p) {
+        __syncthreads();  // the previous pass is done with lds
+        {
+            uint4 *d = reinterpret_cast<uint4 *>(band + ry * PW + cx);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) d[k] = make_uint4(raw[4 * k], raw[4 * k + 1], raw[4 * k + 2], raw[4 * k + 3]);
+        }
+        __syncthreads();
+        const int sq = p * SEQ + tid / L;
+        const int r = sq % NP, tx = sq / NP;  // row pair within the band, tile column
+        float2 v[E1];
+        {
+            const uint16_t *r0 = band + (2 * r) * PW + tx * S + q;
+#pragma unroll
+            for (int n1 = 0; n1 < E1; ++n1) v[n1] = make_float2((float)r0[n1 * L], (float)r0[PW + n1 * L]);
+        }
+        __syncthreads();  // every sequence has its pixels before the FFTs use lds
+        float2 *seqbuf = lds + (tid / L) * S;
+        big_fft<S>(v, seqbuf, q, twg);
+        const int tile = (Y0 / S) * sc.tps + tx;
+        rows_to_xt<S>(seqbuf, sc.xt, f, sc.tps * sc.tps, tile, (Y0 % S) / 2 + r, q, sc.xcols);
+    }
+}
+
+template <int PIX, int LD>
+__global__ __launch_bounds__(BAND_NTH) void tile_fft_bands(BandArgs a, const float2 *__restrict__ twg) {
+    __shared__ __align__(16) float2 lds[8192];  // the band, then the sequences (64 KB)
+    const int tid = threadIdx.x;
+    const int M = a.M;
+    const int nf = gridDim.x / a.nband;
+    const int f = blockIdx.x % nf, band = blockIdx.x / nf;
+    const int Y0 = band * (BAND_PX / M);
+    const int tpr = M / 64;  // threads per band row
+    const int ry = tid / tpr, cx = (tid % tpr) * 64;
+    uint32_t raw[32];
+    load_band<PIX, LD>(a.ps.frames + (long long)f * a.ps.frame_bytes, a.ps, cx, Y0 + ry, raw);
+    // slot k of a.sc holds tile size 1024 >> k (tps = 0: scale absent)
+    if (a.sc[0].tps) band_rows<1024>(raw, M, a.sc[0], lds, twg, tid, f, Y0, ry, cx);
+    if (a.sc[1].tps) band_rows<512>(raw, M, a.sc[1], lds, twg, tid, f, Y0, ry, cx);
+    if (a.sc[2].tps) band_rows<256>(raw, M, a.sc[2], lds, twg, tid, f, Y0, ry, cx);
+}
+
+static int tiler_load_mode(const PixSrc &ps, int pix) {
+    if (pix == PIX_ANY) return LD_PX;
+    const long long bits = ps.row_bytes | ps.frame_bytes | (long long)(uintptr_t)ps.frames | (long long)ps.x_off * ps.bpp;
+    return (bits & 15) == 0 ? LD_16 : ((bits & 3) == 0 ? LD_4 : LD_PX);
+}
+
+template <class K>
+static void tiler_dispatch(const PixSrc &ps, K launch) {
+    const int pix = pixel_mode(ps), ld = tiler_load_mode(ps, pix);
+    if (pix == PIX_U8_1 && ld == LD_16) launch(std::integral_constant<int, PIX_U8_1>{}, std::integral_constant<int, LD_16>{});
+    else if (pix == PIX_U8_1 && ld == LD_4) launch(std::integral_constant<int, PIX_U8_1>{}, std::integral_constant<int, LD_4>{});
+    else if (pix == PIX_U16LE_2 && ld == LD_16) launch(std::integral_constant<int, PIX_U16LE_2>{}, std::integral_constant<int, LD_16>{});
+    else if (pix == PIX_U16LE_2 && ld == LD_4) launch(std::integral_constant<int, PIX_U16LE_2>{}, std::integral_constant<int, LD_4>{});
+    else launch(std::integral_constant<int, PIX_ANY>{}, std::integral_constant<int, LD_PX>{});
+}
+
+bool tiler_block_scale(int scale, int M) { return scale >= 8 && scale <= BLK && M >= BLK; }
+bool tiler_band_scale(int scale, int M) {
+    return (scale == 256 || scale == 512 || scale == 1024) && M <= BAND_PX / 2;
+}
+
+hipError_t launch_tiler_blocks(const PixSrc &ps, const TilerScale *sc, int n, int M, int n_frames, const float2 *tw,
+                               hipStream_t st) {
+    if (n < 1 || n > MAX_BLOCK_SCALES || M < BLK || M % BLK) return hipErrorInvalidValue;
+    BlockArgs a{};
+    a.ps = ps;
+    a.bpr = M / BLK;
+    a.nblk = a.bpr * a.bpr;
+    for (int i = 0; i < n; ++i) {
+        if (!tiler_block_scale(sc[i].s, M) || (i && sc[i].s >= sc[i - 1].s)) return hipErrorInvalidValue;
+        a.sc[ilog2(BLK / sc[i].s)] = BlockScale{sc[i].rd, sc[i].s, M / sc[i].s};
+    }
+    const long long blocks = (long long)a.nblk * n_frames;
+    if (blocks < 1 || blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    tiler_dispatch(ps, [&](auto pix, auto ld) {
+        hipLaunchKernelGGL((tile_fft_blocks<decltype(pix)::value, decltype(ld)::value>), dim3((unsigned)blocks),
+                           dim3(BLK_NTH), 0, st, a, tw);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_tiler_bands(const PixSrc &ps, const TilerScale *sc, int n, int M, int n_frames, const float2 *tw,
+                              hipStream_t st) {
+    if (n < 1 || n > MAX_BAND_SCALES || M < 256 || M > BAND_PX / 2 || (M & (M - 1))) return hipErrorInvalidValue;
+    BandArgs a{};
+    a.ps = ps;
+    a.M = M;
+    a.nband = M / (BAND_PX / M);
+    for (int i = 0; i < n; ++i) {
+        if (!tiler_band_scale(sc[i].s, M) || sc[i].s > M || !sc[i].xt || (i && sc[i].s >= sc[i - 1].s))
+            return hipErrorInvalidValue;
+        a.sc[ilog2(1024 / sc[i].s)] = BandScale{sc[i].xt, sc[i].s, M / sc[i].s, sc[i].rd.lay.pass_cols};
+    }
+    const long long blocks = (long long)a.nband * n_frames;
+    if (blocks < 1 || blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    tiler_dispatch(ps, [&](auto pix, auto ld) {
+        hipLaunchKernelGGL((tile_fft_bands<decltype(pix)::value, decltype(ld)::value>), dim3((unsigned)blocks),
+                           dim3(BAND_NTH), 0, st, a, tw);
+    });
+    // the column passes, per scale
+    for (int i = 0; i < n; ++i) {
+        const int tps = M / sc[i].s, tiles = tps * tps;
+        switch (sc[i].s) {
+        case 256: launch_cols<256>(sc[i].rd, tiles, n_frames, tw, sc[i].xt, st); break;
+        case 512: launch_cols<512>(sc[i].rd, tiles, n_frames, tw, sc[i].xt, st); break;
+        case 1024: launch_cols<1024>(sc[i].rd, tiles, n_frames, tw, sc[i].xt, st); break;
+        default: break;
+        }
+    }
+    return hipGetLastError();
 }
 
 }  // namespace ddm

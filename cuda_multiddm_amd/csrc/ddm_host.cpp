@@ -249,9 +249,8 @@ size_t device_footprint(const Params &p) {
     // chunk: counted at that size, so an admitted stream never outgrows it
     const int batch = default_fft_batch(p.chunk);
     bytes += UPLOAD_STAGES * (size_t)(batch - p.chunk) * p.M * p.M * p.bpp;
-    size_t xt = 0;
-    for (const ScaleGeom &g : p.geom) xt = std::max(xt, (size_t)big_fft_scratch_elems(p.M, g.s, batch) * 8);
-    return bytes + xt;
+    // (one region per 256 .. 1024 scale: the multi-scale tiler's size)
+    return bytes + (size_t)fft_pass_elems(p.M, p.scales, batch, true) * 8;
 }
 
 // Member lists of one (scale, lambda, tolerance) ring, cached for the life of

@@ -35,6 +35,17 @@ inline long long big_fft_scratch_elems(int M, int scale, int n) {
     return (scale == 256 || scale == 512 || scale == 1024) ? (long long)n * M * M / 2 : 0;
 }
 
+// The whole pass buffer of a batch of n frames: one scale's region when the
+// 256 .. 1024 scales take turns (per-scale launches), one region per scale
+// when the multi-scale tiler's row bands fill them all before the column
+// passes (two or more such scales and fused = true).
+inline long long fft_pass_elems(int M, const std::vector<int> &scales, int n, bool fused) {
+    int big = 0;
+    for (int s : scales) big += big_fft_scratch_elems(M, s, 1) > 0;
+    const long long one = (long long)n * M * M / 2;
+    return big == 0 ? 0 : (fused && big >= 2 ? big * one : one);
+}
+
 // The part of an s x s tile's half spectrum [ky][kx <= s/2] a spectrum ring
 // holds, and its order.  Only elements some q ring contains reach the
 // averages, so an engine may keep just a superset of those:

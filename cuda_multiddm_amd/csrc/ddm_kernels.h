@@ -123,6 +123,13 @@ int engine_batch_stages(ddm_engine *e, int k);
 // FP64 (on st) when due.
 int engine_batch_commit(ddm_engine *e, int n, int b, hipStream_t st);
 hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st);
+// The transforms of every scale of one batch (a[0 .. n), descending, the
+// same frames; a[i].xt the start of a pass buffer of fft_pass_elems values).
+// With fused, the multi-scale tiler takes the scales it covers when two or
+// more of a kind are present (8 .. 128 register transforms; 256 .. 1024),
+// the others are launched per scale.  *launches (optional) counts kernel
+// launches.  The spectra are the same bit for bit either way.
+hipError_t launch_tile_fft_scales(const TileFftArgs *a, int n, bool fused, hipStream_t st, int *launches);
 hipError_t launch_lag(const LagArgs &a, hipStream_t st);
 hipError_t launch_azimuthal(const AzArgs &a, hipStream_t st);
 // acc64[i] += acc[i] + comp[i]; acc[i] = comp[i] = 0 for i < n (n a multiple of 4).

@@ -21,6 +21,7 @@
 // written.  Twiddles come from a table rounded once from double precision.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <mutex>
 #include <vector>
@@ -584,6 +585,65 @@ hipError_t launch_tile_fft(const TileFftArgs &a, hipStream_t st) {
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+static bool tiler_block_arg(const TileFftArgs &a) {
+    const bool mfma = a.precision != DDM_PREC_FP32 && (a.scale == 64 || a.scale == 128);
+    return !mfma && a.family != DDM_FFT_LDS && tiler_block_scale(a.scale, a.largest_scale);
+}
+
+hipError_t launch_tile_fft_scales(const TileFftArgs *a, int n, bool fused, hipStream_t st, int *launches) {
+    int nl = 0;
+    std::vector<int> blk, band;
+    if (fused) {
+        for (int i = 0; i < n; ++i) {
+            if (tiler_block_arg(a[i])) blk.push_back(i);
+            else if (a[i].xt && tiler_band_scale(a[i].scale, a[i].largest_scale)) band.push_back(i);
+        }
+        if (blk.size() < 2) blk.clear();
+        if (band.size() < 2) band.clear();
+    }
+    for (int i = 0; i < n; ++i) {
+        if (std::find(blk.begin(), blk.end(), i) != blk.end() || std::find(band.begin(), band.end(), i) != band.end())
+            continue;
+        const hipError_t e = launch_tile_fft(a[i], st);
+        if (e != hipSuccess) return e;
+        nl += a[i].scale >= 256 ? 2 : 1;
+    }
+    if (blk.empty() && band.empty()) {
+        if (launches) *launches += nl;
+        return hipSuccess;
+    }
+    const float2 *tw = nullptr;
+    hipError_t e = twiddle_table(&tw);
+    if (e != hipSuccess) return e;
+    const TileFftArgs &a0 = a[blk.empty() ? band[0] : blk[0]];
+    const PixSrc ps{(const uint8_t *)a0.frames, a0.frame_bytes, (long long)a0.width * a0.bpp, a0.bpp, a0.fmt,
+                    a0.x_off, a0.y_off};
+    const int M = a0.largest_scale;
+    auto scale_of = [&](int i, float2 *xt) {
+        const TileLayout lay = a[i].lay.elems ? a[i].lay : TileLayout::full(a[i].scale);
+        return TilerScale{RingDst{(float2 *)a[i].ring, a[i].elem_off, a[i].ring_slots, a[i].first_slot, lay},
+                          a[i].scale, xt};
+    };
+    if (!blk.empty()) {
+        TilerScale sc[5];
+        for (size_t k = 0; k < blk.size(); ++k) sc[k] = scale_of(blk[k], nullptr);
+        e = launch_tiler_blocks(ps, sc, (int)blk.size(), M, a0.n_frames, tw, st);
+        if (e != hipSuccess) return e;
+        nl += 1;
+    }
+    if (!band.empty()) {
+        // one pass-buffer region per scale
+        const long long region = big_fft_scratch_elems(M, a[band[0]].scale, a0.n_frames);
+        TilerScale sc[3];
+        for (size_t k = 0; k < band.size(); ++k) sc[k] = scale_of(band[k], (float2 *)a[band[k]].xt + k * region);
+        e = launch_tiler_bands(ps, sc, (int)band.size(), M, a0.n_frames, tw, st);
+        if (e != hipSuccess) return e;
+        nl += 1 + (int)band.size();
+    }
+    if (launches) *launches += nl;
+    return hipSuccess;
 }
 
 template <int WPR, int SRC>

@@ -31,6 +31,7 @@ PREC_FP32, PREC_BF16X2, PREC_FP16X2 = 0, 1, 2
 ACC_COMPENSATED, ACC_PLAIN = 0, 1
 SPECTRUM_MASKED, SPECTRUM_FULL = 0, 1
 FFT_REGISTER, FFT_LDS = 0, 1
+TILER_FUSED, TILER_PER_SCALE = 0, 1
 SERVER_BATCHED, SERVER_NO_GRAPH, SERVER_GROUP = 1, 2, 4
 
 # Every symbol include/ddm_amd.h declares.
@@ -44,7 +45,7 @@ EXPORTED = [
     "ddm_server_create", "ddm_server_destroy", "ddm_server_open", "ddm_server_submit", "ddm_server_results",
     "ddm_server_engine", "ddm_server_drain", "ddm_server_close", "ddm_server_latencies", "ddm_server_stats",
     "ddm_server_last_error", "ddm_server_create2", "ddm_server_set_hold", "ddm_fit_results", "ddm_k_fit", "ddm_set_precision", "ddm_autotune", "ddm_reserve",
-    "ddm_set_fft_family", "ddm_get_fft_families",
+    "ddm_set_fft_family", "ddm_get_fft_families", "ddm_set_tiler", "ddm_get_tiler",
 ]
 
 
@@ -123,6 +124,8 @@ def lib():
     L.ddm_autotune.argtypes = [vp, ctypes.c_char_p, ip, ip]
     L.ddm_set_fft_family.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.ddm_get_fft_families.argtypes = [vp, ip]
+    L.ddm_set_tiler.argtypes = [vp, ctypes.c_int]
+    L.ddm_get_tiler.argtypes = [vp]
     L.ddm_reserve.argtypes = [vp, ctypes.c_int]
     L.ddm_reset.argtypes = [vp]
     L.ddm_get_info.argtypes = [vp, ctypes.POINTER(Info)]
@@ -326,6 +329,15 @@ class Engine:
         out = (ctypes.c_int * len(self.scales))()
         self._check(lib().ddm_get_fft_families(self._h, out))
         return list(out)
+
+    def set_tiler(self, mode: int):
+        """Multi-scale tiler (TILER_FUSED: each frame read once for the 8..128
+        scales and once for the 256..1024 row passes; TILER_PER_SCALE: one
+        transform launch per scale), before the first frame."""
+        self._check(lib().ddm_set_tiler(self._h, int(mode)))
+
+    def tiler(self) -> int:
+        return int(lib().ddm_get_tiler(self._h))
 
     def reserve(self, max_frames: int):
         """Allocates now the transient buffers a submission of up to

@@ -105,6 +105,16 @@ enum ddm_spectrum { DDM_SPECTRUM_MASKED = 0, DDM_SPECTRUM_FULL = 1 };
  * it).  Both compute the same transform; the autotuner keeps the faster. */
 enum ddm_fft_family { DDM_FFT_REGISTER = 0, DDM_FFT_LDS = 1 };
 
+/* Multi-scale tiler (ddm_set_tiler).  DDM_TILER_FUSED: each frame's pixels
+ * are read once for all 8 .. 128 tile scales (128 x 128 blocks staged in
+ * LDS, every scale's tiles cut from there) and once for the row passes of
+ * all 256 .. 1024 scales (row bands), instead of once per scale; the FFT
+ * pass buffer then holds one region per 256 .. 1024 scale.
+ * DDM_TILER_PER_SCALE (default): one transform launch per scale, each
+ * reading the frame, as the reference does (src/DDM.cu:171-192).  The
+ * spectra are the same bit for bit. */
+enum ddm_tiler { DDM_TILER_FUSED = 0, DDM_TILER_PER_SCALE = 1 };
+
 typedef struct ddm_desc {
     int width;            /* frame width in pixels */
     int height;           /* frame height in pixels */
@@ -245,6 +255,14 @@ int ddm_set_fft_family(ddm_engine *eng, int scale_idx, int family);
 
 /* Each scale's transform family (scale_count entries, scales descending). */
 int ddm_get_fft_families(const ddm_engine *eng, int *families);
+
+/* The multi-scale tiler mode (enum ddm_tiler), before the first frame.
+ * Replaces the per-scale cut of parseBufferScalePow2 in the chunk loop
+ * (src/DDM.cu:171-192, src/DDM_kernel.cuh:43-78). */
+int ddm_set_tiler(ddm_engine *eng, int mode);
+
+/* The engine's tiler mode (enum ddm_tiler), or DDM_EINVAL. */
+int ddm_get_tiler(const ddm_engine *eng);
 
 /* Allocates now the transient device buffers that a submission of up to
  * max_frames frames per call would otherwise allocate on first use (the FFT
