@@ -354,9 +354,10 @@ def main():
                     help="lag sums: compensated FP32 folded into FP64 every 10^4 pairs, or plain FP32")
     ap.add_argument("--spectrum", default="masked", choices=["masked", "full"],
                     help="spectrum elements kept: the band holding every q-ring member, or all of them")
-    ap.add_argument("--tiler", default="per-scale", choices=["fused", "per-scale"],
+    ap.add_argument("--tiler", default="auto", choices=["auto", "fused", "per-scale"],
                     help="multi-scale tiler: each frame read once for the 8..128 scales and once for the 256..1024 "
-                         "row passes, or one transform launch per scale")
+                         "row passes (fused), or one transform launch per scale; auto: the autotuner's choice "
+                         "(per scale with --no-autotune)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the engine's default schedule instead of the autotuner's choice of lag block, FFT "
                          "batch and transform families (ddm_autotune, run at engine set-up)")
@@ -424,12 +425,14 @@ def main():
                        accumulation=ddm.ACC_PLAIN if args.accumulation == "plain" else ddm.ACC_COMPENSATED,
                        spectrum=SPECTRUM[args.spectrum])
         e.set_precision(prec)
-        e.set_tiler(ddm.TILER_FUSED if args.tiler == "fused" else ddm.TILER_PER_SCALE)
         if args.autotune:
             # timed on the first engine, read back from the cache by the others
             # (before any frame, outside the timed region)
             schedule["lag_block"], schedule["fft_batch_frames"] = e.autotune(tune_cache)
             schedule["fft_families"] = ["lds" if f == ddm.FFT_LDS else "register" for f in e.fft_families()]
+        if args.tiler != "auto":
+            e.set_tiler(ddm.TILER_FUSED if args.tiler == "fused" else ddm.TILER_PER_SCALE)
+        schedule["tiler"] = "fused" if e.tiler() == ddm.TILER_FUSED else "per-scale"
         # buffers the first submission would allocate (pass buffer, staging)
         # belong to engine creation, outside the timed region
         e.reserve(N)
@@ -528,7 +531,7 @@ def main():
             "fft_batch_frames": schedule.get("fft_batch_frames", W.chunk * max(1, 128 // W.chunk)),
             "lag_block": lag_block, "schedule": "autotuned" if args.autotune else "default",
             "fft_families": schedule.get("fft_families", ["register"] * len(W.scales)),
-            "precision": args.precision, "accumulation": args.accumulation, "spectrum": args.spectrum, "tiler": args.tiler,
+            "precision": args.precision, "accumulation": args.accumulation, "spectrum": args.spectrum, "tiler": schedule.get("tiler"),
         },
         "stage_ms_per_step": {"fft": round(st.fft_ms, 3), "lag": round(st.lag_ms, 3),
                               "azimuthal": round(st.azimuthal_ms, 3)},

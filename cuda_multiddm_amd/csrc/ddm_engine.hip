@@ -423,8 +423,8 @@ static int flush_lags(ddm_engine *e) {
 // The between-pass buffer of the 256 / 512 tile FFTs, large enough for a
 // batch of n frames.  Grown on the engine's stream, so work already queued
 // on it keeps the old buffer until it is done.
-static int ensure_xt(ddm_engine *e, int n) {
-    const long long need = ddm::fft_pass_elems(e->p.M, e->p.scales, n, e->tiler == DDM_TILER_FUSED);
+static int ensure_xt(ddm_engine *e, int n, bool fused) {
+    const long long need = ddm::fft_pass_elems(e->p.M, e->p.scales, n, fused);
     if (need <= e->xt_elems) return DDM_OK;
     if (e->d_xt) {
         HIP_TRY(e, hipFreeAsync(e->d_xt, e->st));
@@ -447,7 +447,7 @@ static int process_chunk(ddm_engine *e, const uint8_t *src, int n, long long fra
     const long long t0 = e->frames;
     const int R = p.ring_slots;
     const int first_slot = (int)(t0 % R);
-    int rc = ensure_xt(e, n);
+    int rc = ensure_xt(e, n, e->tiler == DDM_TILER_FUSED);
     if (rc) return rc;
     Mark chunk, fft;
     mark_begin(e, &chunk, MK_CHUNK);
@@ -556,7 +556,7 @@ extern "C" int ddm_reserve(ddm_engine *e, int max_frames) {
     if (max_frames < 0) return fail(e, DDM_EINVAL, "bad frame count");
     const int chunk = e->p.chunk;
     const int batch = std::min(e->fft_batch, std::max(max_frames, 1));
-    int rc = ensure_xt(e, batch);
+    int rc = ensure_xt(e, batch, e->tiler == DDM_TILER_FUSED);
     if (rc) return rc;
     if (max_frames > e->stage_frames && e->stage_frames < e->fft_batch) {
         rc = grow_stage(e, std::min(e->fft_batch, (max_frames + chunk - 1) / chunk * chunk));
@@ -959,7 +959,7 @@ extern "C" int ddm_set_precision(ddm_engine *e, int mode) {
 // Autotuner: the lag block and the FFT batch of device-resident input are
 // timed on this engine's own buffers (synthetic frames, the real geometry)
 // and the fastest per frame is kept.  Results are cached per device and
-// geometry in a text file, one "key<TAB>lag_block fft_batch" line each.
+// geometry in a text file, one "key<TAB>lag_block fft_batch tiler families" line each.
 // ---------------------------------------------------------------------------
 static std::string tune_key(const ddm_engine *e) {
     const Params &p = e->p;
@@ -979,9 +979,9 @@ static std::string tune_key(const ddm_engine *e) {
     return k + hx;
 }
 
-// A cache line is `key<TAB>J B f0 f1 ...`: the lag block, the FFT batch and
-// the transform family of every scale (descending).
-static bool tune_lookup(const char *path, const std::string &key, int *J, int *B, std::vector<int> *fam) {
+// A cache line is `key<TAB>J B T f0 f1 ...`: the lag block, the FFT batch,
+// the tiler mode and the transform family of every scale (descending).
+static bool tune_lookup(const char *path, const std::string &key, int *J, int *B, int *T, std::vector<int> *fam) {
     if (!path) return false;
     FILE *f = std::fopen(path, "r");
     if (!f) return false;
@@ -990,8 +990,8 @@ static bool tune_lookup(const char *path, const std::string &key, int *J, int *B
     while (std::fgets(line, sizeof line, f)) {
         const char *tab = std::strchr(line, '\t');
         if (!tab || std::string(line, tab - line) != key) continue;
-        int j = 0, b = 0, used = 0;
-        if (std::sscanf(tab + 1, "%d %d%n", &j, &b, &used) != 2) continue;
+        int j = 0, b = 0, t = 0, used = 0;
+        if (std::sscanf(tab + 1, "%d %d %d%n", &j, &b, &t, &used) != 3) continue;
         std::vector<int> fl;
         const char *at = tab + 1 + used;
         int v = 0, n = 0;
@@ -1002,6 +1002,7 @@ static bool tune_lookup(const char *path, const std::string &key, int *J, int *B
         if (fl.size() != fam->size()) continue;
         *J = j;  // the last matching line wins
         *B = b;
+        *T = t;
         *fam = fl;
         found = true;
     }
@@ -1040,10 +1041,11 @@ extern "C" int ddm_autotune(ddm_engine *e, const char *cache_path, int *lag_bloc
     if (rc) return rc;
     const Params &p = e->p;
     const std::string key = tune_key(e);
-    int J = 0, B = 0;
+    int J = 0, B = 0, T = DDM_TILER_PER_SCALE;
     std::vector<int> fam(p.geom.size(), DDM_FFT_REGISTER);
-    bool cached = tune_lookup(cache_path, key, &J, &B, &fam) && J >= 32 && J <= p.lag_block && (J & (J - 1)) == 0 &&
-                  B >= p.chunk && B % p.chunk == 0 && B <= p.chunk + p.lag_block;
+    bool cached = tune_lookup(cache_path, key, &J, &B, &T, &fam) && J >= 32 && J <= p.lag_block &&
+                  (J & (J - 1)) == 0 && B >= p.chunk && B % p.chunk == 0 && B <= p.chunk + p.lag_block &&
+                  (T == DDM_TILER_FUSED || T == DDM_TILER_PER_SCALE);
     for (size_t si = 0; cached && si < fam.size(); ++si)
         cached = fam[si] == DDM_FFT_REGISTER || (fam[si] == DDM_FFT_LDS && family_choice(p.geom[si].s));
     if (!cached) {
@@ -1087,7 +1089,7 @@ extern "C" int ddm_autotune(ddm_engine *e, const char *cache_path, int *lag_bloc
         rc = dev_alloc(e, &frames, (size_t)(fb * bmax), "autotune frames");
         if (rc) return rc;
         HIP_TRY(e, hipMemsetAsync(frames, 0x5a, (size_t)(fb * bmax), e->st));
-        rc = ensure_xt(e, bmax);
+        rc = ensure_xt(e, bmax, true);  // room for the tiler's regions too
         if (rc) return rc;
         best = 1e30f;
         float def_ms = 1e30f;
@@ -1169,6 +1171,45 @@ extern "C" int ddm_autotune(ddm_engine *e, const char *cache_path, int *lag_bloc
             }
             if (t[DDM_FFT_LDS] < 0.98f * t[DDM_FFT_REGISTER]) fam[si] = DDM_FFT_LDS;
         }
+        // tiler: every scale (chosen families) over the chosen batches, one
+        // launch per scale against the multi-scale tiler
+        float tt[2] = {1e30f, 1e30f};
+        for (int tm : {DDM_TILER_PER_SCALE, DDM_TILER_FUSED}) {
+            auto launch = [&]() -> hipError_t {
+                std::vector<ddm::TileFftArgs> args(p.geom.size());
+                for (int done = 0; done < bmax; done += B) {
+                    for (size_t si = 0; si < p.geom.size(); ++si) {
+                        ddm::TileFftArgs &a = args[si];
+                        a.frames = frames + (long long)done * fb;
+                        a.n_frames = std::min(B, bmax - done);
+                        a.frame_bytes = fb;
+                        a.width = p.width;
+                        a.bpp = p.bpp;
+                        a.fmt = p.fmt;
+                        a.x_off = p.x_off;
+                        a.y_off = p.y_off;
+                        a.largest_scale = p.M;
+                        a.scale = p.geom[si].s;
+                        a.ring = e->d_ring;
+                        a.ring_slots = p.ring_slots;
+                        a.first_slot = 0;
+                        a.elems_per_frame = p.elems_padded;
+                        a.elem_off = p.geom[si].elem_off;
+                        a.lay = e->lay[si];
+                        a.family = fam[si];
+                        a.precision = e->precision;
+                        a.xt = e->d_xt;
+                    }
+                    const hipError_t err = ddm::launch_tile_fft_scales(args.data(), (int)args.size(),
+                                                                       tm == DDM_TILER_FUSED, e->st, nullptr);
+                    if (err != hipSuccess) return err;
+                }
+                return hipSuccess;
+            };
+            rc = time_ms(e, launch, &tt[tm]);
+            if (rc) return rc;
+        }
+        T = tt[DDM_TILER_FUSED] < 0.98f * tt[DDM_TILER_PER_SCALE] ? DDM_TILER_FUSED : DDM_TILER_PER_SCALE;
         HIP_TRY(e, hipFreeAsync(frames, e->st));
         e->dev_bytes -= (size_t)(fb * bmax);
         // the timing runs accumulated into the lag sums
@@ -1178,7 +1219,7 @@ extern "C" int ddm_autotune(ddm_engine *e, const char *cache_path, int *lag_bloc
         HIP_TRY(e, hipStreamSynchronize(e->st));
         if (cache_path) {
             if (FILE *f = std::fopen(cache_path, "a")) {
-                std::fprintf(f, "%s\t%d %d", key.c_str(), J, B);
+                std::fprintf(f, "%s\t%d %d %d", key.c_str(), J, B, T);
                 for (int v : fam) std::fprintf(f, " %d", v);
                 std::fprintf(f, "\n");
                 std::fclose(f);
@@ -1187,6 +1228,7 @@ extern "C" int ddm_autotune(ddm_engine *e, const char *cache_path, int *lag_bloc
     }
     e->lag_block = J;
     e->fft_batch = B;
+    e->tiler = T;
     e->family = fam;
     if (lag_block_out) *lag_block_out = J;
     if (fft_batch_out) *fft_batch_out = B;

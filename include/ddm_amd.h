@@ -235,11 +235,11 @@ int ddm_set_precision(ddm_engine *eng, int mode);
 
 /* Autotuner: before the first frame, times the lag-engine block (64, 128 or
  * 256 frames, at most the descriptor's lag_block), the FFT batch of
- * device-resident input (1, 2, 4 or 8 chunks) and, for every 8 .. 128 tile
- * scale, the tile transform family (enum ddm_fft_family: radix 16 / 32 in
- * registers or radix 4 / 2 in LDS) on this engine's buffers with synthetic
- * frames, and keeps the fastest per frame (a candidate must beat the default
- * by 2 %).  cache_path (may be NULL): text file of earlier choices, keyed by
+ * device-resident input (1, 2, 4 or 8 chunks), for every 8 .. 128 tile
+ * scale the tile transform family (enum ddm_fft_family: radix 16 / 32 in
+ * registers or radix 4 / 2 in LDS), and the tiler mode (enum ddm_tiler) on
+ * this engine's buffers with synthetic frames, and keeps the fastest per
+ * frame (a candidate must beat the default by 2 %).  cache_path (may be NULL): text file of earlier choices, keyed by
  * device and geometry; a hit skips the timing, a miss appends the new
  * choice.  Writes the choice to *lag_block / *fft_batch_frames (either may
  * be NULL; ddm_get_fft_families reads the families).  Results do not depend

@@ -259,10 +259,11 @@ def test_autotune_schedule_and_cache(gpu, tmp_path):
     torch.cuda.synchronize()
     with ddm.Engine(128, 128, scales, lags, lams, 1.2, 20) as e:
         j, b = e.autotune(cache)
-        fam = e.fft_families()
-        print(f"autotune: lag block {j}, FFT batch {b}, transform families {fam}", flush=True)
+        fam, tiler = e.fft_families(), e.tiler()
+        print(f"autotune: lag block {j}, FFT batch {b}, transform families {fam}, tiler {tiler}", flush=True)
         assert j in (64, 128, 256) and b in (20, 40, 80, 160)
         assert all(f in (ddm.FFT_REGISTER, ddm.FFT_LDS) for f in fam)
+        assert tiler in (ddm.TILER_FUSED, ddm.TILER_PER_SCALE)
         assert e.info().lag_block == j
         e.submit_device(t.data_ptr(), frames.shape[0])
         e.sync()
@@ -272,9 +273,9 @@ def test_autotune_schedule_and_cache(gpu, tmp_path):
     ref = o.run(o.Config(128, 128, frames.shape[0], scales, lags, lams, np.float32(1.2).item()), frames)
     assert o.max_rel_err(got, ref) <= TOL
     lines = open(cache).read().splitlines()
-    assert len(lines) == 1 and lines[0].endswith(f"\t{j} {b} " + " ".join(str(f) for f in fam))
+    assert len(lines) == 1 and lines[0].endswith(f"\t{j} {b} {tiler} " + " ".join(str(f) for f in fam))
     with ddm.Engine(128, 128, scales, lags, lams, 1.2, 20) as e2:
-        assert e2.autotune(cache) == (j, b) and e2.fft_families() == fam
+        assert e2.autotune(cache) == (j, b) and e2.fft_families() == fam and e2.tiler() == tiler
     assert len(open(cache).read().splitlines()) == 1  # a hit appends nothing
 
 
