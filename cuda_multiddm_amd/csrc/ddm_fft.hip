@@ -1213,6 +1213,42 @@ __device__ __forceinline__ void stage_block(const uint8_t *fr, const PixSrc &ps,
     }
 }
 
+// Every scale's tiles of block (bx, by) of batch frame f in turn, largest
+// first, pixels from src; slot k of a.sc holds tile size 128 >> k (tps = 0:
+// scale absent).
+template <bool A4, class Src>
+__device__ __forceinline__ void block_scales(const Src &src, const BlockArgs &a, const float2 *__restrict__ twg,
+                                             float2 *lds, int *rtab, int tid, int f, int bx, int by) {
+    auto tiles_of = [&](int k) { return TileSet{by * (1 << k) * a.sc[k].tps + bx * (1 << k), k, 1 << (2 * k)}; };
+    if (a.sc[0].tps) {
+        reg_tiles<128>(src, a.sc[0].rd, a.sc[0].tps, tiles_of(0), twg, lds, rtab, tid, f);
+        __syncthreads();  // the next scale reuses lds and rtab
+    }
+    if (a.sc[1].tps) {
+        reg_tiles<64>(src, a.sc[1].rd, a.sc[1].tps, tiles_of(1), twg, lds, rtab, tid, f);
+        __syncthreads();
+    }
+    if (a.sc[2].tps) {
+        // sixteen 32 tiles take 256 of the 512 threads
+        lane_tiles<32, A4, 256, BLK_NTH>(src, a.sc[2].rd, a.sc[2].tps, tiles_of(2), lds, rtab, tid, f);
+        __syncthreads();
+    }
+    if (a.sc[3].tps) {
+        lane_tiles<16, A4, BLK_NTH, BLK_NTH>(src, a.sc[3].rd, a.sc[3].tps, tiles_of(3), lds, rtab, tid, f);
+        __syncthreads();
+    }
+    if (a.sc[4].tps) {
+        // 256 tiles: two rounds of 128 (8 rows of 16 tiles each)
+        const TileSet ts = tiles_of(4);
+        lane_tiles<8, A4, BLK_NTH, BLK_NTH>(src, a.sc[4].rd, a.sc[4].tps, TileSet{ts.first, 4, 128}, lds, rtab, tid,
+                                            f);
+        __syncthreads();
+        lane_tiles<8, A4, BLK_NTH, BLK_NTH>(src, a.sc[4].rd, a.sc[4].tps, TileSet{ts.first + 8 * a.sc[4].tps, 4, 128},
+                                            lds, rtab, tid, f);
+    }
+}
+
+// The block's pixels staged once in LDS, every scale cut from there.
 template <int PIX, int LD>
 __global__ __launch_bounds__(BLK_NTH) void tile_fft_blocks(BlockArgs a, const float2 *__restrict__ twg) {
     __shared__ __align__(16) uint16_t blk[BLK * BLK_PITCH];
@@ -1226,36 +1262,7 @@ __global__ __launch_bounds__(BLK_NTH) void tile_fft_blocks(BlockArgs a, const fl
     const int X0 = bx * BLK, Y0 = by * BLK;
     stage_block<PIX, LD>(a.ps.frames + (long long)f * a.ps.frame_bytes, a.ps, X0, Y0, blk, tid);
     __syncthreads();
-    const BlockPix src{blk, X0, Y0};
-    // each scale's tiles of the block in turn, largest first; slot k of
-    // a.sc holds tile size 128 >> k (tps = 0: scale absent)
-    auto tiles_of = [&](int k) { return TileSet{by * (1 << k) * a.sc[k].tps + bx * (1 << k), k, 1 << (2 * k)}; };
-    if (a.sc[0].tps) {
-        reg_tiles<128>(src, a.sc[0].rd, a.sc[0].tps, tiles_of(0), twg, lds, rtab, tid, f);
-        __syncthreads();  // the next scale reuses lds and rtab
-    }
-    if (a.sc[1].tps) {
-        reg_tiles<64>(src, a.sc[1].rd, a.sc[1].tps, tiles_of(1), twg, lds, rtab, tid, f);
-        __syncthreads();
-    }
-    if (a.sc[2].tps) {
-        // sixteen 32 tiles take 256 of the 512 threads
-        lane_tiles<32, true, 256, BLK_NTH>(src, a.sc[2].rd, a.sc[2].tps, tiles_of(2), lds, rtab, tid, f);
-        __syncthreads();
-    }
-    if (a.sc[3].tps) {
-        lane_tiles<16, true, BLK_NTH, BLK_NTH>(src, a.sc[3].rd, a.sc[3].tps, tiles_of(3), lds, rtab, tid, f);
-        __syncthreads();
-    }
-    if (a.sc[4].tps) {
-        // 256 tiles: two rounds of 128 (8 rows of 16 tiles each)
-        const TileSet ts = tiles_of(4);
-        lane_tiles<8, true, BLK_NTH, BLK_NTH>(src, a.sc[4].rd, a.sc[4].tps, TileSet{ts.first, 4, 128}, lds, rtab, tid,
-                                              f);
-        __syncthreads();
-        lane_tiles<8, true, BLK_NTH, BLK_NTH>(src, a.sc[4].rd, a.sc[4].tps, TileSet{ts.first + 8 * a.sc[4].tps, 4, 128},
-                                              lds, rtab, tid, f);
-    }
+    block_scales<true>(BlockPix{blk, X0, Y0}, a, twg, lds, rtab, tid, f, bx, by);
 }
 
 constexpr int BAND_NTH = 256;
