@@ -1,6 +1,8 @@
 // Mixed-precision tile transform for 64 x 64 and 128 x 128 tiles: the 2-D
-// R2C DFT as dense matrix products on the matrix cores (v_mfma_f32_16x16x16
-// bf16 / f16), every operand split into a high and a low 16-bit part
+// R2C DFT as dense matrix products on the matrix cores (gfx950's
+// v_mfma_f32_16x16x32 bf16 / f16: eight 16-bit values of A and of B per lane,
+// A[i][8 (lane / 16) ..] with i = lane % 16, B[8 (lane / 16) ..][lane % 16],
+// D[4 (lane / 16) + r][lane % 16] out), every operand split into a high and a low 16-bit part
 // (a = a_h + a_l) and each product formed as a_h b_h + a_h b_l + a_l b_h with
 // FP32 accumulation.  It replaces the FFT of those scales when an engine is
 // switched to DDM_PREC_BF16X2 or DDM_PREC_FP16X2; the output layout is the
@@ -32,8 +34,8 @@ namespace ddm {
 
 namespace {
 
-typedef short s16x4 __attribute__((ext_vector_type(4)));
-typedef _Float16 h16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 b16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int MF_NTH = 256;  // 4 waves
@@ -68,30 +70,59 @@ struct F16 {
     }
 };
 
+// v = hi + lo with hi, lo 16-bit values (round to nearest even; gfx950
+// converts two values per instruction), two values per 32-bit word.
 template <bool BF>
-__device__ __forceinline__ f32x4 mfma(uint2 a, uint2 b, f32x4 c) {
+__device__ __forceinline__ void split2(float a, float b, uint32_t *hi, uint32_t *lo) {
     if constexpr (BF) {
-        s16x4 av, bv;
-        memcpy(&av, &a, 8);
-        memcpy(&bv, &b, 8);
-        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, c, 0, 0, 0);
+        const __bf16 ha = (__bf16)a, hb = (__bf16)b;
+        const __bf16 la = (__bf16)(a - (float)ha), lb = (__bf16)(b - (float)hb);
+        uint16_t u[4];
+        memcpy(&u[0], &ha, 2);
+        memcpy(&u[1], &hb, 2);
+        memcpy(&u[2], &la, 2);
+        memcpy(&u[3], &lb, 2);
+        *hi = u[0] | ((uint32_t)u[1] << 16);
+        *lo = u[2] | ((uint32_t)u[3] << 16);
     } else {
-        h16x4 av, bv;
-        memcpy(&av, &a, 8);
-        memcpy(&bv, &b, 8);
-        return __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, c, 0, 0, 0);
+        const _Float16 ha = (_Float16)a, hb = (_Float16)b;
+        const _Float16 la = (_Float16)(a - (float)ha), lb = (_Float16)(b - (float)hb);
+        uint16_t u[4];
+        memcpy(&u[0], &ha, 2);
+        memcpy(&u[1], &hb, 2);
+        memcpy(&u[2], &la, 2);
+        memcpy(&u[3], &lb, 2);
+        *hi = u[0] | ((uint32_t)u[1] << 16);
+        *lo = u[2] | ((uint32_t)u[3] << 16);
+    }
+}
+
+template <bool BF>
+__device__ __forceinline__ f32x4 mfma(uint4 a, uint4 b, f32x4 c) {
+    if constexpr (BF) {
+        b16x8 av, bv;
+        memcpy(&av, &a, 16);
+        memcpy(&bv, &b, 16);
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
+    } else {
+        h16x8 av, bv;
+        memcpy(&av, &a, 16);
+        memcpy(&bv, &b, 16);
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, c, 0, 0, 0);
     }
 }
 
 // a b with both split: a_h b_h + a_h b_l + a_l b_h
 template <bool BF>
-__device__ __forceinline__ f32x4 mfma3(uint2 ah, uint2 al, uint2 bh, uint2 bl, f32x4 c) {
+__device__ __forceinline__ f32x4 mfma3(uint4 ah, uint4 al, uint4 bh, uint4 bl, f32x4 c) {
     c = mfma<BF>(ah, bh, c);
     c = mfma<BF>(ah, bl, c);
     return mfma<BF>(al, bh, c);
 }
 
-__device__ __forceinline__ uint2 neg4(uint2 v) { return make_uint2(v.x ^ 0x80008000u, v.y ^ 0x80008000u); }
+__device__ __forceinline__ uint4 neg8(uint4 v) {
+    return make_uint4(v.x ^ 0x80008000u, v.y ^ 0x80008000u, v.z ^ 0x80008000u, v.w ^ 0x80008000u);
+}
 
 template <int S>
 struct MfCfg {
@@ -99,21 +130,37 @@ struct MfCfg {
     static constexpr int TPW = S == 128 ? 1 : 4;  // tiles per workgroup
     static constexpr int RT = S == 128 ? 2 : 4;   // 16-row tiles per wave
     static constexpr int CT = S / 16;             // 16-column tiles per wave
-    static constexpr int KS = S / 16;             // k steps
+    static constexpr int KS = S / 32;             // k steps
     static constexpr int TILE = S * S;            // 16-bit values per matrix per tile
 };
+
+// LDS rows of X and Y^T: the 16-byte chunk c of row y is stored at chunk
+// c ^ key(y), so that the 16 rows one operand read of a wave touches (lanes
+// i = 0 .. 15, each reading eight values of its row) land in distinct banks
+// (128 tiles: every row starts at bank 0; 64 tiles: every other row does).
+template <int S>
+__device__ __forceinline__ int chunk_key(int y) {
+    return S == 128 ? (y & 15) : ((y >> 1) & 7);
+}
+template <int S>
+__device__ __forceinline__ int swz(int y, int x) {  // x: a multiple of 4 below S
+    return ((((x >> 3) ^ chunk_key<S>(y)) << 3) | (x & 7));
+}
+
+// Pixel loads of the transform: 8 consecutive pixels from 8- (u8) or
+// 16-byte (u16) aligned rows, or one by one.
+enum { MPIX_ANY = 0, MPIX_U8 = 1, MPIX_U16 = 2 };
 
 }  // namespace
 
 // mats: six S x S matrices of 16-bit values, each row-major:
 // W1T_h, W1T_l ([c][x]), C2_h, C2_l, S2_h, S2_l ([ky][y]).
-template <int S, bool BF>
+template <int S, bool BF, int PIX>
 __global__ __launch_bounds__(MF_NTH) void tile_dft_mfma(PixSrc ps, RingDst rd, int tps, int tiles, int groups,
                                                        const uint16_t *__restrict__ mats) {
     using C = MfCfg<S>;
-    using Fm = F16<BF>;
     constexpr int H = C::H, TPW = C::TPW, RT = C::RT, CT = C::CT, KS = C::KS, TILE = C::TILE;
-    __shared__ uint16_t xs[2][TPW * TILE];  // hi / lo: X[y][x], later Y^T[c][y]
+    __shared__ __align__(16) uint16_t xs[2][TPW * TILE];  // hi / lo: X[y][x], later Y^T[c][y]
     __shared__ float2 zc[TPW][S];           // packed column 0 of the result
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int nf = gridDim.x / groups;  // frames vary fastest
@@ -121,23 +168,52 @@ __global__ __launch_bounds__(MF_NTH) void tile_dft_mfma(PixSrc ps, RingDst rd, i
     const int t = S == 128 ? 0 : w;  // this wave's tile within the workgroup
     const int tile = (blockIdx.x / nf) * TPW + t;
     const int r0 = S == 128 ? 32 * w : 0;  // first output row of this wave
-    const int li = lane & 15, lk = 4 * (lane >> 4);
+    // operand k of this lane (8 values), output rows (4 values)
+    const int li = lane & 15, lk = 8 * (lane >> 4), lo = 4 * (lane >> 4);
     const uint16_t *W1h = mats, *W1l = mats + TILE, *C2h = mats + 2 * TILE, *C2l = mats + 3 * TILE,
                    *S2h = mats + 4 * TILE, *S2l = mats + 5 * TILE;
     uint16_t *xh = xs[0], *xl = xs[1];
 
-    // ---- pixels -> X (scaled by 2^-8, split) ----
+    // ---- pixels -> X (scaled by 2^-8, split), 8 pixels per item ----
     {
         const uint8_t *fr = ps.frames + (long long)f * ps.frame_bytes;
-        for (int i = tid; i < TPW * TILE; i += MF_NTH) {
-            const int tt = i / TILE, y = (i / S) % S, x = i % S;
+        for (int i = tid; i < TPW * TILE / 8; i += MF_NTH) {
+            const int tt = i / (TILE / 8), y = (i / (S / 8)) % S, x = 8 * (i % (S / 8));
             const int tl = (blockIdx.x / nf) * TPW + tt;
-            float v = 0.f;
-            if (tl < tiles) v = load_px(fr, ps, ps.x_off + (tl % tps) * S + x, ps.y_off + (tl / tps) * S + y);
-            v *= 1.0f / 256.0f;
-            const uint16_t h = Fm::from(v);
-            xh[i] = h;
-            xl[i] = Fm::from(v - Fm::to(h));
+            float v[8];
+            if (tl < tiles) {
+                const int gx = ps.x_off + (tl % tps) * S + x, gy = ps.y_off + (tl / tps) * S + y;
+                const uint8_t *p = fr + (long long)gy * ps.row_bytes + (long long)gx * ps.bpp;
+                if constexpr (PIX == MPIX_U8) {
+                    const uint2 u = *reinterpret_cast<const uint2 *>(p);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        v[k] = (float)((u.x >> (8 * k)) & 0xffu);
+                        v[4 + k] = (float)((u.y >> (8 * k)) & 0xffu);
+                    }
+                } else if constexpr (PIX == MPIX_U16) {
+                    const uint4 u = *reinterpret_cast<const uint4 *>(p);
+                    const uint32_t wv[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        v[2 * k] = (float)(wv[k] & 0xffffu);
+                        v[2 * k + 1] = (float)(wv[k] >> 16);
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) v[k] = load_px(fr, ps, gx + k, gy);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) v[k] = 0.f;
+            }
+            uint32_t hv[4], lv[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                split2<BF>(v[2 * k] * (1.0f / 256.0f), v[2 * k + 1] * (1.0f / 256.0f), &hv[k], &lv[k]);
+            const int o = tt * TILE + y * S + swz<S>(y, x);
+            *reinterpret_cast<uint4 *>(xh + o) = make_uint4(hv[0], hv[1], hv[2], hv[3]);
+            *reinterpret_cast<uint4 *>(xl + o) = make_uint4(lv[0], lv[1], lv[2], lv[3]);
         }
     }
     __syncthreads();
@@ -151,19 +227,19 @@ __global__ __launch_bounds__(MF_NTH) void tile_dft_mfma(PixSrc ps, RingDst rd, i
     {
         const uint16_t *xt_h = xh + t * TILE, *xt_l = xl + t * TILE;
         for (int ks = 0; ks < KS; ++ks) {
-            const int k0 = 16 * ks + lk;
-            uint2 ah[RT], al[RT];
+            const int k0 = 32 * ks + lk;
+            uint4 ah[RT], al[RT];
 #pragma unroll
             for (int a = 0; a < RT; ++a) {
                 const int row = r0 + 16 * a + li;
-                ah[a] = *reinterpret_cast<const uint2 *>(xt_h + row * S + k0);
-                al[a] = *reinterpret_cast<const uint2 *>(xt_l + row * S + k0);
+                ah[a] = *reinterpret_cast<const uint4 *>(xt_h + row * S + swz<S>(row, k0));
+                al[a] = *reinterpret_cast<const uint4 *>(xt_l + row * S + swz<S>(row, k0));
             }
 #pragma unroll
             for (int b = 0; b < CT; ++b) {
                 const int col = 16 * b + li;
-                const uint2 bh = *reinterpret_cast<const uint2 *>(W1h + col * S + k0);
-                const uint2 bl = *reinterpret_cast<const uint2 *>(W1l + col * S + k0);
+                const uint4 bh = *reinterpret_cast<const uint4 *>(W1h + col * S + k0);
+                const uint4 bl = *reinterpret_cast<const uint4 *>(W1l + col * S + k0);
 #pragma unroll
                 for (int a = 0; a < RT; ++a) acc[a][b] = mfma3<BF>(ah[a], al[a], bh, bl, acc[a][b]);
             }
@@ -178,21 +254,12 @@ __global__ __launch_bounds__(MF_NTH) void tile_dft_mfma(PixSrc ps, RingDst rd, i
 #pragma unroll
             for (int b = 0; b < CT; ++b) {
                 // lane holds Y[row .. row + 3][col]
-                const int row = r0 + 16 * a + lk, col = 16 * b + li;
-                uint16_t hv[4], lv[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const float v = acc[a][b][r] * (1.0f / 256.0f);
-                    hv[r] = Fm::from(v);
-                    lv[r] = Fm::from(v - Fm::to(hv[r]));
-                }
+                const int row = r0 + 16 * a + lo, col = 16 * b + li;
                 uint2 ph, pl;
-                ph.x = hv[0] | ((uint32_t)hv[1] << 16);
-                ph.y = hv[2] | ((uint32_t)hv[3] << 16);
-                pl.x = lv[0] | ((uint32_t)lv[1] << 16);
-                pl.y = lv[2] | ((uint32_t)lv[3] << 16);
-                *reinterpret_cast<uint2 *>(yt_h + col * S + row) = ph;
-                *reinterpret_cast<uint2 *>(yt_l + col * S + row) = pl;
+                split2<BF>(acc[a][b][0] * (1.0f / 256.0f), acc[a][b][1] * (1.0f / 256.0f), &ph.x, &pl.x);
+                split2<BF>(acc[a][b][2] * (1.0f / 256.0f), acc[a][b][3] * (1.0f / 256.0f), &ph.y, &pl.y);
+                *reinterpret_cast<uint2 *>(yt_h + col * S + swz<S>(col, row)) = ph;
+                *reinterpret_cast<uint2 *>(yt_l + col * S + swz<S>(col, row)) = pl;
             }
     }
     __syncthreads();
@@ -204,28 +271,28 @@ __global__ __launch_bounds__(MF_NTH) void tile_dft_mfma(PixSrc ps, RingDst rd, i
     {
         const uint16_t *yt_h = xh + t * TILE, *yt_l = xl + t * TILE;
         for (int ks = 0; ks < KS; ++ks) {
-            const int k0 = 16 * ks + lk;
-            uint2 ch[RT], cl[RT], sh[RT], sl[RT];
+            const int k0 = 32 * ks + lk;
+            uint4 ch[RT], cl[RT], sh[RT], sl[RT];
 #pragma unroll
             for (int a = 0; a < RT; ++a) {
                 const int row = r0 + 16 * a + li;
-                ch[a] = *reinterpret_cast<const uint2 *>(C2h + row * S + k0);
-                cl[a] = *reinterpret_cast<const uint2 *>(C2l + row * S + k0);
-                sh[a] = *reinterpret_cast<const uint2 *>(S2h + row * S + k0);
-                sl[a] = *reinterpret_cast<const uint2 *>(S2l + row * S + k0);
+                ch[a] = *reinterpret_cast<const uint4 *>(C2h + row * S + k0);
+                cl[a] = *reinterpret_cast<const uint4 *>(C2l + row * S + k0);
+                sh[a] = *reinterpret_cast<const uint4 *>(S2h + row * S + k0);
+                sl[a] = *reinterpret_cast<const uint4 *>(S2l + row * S + k0);
             }
 #pragma unroll
             for (int b = 0; b < CT; ++b) {
                 const int col = 16 * b + li;
-                const uint2 yh = *reinterpret_cast<const uint2 *>(yt_h + col * S + k0);
-                const uint2 yl = *reinterpret_cast<const uint2 *>(yt_l + col * S + k0);
+                const uint4 yh = *reinterpret_cast<const uint4 *>(yt_h + col * S + swz<S>(col, k0));
+                const uint4 yl = *reinterpret_cast<const uint4 *>(yt_l + col * S + swz<S>(col, k0));
                 // Y'[:, col]: column col + H, or minus column col - H
                 const int pc = col < H ? col + H : col - H;
-                uint2 qh = *reinterpret_cast<const uint2 *>(yt_h + pc * S + k0);
-                uint2 ql = *reinterpret_cast<const uint2 *>(yt_l + pc * S + k0);
+                uint4 qh = *reinterpret_cast<const uint4 *>(yt_h + pc * S + swz<S>(pc, k0));
+                uint4 ql = *reinterpret_cast<const uint4 *>(yt_l + pc * S + swz<S>(pc, k0));
                 if (col >= H) {
-                    qh = neg4(qh);
-                    ql = neg4(ql);
+                    qh = neg8(qh);
+                    ql = neg8(ql);
                 }
 #pragma unroll
                 for (int a = 0; a < RT; ++a) {
@@ -244,7 +311,7 @@ __global__ __launch_bounds__(MF_NTH) void tile_dft_mfma(PixSrc ps, RingDst rd, i
         for (int a = 0; a < RT; ++a)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                zc[t][r0 + 16 * a + lk + r] = make_float2(acc[a][0][r] * BACK, acc[a][CT / 2][r] * BACK);
+                zc[t][r0 + 16 * a + lo + r] = make_float2(acc[a][0][r] * BACK, acc[a][CT / 2][r] * BACK);
     }
     __syncthreads();
     if (!valid) return;
@@ -255,7 +322,7 @@ __global__ __launch_bounds__(MF_NTH) void tile_dft_mfma(PixSrc ps, RingDst rd, i
         for (int b = 0; b < CT / 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int ky = r0 + 16 * a + lk + r, k = 16 * b + li;
+                const int ky = r0 + 16 * a + lo + r, k = 16 * b + li;
                 if (k != 0) out.put(ky, k, make_float2(acc[a][b][r] * BACK, acc[a][b + CT / 2][r] * BACK));
             }
     // separate the packed column: DC (kx = 0) and Nyquist (kx = H)
@@ -338,8 +405,15 @@ static hipError_t launch_mf(const PixSrc &ps, const RingDst &rd, int tps, int ti
     if (e != hipSuccess) return e;
     constexpr int TPW = MfCfg<S>::TPW;
     const int groups = (tiles + TPW - 1) / TPW;
-    hipLaunchKernelGGL((tile_dft_mfma<S, BF>), dim3((unsigned)(groups * n_frames)), dim3(MF_NTH), 0, st, ps, rd,
-                       tps, tiles, groups, mats);
+    const dim3 grid((unsigned)(groups * n_frames)), block(MF_NTH);
+    // 8-pixel loads need every row piece aligned to 8 pixels' bytes
+    const long long bits = ps.row_bytes | ps.frame_bytes | (long long)(uintptr_t)ps.frames | (long long)ps.x_off * ps.bpp;
+    if (ps.fmt == DDM_PIX_U8 && ps.bpp == 1 && (bits & 7) == 0)
+        hipLaunchKernelGGL((tile_dft_mfma<S, BF, MPIX_U8>), grid, block, 0, st, ps, rd, tps, tiles, groups, mats);
+    else if (ps.fmt == DDM_PIX_U16LE && ps.bpp == 2 && (bits & 15) == 0)
+        hipLaunchKernelGGL((tile_dft_mfma<S, BF, MPIX_U16>), grid, block, 0, st, ps, rd, tps, tiles, groups, mats);
+    else
+        hipLaunchKernelGGL((tile_dft_mfma<S, BF, MPIX_ANY>), grid, block, 0, st, ps, rd, tps, tiles, groups, mats);
     return hipGetLastError();
 }
 
