@@ -64,7 +64,7 @@ struct ddm_engine {
     int *d_lay = nullptr;              // tile layout tables of every scale
     std::vector<ddm::TileLayout> lay;  // per scale, pointing into d_lay
     std::vector<int> family;           // per scale, enum ddm_fft_family
-    int tiler = DDM_TILER_PER_SCALE;   // enum ddm_tiler
+    int tiler = DDM_TILER_PER_SCALE;   // enum ddm_tiler (default set by region size at creation)
     double *d_norm = nullptr, *d_out = nullptr;
     double *d_tau = nullptr, *d_fit = nullptr;  // allocated by the first ddm_fit_results
     float *d_xt = nullptr;    // row spectra of 256 / 512 tiles between FFT passes
@@ -294,6 +294,10 @@ extern "C" int ddm_create(const ddm_desc *desc, ddm_engine **out) {
         HTRY(up);
     }
     e->family.assign(p.geom.size(), DDM_FFT_REGISTER);
+    // small regions: a chunk's transforms are short launches, and fusing them
+    // pays (256^2 serving streams +7 %); larger regions are bandwidth-bound
+    // and keep one launch per scale (512^2 -5 to -13 %, 1024^2 -4 to -11 %)
+    e->tiler = p.M <= ddm::kTilerFusedMaxRegion ? DDM_TILER_FUSED : DDM_TILER_PER_SCALE;
     const size_t roi_frame = (size_t)p.M * p.M * p.bpp;
     // pinned host staging is allocated on the first host submission that needs it
     for (int i = 0; i < e->nstage; ++i) TRY(dev_alloc(e, &e->d_stage[i], roi_frame * p.chunk, "upload staging"));

@@ -96,6 +96,9 @@ inline int default_fft_batch(int chunk) { return chunk * (128 / chunk > 1 ? 128 
 // slots * RING_GROUP * 8 bytes must stay below 2^31.
 constexpr long long kMaxRingSlots = (1LL << 31) / (RING_GROUP * 8);
 
+// Largest region (M) whose engine uses the multi-scale tiler by default.
+constexpr int kTilerFusedMaxRegion = 256;
+
 // Frames per lag-engine pass when the descriptor leaves it to the engine.
 constexpr int kDefaultLagBlock = 256;
 // Lag values above this are rejected (ring memory would be absurd).

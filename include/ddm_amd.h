@@ -110,9 +110,11 @@ enum ddm_fft_family { DDM_FFT_REGISTER = 0, DDM_FFT_LDS = 1 };
  * LDS, every scale's tiles cut from there) and once for the row passes of
  * all 256 .. 1024 scales (row bands), instead of once per scale; the FFT
  * pass buffer then holds one region per 256 .. 1024 scale.
- * DDM_TILER_PER_SCALE (default): one transform launch per scale, each
- * reading the frame, as the reference does (src/DDM.cu:171-192).  The
- * spectra are the same bit for bit. */
+ * DDM_TILER_PER_SCALE: one transform launch per scale, each reading the
+ * frame, as the reference does (src/DDM.cu:171-192).  The spectra are the
+ * same bit for bit.  Default: fused for regions up to 256 x 256 (short,
+ * launch-bound transforms), per scale above (bandwidth-bound); ddm_autotune
+ * times both. */
 enum ddm_tiler { DDM_TILER_FUSED = 0, DDM_TILER_PER_SCALE = 1 };
 
 typedef struct ddm_desc {

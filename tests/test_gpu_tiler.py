@@ -162,3 +162,14 @@ def test_tiler_mode_fixed_after_first_frame(gpu):
         e.submit_host(frames[:4].reshape(-1).view(np.uint8), 4)
         with pytest.raises(ddm.DDMError):
             e.set_tiler(ddm.TILER_PER_SCALE)
+
+
+def test_tiler_default_by_region(gpu):
+    """Regions up to 256 x 256 default to the fused tiler (their per-scale
+    transforms are short launches), larger ones to per-scale launches."""
+    for size, want in ((128, ddm.TILER_FUSED), (256, ddm.TILER_FUSED), (512, ddm.TILER_PER_SCALE),
+                       (1024, ddm.TILER_PER_SCALE)):
+        scales = [s for s in (1024, 512, 256, 128, 64, 32) if s <= size]
+        with ddm.Engine(size, size, scales, LAGS, LAMBDAS, 1.2, 16) as e:
+            assert e.tiler() == want, size
+
