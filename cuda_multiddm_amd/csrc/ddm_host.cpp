@@ -249,8 +249,9 @@ size_t device_footprint(const Params &p) {
     // chunk: counted at that size, so an admitted stream never outgrows it
     const int batch = default_fft_batch(p.chunk);
     bytes += UPLOAD_STAGES * (size_t)(batch - p.chunk) * p.M * p.M * p.bpp;
-    // (one region per 256 .. 1024 scale: the multi-scale tiler's size)
-    return bytes + (size_t)fft_pass_elems(p.M, p.scales, batch, true) * 8;
+    // (one region per 256 .. 1024 scale when the engine's default tiler mode,
+    // set by the region size, is the multi-scale tiler)
+    return bytes + (size_t)fft_pass_elems(p.M, p.scales, batch, p.M <= kTilerFusedMaxRegion) * 8;
 }
 
 // Member lists of one (scale, lambda, tolerance) ring, cached for the life of

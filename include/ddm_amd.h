@@ -260,7 +260,9 @@ int ddm_get_fft_families(const ddm_engine *eng, int *families);
 
 /* The multi-scale tiler mode (enum ddm_tiler), before the first frame.
  * Replaces the per-scale cut of parseBufferScalePow2 in the chunk loop
- * (src/DDM.cu:171-192, src/DDM_kernel.cuh:43-78). */
+ * (src/DDM.cu:171-192, src/DDM_kernel.cuh:43-78).  The device footprint
+ * (ddm_validate, server admission) counts the default mode's pass buffer;
+ * fusing two or more 256 .. 1024 scales allocates one region per scale. */
 int ddm_set_tiler(ddm_engine *eng, int mode);
 
 /* The engine's tiler mode (enum ddm_tiler), or DDM_EINVAL. */
