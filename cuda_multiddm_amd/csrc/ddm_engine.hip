@@ -514,6 +514,26 @@ static int batch_room(const ddm_engine *e) {
     return (int)std::max<long long>(1, (long long)p.ring_slots - p.max_lag - waiting);
 }
 
+// Largest batch of device-resident frames: an FFT batch, or up to a lag
+// block when that is at most twice the FFT batch (see device_batch).
+static int device_batch_max(const ddm_engine *e) {
+    return std::max(e->fft_batch, std::min(e->lag_block, 2 * e->fft_batch));
+}
+
+// Frames of the next batch of device-resident input: up to the next lag-block
+// boundary when that is between half and twice an FFT batch (each batch then
+// completes one lag pass, and no short batches are left between passes),
+// otherwise an FFT batch; never more than the ring room or the frames left.
+// Lag blocks fall on the same frames either way.
+static int device_batch(const ddm_engine *e, int left) {
+    const long long waiting = e->frames - e->frames_lagged;
+    const int to_block = e->lag_block - (int)(waiting % e->lag_block);
+    const int room = batch_room(e);
+    int n = e->fft_batch;
+    if (2 * to_block >= e->fft_batch && to_block <= 2 * e->fft_batch && to_block <= room) n = to_block;
+    return std::min({n, left, room});
+}
+
 extern "C" int ddm_submit_device(ddm_engine *e, const void *frames, int n_frames) {
     if (!e) return DDM_EINVAL;
     if (n_frames < 0 || (n_frames > 0 && !frames)) return fail(e, DDM_EINVAL, "bad frame buffer");
@@ -521,7 +541,7 @@ extern "C" int ddm_submit_device(ddm_engine *e, const void *frames, int n_frames
     const long long fb = (long long)p.width * p.height * p.bpp;
     const uint8_t *src = (const uint8_t *)frames;
     for (int i = 0; i < n_frames;) {
-        const int n = std::min({e->fft_batch, n_frames - i, batch_room(e)});
+        const int n = device_batch(e, n_frames - i);
         int rc = process_chunk(e, src + (long long)i * fb, n, fb, p.width, p.x_off, p.y_off, nullptr);
         if (rc) return rc;
         i += n;
@@ -559,7 +579,7 @@ extern "C" int ddm_reserve(ddm_engine *e, int max_frames) {
     if (!e) return DDM_EINVAL;
     if (max_frames < 0) return fail(e, DDM_EINVAL, "bad frame count");
     const int chunk = e->p.chunk;
-    const int batch = std::min(e->fft_batch, std::max(max_frames, 1));
+    const int batch = std::min(device_batch_max(e), std::max(max_frames, 1));
     int rc = ensure_xt(e, batch, e->tiler == DDM_TILER_FUSED);
     if (rc) return rc;
     if (max_frames > e->stage_frames && e->stage_frames < e->fft_batch) {
