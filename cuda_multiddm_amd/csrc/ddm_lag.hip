@@ -574,13 +574,15 @@ __device__ __forceinline__ void lag_body(const float2 *__restrict__ ring, float 
         const int len = (n + sd - 1) / sd;
         const int j0 = seg * len, j1 = min(j0 + len, n);
         if (j0 < j1) {
-            if (j0 - (tau_a + kc - 1) >= -dmax) {
-                dense_sums_live<DENSE>(kc, g, nl, R, slot0, tau_a, j0, j1, s);
-            } else {
-                // near the start of the stream some partners do not exist yet
+            // from jv on, every lag of the chunk has all its partners; near the
+            // start of the stream the frames before jv pair only where a
+            // partner exists, lag by lag
+            const int jv = min(j1, max(j0, tau_a + kc - 1 - dmax));
+            if (jv < j1) dense_sums_live<DENSE>(kc, g, nl, R, slot0, tau_a, jv, j1, s);
+            if (j0 < jv) {
 #pragma unroll
                 for (int i = 0; i < DENSE; ++i)
-                    if (i < kc) s[i] = lag_range_sum<U>(g, nl, R, slot0, dmax, tau_a + i, j0, j1);
+                    if (i < kc) s[i] += lag_range_sum<U>(g, nl, R, slot0, dmax, tau_a + i, j0, jv);
             }
         }
     } else if (hist) {
