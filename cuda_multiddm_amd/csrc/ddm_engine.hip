@@ -1084,7 +1084,10 @@ extern "C" int ddm_autotune(ddm_engine *e, const char *cache_path, int *lag_bloc
             a.acc = e->d_acc;
             a.elems_per_frame = p.elems_padded;
             a.ring_slots = p.ring_slots;
-            a.slot0 = 0;
+            // the block starts max_lag slots into the ring, so its history is
+            // one run of slots, as for most blocks of a stream (a block at slot
+            // 0 reads every partner across the ring's wrap)
+            a.slot0 = p.max_lag;
             a.n_new = cand;
             a.dmax = p.max_lag;
             a.lags = e->d_lags;

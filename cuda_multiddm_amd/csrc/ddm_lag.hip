@@ -33,7 +33,7 @@ constexpr int DENSE_SEG = 32;                   // frames per segment of a lag's
 // a valid 32-bit data format, so dword loads return memory.
 constexpr int BUFFER_RSRC_WORD3 = 0x00020000;
 constexpr int DENSE_BATCH = 4;                  // register-window steps scheduled together
-constexpr int ACC_PREFETCH = 4;                 // accumulators each wave reads early
+constexpr int ACC_PREFETCH = 4;                 // accumulators each wave requests ahead of its sums
 
 // Per block size: J = 256 stages 128 KB of new frames, so one workgroup (4
 // waves per SIMD) runs per CU and each wave has room for a wider window,
@@ -53,8 +53,8 @@ struct LagCfg {
 };
 
 // Workgroup barrier ordering LDS only.  Waves never exchange data through
-// global memory here, so loads still in flight (the early accumulator reads)
-// need not complete before it, as they would under __syncthreads().
+// global memory here, so loads still in flight (the accumulator reads issued
+// before the partial sums are exchanged) need not complete before it, as they would under __syncthreads().
 __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_s_barrier();
@@ -377,16 +377,23 @@ __device__ __forceinline__ void hist_sums(const GroupRing &g, const float2 *nl, 
 
 // A lane's accumulators: acc[l][e] and, for compensated accumulation,
 // comp[l][e], the rounding error of every addition into acc so far
-// (acc + comp is the sum carried to twice the precision of FP32).
+// (acc + comp is the sum carried to twice the precision of FP32).  The
+// pointers are the ring group's (wave-uniform, so they stay in scalar
+// registers and each access is a scalar base plus the lane's 32-bit offset).
 template <bool COMP>
 struct AccRef {
     float *a;
     float *c;  // unused unless COMP
+    int lane;
+    __device__ __forceinline__ float *at(int l, long long E) const { return a + l * E + lane; }
+    __device__ __forceinline__ float *cat(int l, long long E) const { return c + l * E + lane; }
 };
 
 // The accumulators of the first PF lags wave w adds (lags w, w + 16, ...),
-// requested when the workgroup starts so that they have arrived by the time
-// the partial sums are added.
+// all requested when the wave's pair work is done, so that they arrive while
+// the partial sums are exchanged in LDS (requested when the workgroup starts,
+// they held registers through the pair work and the compiler spilled them,
+// waiting for each load first).
 template <int PF, bool COMP>
 struct AccPre {
     float a[PF];
@@ -398,7 +405,7 @@ __device__ __forceinline__ void load_acc(AccRef<COMP> ar, long long E, int w, in
 #pragma unroll
     for (int m = 0; m < PF; ++m) {
         const int l = w + m * LAG_WAVES;
-        pre.a[m] = l < nred ? ar.a[(long long)l * E] : 0.f;
+        pre.a[m] = l < nred ? *ar.at(l, E) : 0.f;
     }
 }
 
@@ -410,13 +417,13 @@ __device__ __forceinline__ void load_comp(AccRef<COMP> ar, long long E, int w, i
 #pragma unroll
         for (int m = 0; m < PF; ++m) {
             const int l = w + m * LAG_WAVES;
-            pre.c[m] = l < nred ? ar.c[(long long)l * E] : 0.f;
+            pre.c[m] = l < nred ? *ar.cat(l, E) : 0.f;
         }
     }
 }
 
-// acc[l] += t for the m-th lag this wave adds (l = w + 16 m), from the early
-// read when there is one (m < 0: none).  Compensated: the sum is rounded to
+// acc[l] += t for the m-th lag this wave adds (l = w + 16 m), from the read
+// issued ahead when there is one (m < 0: none).  Compensated: the sum is rounded to
 // FP32 and its exact rounding error (TwoSum, no multiplications, so no
 // contraction can change it) is added to comp.
 template <int PF, bool COMP>
@@ -431,10 +438,10 @@ __device__ __forceinline__ void add_acc(AccRef<COMP> ar, long long E, int l, int
             if constexpr (COMP) c = pre.c[i];
             have = true;
         }
-    float *p = ar.a + (long long)l * E;
+    float *p = ar.at(l, E);
     if (!have) a = *p;
     if constexpr (COMP) {
-        float *q = ar.c + (long long)l * E;
+        float *q = ar.cat(l, E);
         if (!have) c = *q;
         const float s = a + t;
         const float bp = s - a;
@@ -456,6 +463,7 @@ __device__ __forceinline__ void hist_reduce(float2 *nb, AccRef<COMP> accg, long 
                                             const float (&h)[LSM], int w, int wd, int hw0, int nhw, int sd,
                                             int dense, int ls, AccPre<PF, COMP> &pre) {
     static_assert(SL <= SM && LSM % RL == 0, "window sums exceed the partial registers");
+    load_acc(accg, E, w, dense + ls, pre);
     load_comp(accg, E, w, dense + ls, pre);
     float *red = reinterpret_cast<float *>(nb);
     const int lane = threadIdx.x % RING_GROUP;
@@ -477,7 +485,7 @@ __device__ __forceinline__ void hist_reduce(float2 *nb, AccRef<COMP> accg, long 
         lds_barrier();
         const int nd = r == 0 ? dense : 0, nh = min(RL, ls - r * RL);
         // wave w adds the round's lags l = w (mod 16): those are the lags
-        // whose accumulators it read early (w, w + 16, ...)
+        // whose accumulators it requested ahead (w, w + 16, ...)
         const int lo = r == 0 ? 0 : dense + r * RL, hi = dense + r * RL + nh;
         for (int l = lo + (w - lo % LAG_WAVES + LAG_WAVES) % LAG_WAVES; l < hi; l += LAG_WAVES) {
             float t = 0.f;
@@ -552,13 +560,13 @@ __device__ __forceinline__ void lag_body(const float2 *__restrict__ ring, float 
     const bool hist = C::HIST_LAGS && pl.hist;
     const int wd = pl.wd, sd = pl.sd, ss = pl.ss;
     const int ws = LAG_WAVES - wd;
-    // the accumulators this wave will add to are requested beside the staging
-    // loads, so they have arrived when the partial sums are added
-    const long long lane_off = (long long)group * RING_GROUP + lane;
-    AccRef<COMP> accg{acc + lane_off, COMP ? comp + lane_off : nullptr};
+    // the accumulators this wave adds to are all requested when its pair work
+    // is done, before the partial sums are exchanged (held through the pair
+    // work they would take registers its loops need)
+    const long long group_off = (long long)group * RING_GROUP;
+    AccRef<COMP> accg{acc + group_off, COMP ? comp + group_off : nullptr, lane};
     const int nred = hist ? L : (ss ? L : dense_lags);
     AccPre<ACC_PREFETCH, COMP> pre;
-    load_acc(accg, E, w, nred, pre);
     lds_barrier();
     const float2 *nl = nb + lane;
     float s[C::SLOTS_MAX];
@@ -622,6 +630,7 @@ __device__ __forceinline__ void lag_body(const float2 *__restrict__ ring, float 
     }
     if (wd == 0 && ss == 0) return;
     // ---- combine the partials: [wave][slot][lane] in the staging buffer ----
+    load_acc(accg, E, w, nred, pre);
     load_comp(accg, E, w, nred, pre);
     lds_barrier();
     float *red = reinterpret_cast<float *>(nb);
