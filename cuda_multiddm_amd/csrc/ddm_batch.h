@@ -99,6 +99,7 @@ struct LagPlan {
     int pmin, pmax;  // history-major frame range
     int listed;      // blk_start / blk_list hold each history wave's blocks
     int all_hist;    // the lists cover all LAG_WAVES waves (window waves take blocks after their window)
+    int pre;         // frames before the block the register window reads (staged in LDS up to the block size's limit)
     unsigned char blk_start[LAG_WAVES + 1];   // wave k: blk_list[blk_start[k] .. blk_start[k + 1])
     unsigned char blk_list[PLAN_MAX_BLOCKS];  // block b: frames pmin + b * HIST_BLOCK ...
 };

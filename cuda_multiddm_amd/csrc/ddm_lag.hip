@@ -50,6 +50,10 @@ struct LagCfg {
     // register sum each, over blocks of HIST_BLOCK history frames
     static constexpr int HIST_LAGS = WIDE ? 32 : 0;
     static constexpr int HIST_BLOCK = 16;
+    // frames before the block staged in LDS beside it (16 KB more: the
+    // register window of lags up to PRE then finds every partner in LDS);
+    // none for smaller blocks, whose two workgroups per CU fill the LDS
+    static constexpr int PRE = WIDE ? 32 : 0;
 };
 
 // Workgroup barrier ordering LDS only.  Waves never exchange data through
@@ -87,6 +91,7 @@ __device__ f32x2_t raw_buffer_load_f32x2(__amdgpu_buffer_rsrc_t rsrc, int voffse
 struct GroupRing {
     __amdgpu_buffer_rsrc_t rs;
     int voff;  // lane * sizeof(float2)
+    int hs;    // frames t0 - hs .. t0 - 1 are staged in LDS too (rows -hs .. -1)
     static constexpr int SLOT_BYTES = RING_GROUP * sizeof(float2);
     // slot + k, k a compile-time constant: k % 8 goes into the instruction's
     // 12-bit offset, the rest into the scalar offset, so runs of loads share
@@ -165,13 +170,13 @@ __device__ __forceinline__ float lag_range_sum(const GroupRing &g, const float2 
     return sum;
 }
 
-// Frame t0 + r of the block's window: LDS for r >= 0, ring history for r < 0.
+// Frame t0 + r of the block's window: LDS for r >= -hs, ring history below.
 // Frames of valid pairs are at most max lag <= R - n back, one wrap away;
 // older ones (read only for discarded lags) are clamped to slot 0 so the
 // load stays inside the ring.
 __device__ __forceinline__ float2 window_frame(const GroupRing &g, const float2 *nl, int R, int slot0, int r) {
     float2 v;
-    if (r >= 0) {
+    if (r >= -g.hs) {
         v = nl[r * RING_GROUP];
     } else {
         int s = slot0 + r;
@@ -225,10 +230,11 @@ __device__ __forceinline__ void dense_sums(const GroupRing &g, const float2 *nl,
     for (int i = 0; i < D; ++i) s[i] = 0.f;
     for (int jb = j0; jb < j1; jb += D) {
         const int cnt = j1 - jb, r0 = jb - tau_a;  // r0: entering partner of step 0
-        // A full block of D steps whose entering partners all sit in LDS, or
+        // A full block of D steps whose entering partners all sit in LDS (the
+        // staged frames before the block included), or
         // all in the ring history without a wrap, reads them from one base
         // with constant offsets; other blocks go frame by frame.
-        if (cnt >= D && r0 >= 0) {
+        if (cnt >= D && r0 >= -g.hs) {
             dense_block<D, K>(LdsRun{nl + r0 * RING_GROUP}, nl + jb * RING_GROUP, win, s);
             continue;
         }
@@ -290,7 +296,7 @@ __device__ __forceinline__ void hist_load(const GroupRing &g, const float2 *nl, 
         // whole block in LDS, or in the history as one run of slots
         int base = slot0 + p0;
         base += (base < 0) ? R : 0;
-        if (p0 >= 0) {
+        if (p0 >= -g.hs) {
 #pragma unroll
             for (int u = 0; u < B; ++u) fp[u] = nl[(p0 + u) * RING_GROUP];
             return;
@@ -527,7 +533,8 @@ __device__ __forceinline__ void lag_body(const float2 *__restrict__ ring, float 
     constexpr int DENSE = C::DENSE, U = C::UNROLL;
     constexpr int SLOTS = (J / 8) < C::SLOTS_MAX ? (J / 8) : C::SLOTS_MAX;  // partials per wave that fit in LDS
     static_assert(LAG_WAVES * SLOTS * RING_GROUP <= J * RING_GROUP * 2, "partials exceed the staging buffer");
-    __shared__ float2 nb[J * RING_GROUP];
+    constexpr int PRE = C::PRE;
+    __shared__ float2 nb[(PRE + J) * RING_GROUP];
     const int tid = threadIdx.x;
     const int lane = tid % RING_GROUP;
     const int w = __builtin_amdgcn_readfirstlane(tid / RING_GROUP);
@@ -535,10 +542,13 @@ __device__ __forceinline__ void lag_body(const float2 *__restrict__ ring, float 
     g.rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<float2 *>(ring + (long long)group * R * RING_GROUP), 0,
                                              R * (int)(RING_GROUP * sizeof(float2)), BUFFER_RSRC_WORD3);
     g.voff = lane * (int)sizeof(float2);
+    g.hs = min(min(PRE, pl.pre), dmax);
     {
-        // stage the new frames: all loads in flight before the LDS stores
+        // stage the new frames (rows 0 .. n - 1) and the hs frames before
+        // them (rows -hs .. -1): all loads in flight before the LDS stores
         constexpr int K = (J + LAG_WAVES - 1) / LAG_WAVES;
-        float2 v[K];
+        constexpr int KP = PRE > 0 ? (PRE + LAG_WAVES - 1) / LAG_WAVES : 1;
+        float2 v[K], vp[KP];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int j = w + k * LAG_WAVES;
@@ -548,10 +558,28 @@ __device__ __forceinline__ void lag_body(const float2 *__restrict__ ring, float 
                 v[k] = g[s];
             }
         }
+        if constexpr (PRE > 0) {
+#pragma unroll
+            for (int k = 0; k < KP; ++k) {
+                const int r = -1 - (w + k * LAG_WAVES);
+                if (r >= -g.hs) {
+                    int s = slot0 + r;
+                    s += (s < 0) ? R : 0;
+                    vp[k] = g[s];
+                }
+            }
+        }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int j = w + k * LAG_WAVES;
-            if (j < n) nb[j * RING_GROUP + lane] = v[k];
+            if (j < n) nb[(PRE + j) * RING_GROUP + lane] = v[k];
+        }
+        if constexpr (PRE > 0) {
+#pragma unroll
+            for (int k = 0; k < KP; ++k) {
+                const int r = -1 - (w + k * LAG_WAVES);
+                if (r >= -g.hs) nb[(PRE + r) * RING_GROUP + lane] = vp[k];
+            }
         }
     }
     // ---- plan (host-computed, wave-uniform) ----
@@ -568,7 +596,7 @@ __device__ __forceinline__ void lag_body(const float2 *__restrict__ ring, float 
     const int nred = hist ? L : (ss ? L : dense_lags);
     AccPre<ACC_PREFETCH, COMP> pre;
     lds_barrier();
-    const float2 *nl = nb + lane;
+    const float2 *nl = nb + PRE * RING_GROUP + lane;
     float s[C::SLOTS_MAX];
 #pragma unroll
     for (int i = 0; i < C::SLOTS_MAX; ++i) s[i] = 0.f;
@@ -752,6 +780,9 @@ static void make_plan(const int *lags, int L, int n, int dmax, LagPlan &pl) {
                 break;
             }
     pl.dense_lags = dense_lags;
+    // the window of the chunk from lag tau_a reads frames down to
+    // tau_a + DENSE - 1 before the first new frame
+    pl.pre = nchunks ? lags[(nchunks - 1) * DENSE] + DENSE - 1 : 0;
     pl.wd = wd;
     pl.sd = nchunks ? wd / nchunks : 1;
     pl.ss = ss;
