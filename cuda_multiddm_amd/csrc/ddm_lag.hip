@@ -77,6 +77,10 @@ constexpr int PLAN_COST_BLOCK = 200, PLAN_COST_VISIT = 10, PLAN_COST_PAIR = 4;
 // COST_DENSE / COST_HIST of a history pair's cost, plus the step's loads
 constexpr int PLAN_COST_STEP = 8;
 constexpr int PLAN_DENSE_PCT = 70;  // scale of a window wave's starting load (measured best)
+// A block's cost as charged to waves 0-3, 4-7, 8-11 and 12-15: the SIMD
+// issues the oldest ready wave first, so the later waves of a SIMD progress
+// more slowly and are given less (measured best on a 64-lag list)
+constexpr int PLAN_WAVE_PCT[4] = {100, 100, 106, 116};
 
 // The R ring slots of one ring group, read with buffer loads: the lane's
 // offset in the vector operand, the wave-uniform slot in the scalar one.  The
@@ -820,7 +824,7 @@ static void make_plan(const int *lags, int L, int n, int dmax, LagPlan &pl) {
         }
         const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
         owner[b] = k;
-        load[k] += cost;
+        load[k] += cost * PLAN_WAVE_PCT[(all ? k : k + wd) / 4] / 100;
     }
     int at = 0;
     for (int k = 0; k < nw; ++k) {
