@@ -91,6 +91,9 @@ typedef float f32x2_t __attribute__((ext_vector_type(2)));
 // lowers to a single-dword load.
 __device__ f32x2_t raw_buffer_load_f32x2(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset, int aux) __asm(
     "llvm.amdgcn.raw.ptr.buffer.load.v2f32");
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+__device__ f32x4_t raw_buffer_load_f32x4(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset, int aux) __asm(
+    "llvm.amdgcn.raw.ptr.buffer.load.v4f32");
 
 struct GroupRing {
     __amdgpu_buffer_rsrc_t rs;
@@ -538,7 +541,7 @@ __device__ __forceinline__ void lag_body(const float2 *__restrict__ ring, float 
     constexpr int SLOTS = (J / 8) < C::SLOTS_MAX ? (J / 8) : C::SLOTS_MAX;  // partials per wave that fit in LDS
     static_assert(LAG_WAVES * SLOTS * RING_GROUP <= J * RING_GROUP * 2, "partials exceed the staging buffer");
     constexpr int PRE = C::PRE;
-    __shared__ float2 nb[(PRE + J) * RING_GROUP];
+    __shared__ __attribute__((aligned(16))) float2 nb[(PRE + J) * RING_GROUP];
     const int tid = threadIdx.x;
     const int lane = tid % RING_GROUP;
     const int w = __builtin_amdgcn_readfirstlane(tid / RING_GROUP);
@@ -547,7 +550,26 @@ __device__ __forceinline__ void lag_body(const float2 *__restrict__ ring, float 
                                              R * (int)(RING_GROUP * sizeof(float2)), BUFFER_RSRC_WORD3);
     g.voff = lane * (int)sizeof(float2);
     g.hs = min(min(PRE, pl.pre), dmax);
-    {
+    const int srows = g.hs + n, sfirst = slot0 - g.hs;  // staged rows -hs .. n - 1 and their first slot
+    if (sfirst >= 0 && sfirst + srows <= R) {
+        // the rows are one run of ring slots (no wrap): copied as 16-byte
+        // pieces, all loads in flight before the LDS stores
+        constexpr int PIECES_MAX = (PRE + J) * RING_GROUP / 2;
+        constexpr int KB = (PIECES_MAX + LAG_NT - 1) / LAG_NT;
+        const int pieces = srows * (RING_GROUP / 2);
+        f32x4_t v[KB];
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+            const int q = tid + k * LAG_NT;
+            if (q < pieces) v[k] = raw_buffer_load_f32x4(g.rs, q * 16, sfirst * GroupRing::SLOT_BYTES, 0);
+        }
+        f32x4_t *dst = reinterpret_cast<f32x4_t *>(nb + (PRE - g.hs) * RING_GROUP);
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+            const int q = tid + k * LAG_NT;
+            if (q < pieces) dst[q] = v[k];
+        }
+    } else {
         // stage the new frames (rows 0 .. n - 1) and the hs frames before
         // them (rows -hs .. -1): all loads in flight before the LDS stores
         constexpr int K = (J + LAG_WAVES - 1) / LAG_WAVES;
