@@ -467,10 +467,11 @@ __device__ __forceinline__ void add_acc(AccRef<COMP> ar, long long E, int l, int
 }
 
 // Partial sums of a launch with history-major lags meet in the staging buffer
-// (the new frames are no longer read) in rounds of 32 of those lags: in round
-// r, history wave k writes its sums of lags 32r + i to rows wd*SL + 32k + i,
+// (the new frames are no longer read) in rounds of RL of those lags: in round
+// r, history wave k writes its sums of lags RL*r + i to rows wd*SL + RL*k + i,
 // and in round 0 dense wave w also writes its window sums to rows w*SL + i.
-// Each lag is then added up by one wave in a fixed order.
+// Each lag is then added up by one wave in a fixed order (the same for any
+// RL, so the results do not depend on it).
 template <int LSM, int SL, int DN, int RL, int SM, int PF, bool COMP>
 __device__ __forceinline__ void hist_reduce(float2 *nb, AccRef<COMP> accg, long long E, const float (&s)[SM],
                                             const float (&h)[LSM], int w, int wd, int hw0, int nhw, int sd,
@@ -673,7 +674,12 @@ __device__ __forceinline__ void lag_body(const float2 *__restrict__ ring, float 
                 const int vtau = lags[dense_lags + (lane < ls ? lane : 0)];
                 hist_sums<C::HIST_BLOCK>(g, nl, R, slot0, n, vtau, ls, pl, pl.all_hist ? w : w - wd, ws, h);
             }
-            if (pl.all_hist)
+            // every wave's sums of all its lags in one round when their rows
+            // fit in the staging buffer, else in rounds of 16 lags
+            if (pl.all_hist && wd * SLOTS + LAG_WAVES * C::HIST_LAGS <= (PRE + J) * 2)
+                hist_reduce<C::HIST_LAGS, SLOTS, DENSE, C::HIST_LAGS>(nb, accg, E, s, h, w, wd, 0, LAG_WAVES, sd,
+                                                                      dense_lags, ls, pre);
+            else if (pl.all_hist)
                 hist_reduce<C::HIST_LAGS, SLOTS, DENSE, 16>(nb, accg, E, s, h, w, wd, 0, LAG_WAVES, sd, dense_lags,
                                                             ls, pre);
             else
