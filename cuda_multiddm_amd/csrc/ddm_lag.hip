@@ -32,7 +32,7 @@ constexpr int DENSE_SEG = 32;                   // frames per segment of a lag's
 // Word 3 of a raw (untyped, stride 0) buffer resource on gfx9-family parts:
 // a valid 32-bit data format, so dword loads return memory.
 constexpr int BUFFER_RSRC_WORD3 = 0x00020000;
-constexpr int DENSE_BATCH = 4;                  // register-window steps scheduled together
+constexpr int DENSE_BATCH = 2;                  // register-window steps scheduled together
 constexpr int ACC_PREFETCH = 4;                 // accumulators each wave requests ahead of its sums
 
 // Per block size: J = 256 stages 128 KB of new frames, so one workgroup (4
