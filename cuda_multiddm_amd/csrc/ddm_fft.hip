@@ -673,9 +673,11 @@ struct BigCfg {
     static constexpr int NTH = 256;                // 4 waves
     static constexpr int SEQ = NTH / L;            // sequences (rows pairs / columns) per workgroup
     // column pass: 16 columns per workgroup, so that its xt reads and ring
-    // writes move 128-byte row segments (8 waves for S = 1024, 131 KB of LDS)
-    static constexpr int NTH_C = 16 * L;
-    static constexpr int SEQ_C = 16;
+    // writes move 128-byte row segments; 8 for S = 1024 (4 waves, 66 KB of
+    // LDS: two workgroups per CU, whose phases overlap), two workgroups
+    // reading the halves of each 16-column block of the pass buffer
+    static constexpr int SEQ_C = S >= 1024 ? 8 : 16;
+    static constexpr int NTH_C = SEQ_C * L;
     static_assert(E1 == 16 || E1 == 32, "phase 1 is radix 16 or 32");
 };
 
@@ -805,10 +807,24 @@ __device__ __forceinline__ void tile_fft_cols_reg_body(const RingDst &rd, int ti
     // then start, end and band of the Nyquist column
     __shared__ int ctab[2 * SEQ + 4];
     const int tid = threadIdx.x;
-    const int nf = nblk / (NB * tiles);  // frames vary fastest
-    const int f = bid % nf;
-    const int cb = (bid / nf) % NB;
-    const int tile = bid / (nf * NB);
+    int f, cb, tile;
+    if constexpr (SEQ < 16) {
+        // columns c0 and c0 + 8 share a 16-column pass-buffer block: they go
+        // to workgroups b and b + 8, which land on the same XCD (workgroups
+        // are dealt to the 8 XCDs in turn) and so share its L2 (nblk is a
+        // multiple of 16)
+        static_assert(2 * SEQ == 16, "two workgroups per 16-column block");
+        const int pair = (bid >> 4) * 8 + (bid & 7), sib = (bid >> 3) & 1;
+        const int nf = nblk / (NB * tiles);  // frames vary fastest
+        f = pair % nf;
+        cb = 2 * ((pair / nf) % (NB / 2)) + sib;
+        tile = pair / (nf * (NB / 2));
+    } else {
+        const int nf = nblk / (NB * tiles);  // frames vary fastest
+        f = bid % nf;
+        cb = (bid / nf) % NB;
+        tile = bid / (nf * NB);
+    }
     const int c0 = cb * SEQ;
     // the same for the whole workgroup (block 0 also holds the Nyquist column)
     if (c0 >= rd.lay.pass_cols) return;
@@ -1023,7 +1039,8 @@ bool fft_batch_jobs(const PixSrc &ps, const RingDst &rd, int scale, int tps, int
         j.groups = 0;
         j.nblk = (h / seq) * tiles * n_frames;
         out->push_back({FftKey{FFT_ROWS, scale, pix}, j});
-        j.nblk = (h / BigCfg<256>::SEQ_C) * tiles * n_frames;
+        const int seqc = scale >= 1024 ? BigCfg<1024>::SEQ_C : BigCfg<256>::SEQ_C;
+        j.nblk = (h / seqc) * tiles * n_frames;
         out->push_back({FftKey{FFT_COLS, scale, 0}, j});
         return true;
     }
