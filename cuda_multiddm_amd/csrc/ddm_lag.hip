@@ -5,8 +5,10 @@
 // (no 1/s^2 factor: it is a power of two, applied exactly in the average).
 //
 // Mapping: one workgroup per ring group (RING_GROUP = 64 consecutive
-// spectrum elements, one per lane), 16 waves.  The block's new frames are
-// staged once in LDS; the group's ring history is read with buffer loads.
+// spectrum elements, one per lane), 16 waves.  The block's new frames (and,
+// for 256-frame blocks, the frames before it that the register window
+// reaches) are staged once in LDS; the rest of the group's ring history is
+// read with buffer loads.
 // Three ways to form the pairs (new frame j, frame j - tau):
 //  * the leading run of consecutive lags (1, 2, 3, ... in the usual lag
 //    lists): a register window, one new frame and one entering partner per
@@ -58,7 +60,8 @@ struct LagCfg {
 
 // Workgroup barrier ordering LDS only.  Waves never exchange data through
 // global memory here, so loads still in flight (the accumulator reads issued
-// before the partial sums are exchanged) need not complete before it, as they would under __syncthreads().
+// before the partial sums are exchanged) need not complete before it, as they
+// would under __syncthreads().
 __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_s_barrier();
