@@ -673,10 +673,11 @@ struct BigCfg {
     static constexpr int NTH = 256;                // 4 waves
     static constexpr int SEQ = NTH / L;            // sequences (rows pairs / columns) per workgroup
     // column pass: 16 columns per workgroup, so that its xt reads and ring
-    // writes move 128-byte row segments; 8 for S = 1024 (4 waves, 66 KB of
-    // LDS: two workgroups per CU, whose phases overlap), two workgroups
-    // reading the halves of each 16-column block of the pass buffer
-    static constexpr int SEQ_C = S >= 1024 ? 8 : 16;
+    // writes move 128-byte row segments; 8 for S >= 512 (66 KB of LDS for
+    // S = 1024: two workgroups per CU, whose phases overlap, instead of one;
+    // four for S = 512), two workgroups reading the halves of each 16-column
+    // block of the pass buffer
+    static constexpr int SEQ_C = S >= 512 ? 8 : 16;
     static constexpr int NTH_C = SEQ_C * L;
     static_assert(E1 == 16 || E1 == 32, "phase 1 is radix 16 or 32");
 };
@@ -1039,7 +1040,7 @@ bool fft_batch_jobs(const PixSrc &ps, const RingDst &rd, int scale, int tps, int
         j.groups = 0;
         j.nblk = (h / seq) * tiles * n_frames;
         out->push_back({FftKey{FFT_ROWS, scale, pix}, j});
-        const int seqc = scale >= 1024 ? BigCfg<1024>::SEQ_C : BigCfg<256>::SEQ_C;
+        const int seqc = scale >= 512 ? BigCfg<512>::SEQ_C : BigCfg<256>::SEQ_C;
         j.nblk = (h / seqc) * tiles * n_frames;
         out->push_back({FftKey{FFT_COLS, scale, 0}, j});
         return true;
