@@ -43,7 +43,7 @@ def _check_files(prefix, cfg, ref, fps=1.0):
             assert np.max(np.abs(isf[nz] - want[nz]) / want[nz]) <= PRINT_TOL, (s, t)
 
 
-def test_cli_benchmark_mode_config1(gpu, tmp_path):
+def test_cli_benchmark_mode_uniform128(gpu, tmp_path):
     """-B: 64 frames of 128^2 uniform random bytes, tau 1..16, 20 lambdas 4..64, -t 20, -C 30."""
     lams = ddm.lin_lambdas(4, 64, 20)
     out = _run("-B", "-N", 64, "-C", 30, "-t", 20, "-o", tmp_path / "out", *_params(tmp_path, (128,), range(1, 17), lams))

@@ -34,13 +34,13 @@ def _both(frames, scales, lags, lambdas, tol=1.2, chunk=30, bpp=1, pixfmt=ddm.PI
     return got, ref
 
 
-def test_config1_uniform_128(gpu):
+def test_uniform_128_whole_pipeline(gpu):
     """64 frames 128^2 uint8 uniform, scale 128, tau 1..16, 20 lambdas 4..64, tol 20%, chunk 30."""
     frames = ddm.synth_uniform(64, 128, 128, 8, seed=1)
     got, ref = _both(frames, [128], list(range(1, 17)), ddm.lin_lambdas(4, 64, 20), 1.2, 30)
     assert got.shape == ref.shape
     err = o.max_rel_err(got, ref)
-    print("config1 max rel err", err)
+    print("uniform 128^2 max rel err", err)
     assert err <= TOL
 
 

@@ -42,7 +42,7 @@ def _kept(s, lambdas, tol):
     return int(kept_mask(s, lambdas, tol).sum())
 
 
-def test_validate_accepts_config1_and_sorts():
+def test_validate_accepts_uniform128_params_and_sorts():
     rc, msg, info = _v(spectrum=ddm.SPECTRUM_FULL)
     assert rc == 0 and msg == ""
     # ring: one chunk + one lag block (256 by default) + the largest lag

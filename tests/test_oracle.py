@@ -197,8 +197,8 @@ def test_tiling_reference_order():
     cfgb = o.Config(w, h, 2, [16, 8], [1], [4.0], bpp=2, pixfmt=o.PIX_U16BE, x_off=3, y_off=2)
     assert np.array_equal(o.tile(cfgb, be.view(np.uint8), 8, 3, 1), t)
     rgb = rng.integers(0, 256, (1, h, w, 3), dtype=np.uint8)
-    cfg3 = o.Config(w, h, 1, [8], [1], [4.0], bpp=3, x_off=5, y_off=4)
-    assert np.array_equal(o.tile(cfg3, rgb, 8, 0, 0), rgb[0, 4:12, 5:13, 0].astype(float))
+    cfg_rgb = o.Config(w, h, 1, [8], [1], [4.0], bpp=3, x_off=5, y_off=4)
+    assert np.array_equal(o.tile(cfg_rgb, rgb, 8, 0, 0), rgb[0, 4:12, 5:13, 0].astype(float))
 
 
 def test_fp32_baseline_close_to_oracle():
