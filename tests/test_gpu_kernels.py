@@ -291,6 +291,45 @@ def _check_run_history(gpu, run, nhist, step, t0):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("seed", range(24))
+def test_lag_engine_random_lists_bit_exact(gpu, seed):
+    """Random lag lists (a run of consecutive lags of random length and start,
+    then spread lags), block sizes, block starts and ring sizes, so that the
+    staged frames before the block, the 16-byte staging of one run of ring
+    slots and the per-frame staging across the ring's wrap, the register
+    window and the history-major blocks meet in combinations the fixed cases
+    do not pick; exact integer sums."""
+    import torch
+    rng = np.random.default_rng(7000 + seed)
+    J = int(rng.choice([32, 64, 128, 256, 256, 256]))
+    run0 = int(rng.integers(0, 3))
+    run = int(rng.integers(0, 40))
+    lags = list(range(run0, run0 + run))
+    nspread = int(rng.integers(1, 34))
+    last = lags[-1] if lags else run0
+    for _ in range(nspread):
+        last += int(rng.integers(2, 60))
+        lags.append(last)
+    lags = sorted(set(lags))
+    n_new = int(rng.integers(1, J + 1)) if rng.random() < 0.4 else J
+    R = max(lags) + n_new + int(rng.integers(0, 50))
+    t0 = int(rng.integers(0, 3 * R))
+    E = int(rng.integers(1, 5)) * G
+    re = rng.integers(-8, 9, (R, E))
+    im = rng.integers(-8, 9, (R, E))
+    ring_np = re + 1j * im
+    ring = torch.from_numpy(frames_to_ring(re, im)).to(gpu)
+    acc0 = rng.integers(0, 1 << 16, (len(lags), E))
+    acc = torch.from_numpy(acc0.astype(np.float32).reshape(-1)).to(gpu)
+    d_lags = torch.tensor(lags, dtype=torch.int32, device=gpu)
+    print(f"random lag seed={seed} J={J} t0={t0} n_new={n_new} R={R} L={len(lags)} run={run}@{run0}", flush=True)
+    ddm.k_lag_accumulate(ring, R, E, t0, n_new, d_lags, max(lags), acc, J)
+    torch.cuda.synchronize()
+    got = acc.cpu().numpy().reshape(len(lags), E).astype(np.int64)
+    want = acc0 + _lag_expected(ring_np, R, t0, n_new, lags)
+    assert np.array_equal(got, want)
+
+
 def test_stage_entry_points_reject_bad_arguments(gpu):
     import torch
     ring = torch.zeros(16, dtype=torch.float32, device=gpu)
