@@ -331,7 +331,8 @@ __device__ __forceinline__ void hist_block(const float2 *nl, const float2 (&fp)[
     for (int l = la; l < lb; ++l) {
         const int j0 = p0 + __builtin_amdgcn_readlane(vtau, l);
         const float2 *a = nl + j0 * RING_GROUP;
-        float t[4] = {0.f, 0.f, 0.f, 0.f};
+        // two partial sums: chains of 16 FMAs, the fewest adds to fold them
+        float t[2] = {0.f, 0.f};
         if (j0 >= 0 && j0 + B <= n) {
             // all B rows requested before the first pair: the reads overlap
             // (the compiler otherwise keeps a few in flight to save registers)
@@ -342,8 +343,8 @@ __device__ __forceinline__ void hist_block(const float2 *nl, const float2 (&fp)[
             for (int u = 0; u < B; ++u) {
                 const float2 v = va[u];
                 const float dx = v.x - fp[u].x, dy = v.y - fp[u].y;
-                t[u & 3] = fmaf(dx, dx, t[u & 3]);
-                t[u & 3] = fmaf(dy, dy, t[u & 3]);
+                t[u & 1] = fmaf(dx, dx, t[u & 1]);
+                t[u & 1] = fmaf(dy, dy, t[u & 1]);
             }
         } else {
             const int ua = -j0, ub = n - j0;  // valid u: ua <= u < ub
@@ -352,12 +353,12 @@ __device__ __forceinline__ void hist_block(const float2 *nl, const float2 (&fp)[
                 if (u >= ua && u < ub) {
                     const float2 v = a[u * RING_GROUP];
                     const float dx = v.x - fp[u].x, dy = v.y - fp[u].y;
-                    t[u & 3] = fmaf(dx, dx, t[u & 3]);
-                    t[u & 3] = fmaf(dy, dy, t[u & 3]);
+                    t[u & 1] = fmaf(dx, dx, t[u & 1]);
+                    t[u & 1] = fmaf(dy, dy, t[u & 1]);
                 }
             }
         }
-        s[l] += (t[0] + t[1]) + (t[2] + t[3]);
+        s[l] += t[0] + t[1];
     }
 }
 
